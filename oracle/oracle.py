@@ -1,0 +1,109 @@
+"""ctypes view of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker for the HIP engine.  The product package never imports this file.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+STAT_KEYS = ("status", "c_loop_iters", "outer_iters", "r_loop_calls", "s_loop_calls",
+             "saddle_loop_calls", "c_loops_calls", "edited", "n_max", "n_min",
+             "n_saddle1", "n_saddle2")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = ctypes.CDLL(_SO)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        L.oracle_lut.argtypes = [vp, i32]
+        L.oracle_link_type.argtypes = [i32, i32]
+        L.oracle_classify.argtypes = [vp, i32, i32, i32, vp, vp]
+        L.oracle_labels.argtypes = [vp, i32, i32, i32, vp, vp]
+        L.oracle_max_label.argtypes = [vp, i32, i32, i32, i32]
+        L.oracle_min_label.argtypes = [vp, i32, i32, i32, i32]
+        L.oracle_correct.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, vp, vp, i32]
+        L.oracle_bitmap3.argtypes = [vp, i64, vp]
+        L.oracle_bitmap3.restype = i64
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dims(f):
+    D, H, W = f.shape
+    return W, H, D
+
+
+def lut():
+    out = np.zeros(24800, np.int32)
+    n = lib().oracle_lut(_p(out), out.size)
+    return out[:n]
+
+
+def link_type(key, config):
+    return lib().oracle_link_type(key, config)
+
+
+def classify(field):
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    t = np.empty(f.shape, np.uint8)
+    m = np.empty(f.shape, np.uint16)
+    lib().oracle_classify(_p(f), *_dims(f), _p(t), _p(m))
+    return t, m
+
+
+def labels(field):
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    ds = np.empty(f.shape, np.int32)
+    asc = np.empty(f.shape, np.int32)
+    lib().oracle_labels(_p(f), *_dims(f), _p(ds), _p(asc))
+    return ds, asc
+
+
+def max_label(field, v):
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    return lib().oracle_max_label(_p(f), *_dims(f), int(v))
+
+
+def min_label(field, v):
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    return lib().oracle_min_label(_p(f), *_dims(f), int(v))
+
+
+def correct(orig, decp, xi, max_c_iters=0, max_outer=0, threads=0):
+    """Returns (fixed, counts, stats dict).  Inputs are not modified."""
+    o = np.ascontiguousarray(orig, dtype=np.float64)
+    d = np.array(decp, dtype=np.float64, order="C", copy=True)
+    counts = np.zeros(o.shape, np.uint8)
+    st = np.zeros(12, np.int64)
+    lib().oracle_correct(_p(o), _p(d), *_dims(o), float(xi), int(max_c_iters),
+                         int(max_outer), _p(counts), _p(st), int(threads))
+    return d, counts, dict(zip(STAT_KEYS, (int(x) for x in st)))
+
+
+def bitmap3(vals):
+    v = np.ascontiguousarray(vals, dtype=np.uint8).ravel()
+    out = np.zeros((3 * v.size + 7) // 8 + 1, np.uint8)
+    n = lib().oracle_bitmap3(_p(v), v.size, _p(out))
+    if n < 0:
+        raise ValueError("value outside 0..6")
+    return out[:n]
