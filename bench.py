@@ -1,0 +1,156 @@
+"""Benchmark: corrected voxels/s of the ExtreM correction on one MI355X.
+
+Workload (default): one float32 field 256x256x256, sum of 32 anisotropic
+Gaussians + 1e-3*range uniform noise (seed 2), xi = 1e-4 * range, decp the
+uniform quantiser 2*xi*rint(orig/(2*xi)).  A step is one full correction call
+(main.cpp:65-446 semantics) on device-resident inputs; each step gets its own
+pristine copy of decp, prepared before the timed region.
+
+Prints one JSON line.  roofline is for the kernel family with the largest
+summed time (HIP events on the engine's stream); cpu_baseline is the CPU
+oracle on a bounded sample of the same workload recipe.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per voxel for one launch of each full-grid family.
+BYTES_PER_VOXEL = {
+    "classify": 8 + 1 + 2,        # read f64 field, write u8 type + u16 mask
+    "check_full": 8 + 2,          # read f64 decp + u16 original mask
+    "steepest": 8 + 4 + 4,        # read f64 field, write two i32 pointers
+    "jump": 4 * 4,                # read + write both i32 label arrays
+}
+
+
+def workload(name):
+    from extrem_z_amd import synth
+    if name == "256":
+        o, d, xi = synth.make_case("gauss", (256, 256, 256), seed=2, er=1e-4, n_gauss=32)
+        desc = "gauss32-256^3-er1e-4"
+    elif name == "512":
+        o, d, xi = synth.make_case("grf", (512, 512, 512), seed=3, xi=1e-3)
+        desc = "grf-k^-3-512^3-xi1e-3"
+    elif name == "64":
+        o, d, xi = synth.make_case("gauss", (64, 64, 64), seed=1, er=1e-3, n_gauss=8)
+        desc = "gauss8-64^3-er1e-3"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return o, d, xi, desc
+
+
+def cpu_baseline(seconds_budget=20.0, threads=16):
+    """Oracle on a bounded sample: a 96^3 field built by the same recipe."""
+    from extrem_z_amd import synth
+    from oracle import oracle
+    o, d, xi = synth.make_case("gauss", (96, 96, 96), seed=2, er=1e-4, n_gauss=2)
+    t = time.perf_counter()
+    f, c, st = oracle.correct(o, d, xi, threads=threads)
+    dt = time.perf_counter() - t
+    n = o.size
+    return {"value": n / dt, "unit": "voxels/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ restatement, OpenMP {threads} threads) on one 96^3 field, "
+                      f"2 Gaussians + 1e-3 noise, er=1e-4: {dt:.2f} s, "
+                      f"{st['outer_iters']} outer iterations"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="256")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the multi-GPU path is not part of this build "
+              "(fields and jobs shard whole per GPU; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import extrem_z_amd as ez
+
+    orig, decp, xi, desc = workload(args.workload)
+    D, H, W = orig.shape
+    N = orig.size
+    print(f"bench: workload {desc}, N={N}, steps={args.steps}, warmup={args.warmup}",
+          file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    d_orig = torch.from_numpy(orig.ravel()).to(dev)
+    copies = [torch.from_numpy(decp.ravel()).to(dev) for _ in range(args.steps + args.warmup)]
+    d_counts = torch.zeros(N, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    ctx = ez.Context(0)
+    for i in range(args.warmup):
+        st = ctx.correct_device(d_orig.data_ptr(), copies[i].data_ptr(), (W, H, D), xi,
+                                d_counts.data_ptr())
+        print(f"bench: warmup {i} done, {st['wall_ms']:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = ctx.correct_device(d_orig.data_ptr(), copies[args.warmup + i].data_ptr(), (W, H, D),
+                                xi, d_counts.data_ptr(), timing=True)
+        stats.append(st)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    for st in stats:
+        assert st["status"] == 0, st
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = N * args.steps / elapsed
+
+    # roofline of the dominant full-grid kernel family
+    fam_ms = {k: sum(s["kernel_ms"][k] for s in stats) for k in stats[0]["kernel_ms"]}
+    fam_n = {k: sum(s["kernel_launches"][k] for s in stats) for k in stats[0]["kernel_launches"]}
+    ranked = sorted((k for k in BYTES_PER_VOXEL if fam_n[k]), key=lambda k: -fam_ms[k])
+    dom = ranked[0]
+    avg_ms = fam_ms[dom] / fam_n[dom]
+    bytes_launch = BYTES_PER_VOXEL[dom] * N
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": dom, "avg_launch_ms": round(avg_ms, 4),
+                "bytes_per_launch": bytes_launch}
+    out = {
+        "metric": "corrected voxels/s (byte-exact vs CPU oracle)",
+        "value": value,
+        "unit": "voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "shape": [W, H, D], "xi": xi,
+                   "outer_iters": stats[-1]["outer_iters"],
+                   "c_loop_iters": stats[-1]["c_loop_iters"],
+                   "edited": stats[-1]["edited"],
+                   "host_syncs": stats[-1]["host_syncs"],
+                   "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in fam_ms.items()},
+                   "launches_per_step": {k: v // args.steps for k, v in fam_n.items()}},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        print("bench: cpu baseline (oracle on a bounded sample)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(threads=args.cpu_threads)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

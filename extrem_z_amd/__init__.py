@@ -1,0 +1,212 @@
+"""MI355X ExtreM correction engine: Python view of the C ABI (include/extremz.h).
+
+The engine is libextremz.so (HIP kernels for gfx950 behind a C ABI).  This
+module only loads it and converts numpy arrays; there is no CPU fallback: a
+missing library or device raises.
+
+Functions mirror the reference's correction path:
+  correct()        the main.cpp:65-446 correction call (decp -> fixed)
+  classify()       classifyAllVertices (src/ExtreM.cpp:1088-1097)
+  labels()         ComputeDescending/AscendingManifold (src/ExtreM.cpp:144-183, 502-541)
+  encode()         cost() edit streams (src/ExtreM.cpp:272-389)
+  getdata()        the raw float64 reader + bound of getdata (src/ExtreM.cpp:95-138)
+  saveArrayToBin() the raw float64 writer (src/ExtreM.cpp:2918-2928)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libextremz.so")
+
+STAT_KEYS = ("status", "c_loop_iters", "outer_iters", "r_loop_calls", "s_loop_calls",
+             "saddle_loop_calls", "c_loops_calls", "edited", "n_max", "n_min", "n_saddle1",
+             "n_saddle2", "label_passes", "jump_rounds", "host_syncs", "frontier_items")
+KERNEL_FAMILIES = ("classify", "check_full", "check_incr", "steepest", "jump", "saddle",
+                   "propose", "apply")
+
+ERRORS = {-3: "iteration cap reached", -5: "HIP runtime error", -12: "device allocation failed",
+          -19: "no gfx950 device", -22: "invalid argument"}
+
+
+class ExtremzError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__(f"extremz error {code} ({ERRORS.get(code, '?')}): {msg}")
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("max_c_iters", ctypes.c_int), ("max_outer", ctypes.c_int),
+                ("timing", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("counts", ctypes.c_int64 * 16), ("kernel_ms", ctypes.c_double * 8),
+                ("kernel_launches", ctypes.c_int64 * 8), ("wall_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        d = {k: int(self.counts[i]) for i, k in enumerate(STAT_KEYS)}
+        d["kernel_ms"] = {k: float(self.kernel_ms[i]) for i, k in enumerate(KERNEL_FAMILIES)}
+        d["kernel_launches"] = {k: int(self.kernel_launches[i])
+                                for i, k in enumerate(KERNEL_FAMILIES)}
+        d["wall_ms"] = float(self.wall_ms)
+        return d
+
+
+_lib = None
+
+
+def lib():
+    """Loads libextremz.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ExtremzError(-19, f"{LIB_PATH} is missing; run extrem_z_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        L.extremz_ctx_create.argtypes = [i32, ctypes.POINTER(vp)]
+        L.extremz_ctx_destroy.argtypes = [vp]
+        L.extremz_correct.argtypes = [vp, i32, vp, i32, i32, i32, dbl, vp, vp, vp]
+        L.extremz_correct_device.argtypes = [vp, vp, vp, i32, i32, i32, dbl, vp, vp, vp]
+        L.extremz_classify.argtypes = [vp, i32, i32, i32, vp, vp]
+        L.extremz_labels.argtypes = [vp, i32, i32, i32, vp, vp]
+        L.extremz_lut.argtypes = [vp, i32]
+        L.extremz_encode.argtypes = [vp, vp, vp, vp, i64, dbl, vp, vp, vp, vp, vp]
+        L.extremz_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise ExtremzError(rc, lib().extremz_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dims(a):
+    if a.ndim != 3:
+        raise ValueError("fields are 3-D arrays shaped (D, H, W)")
+    D, H, W = a.shape
+    return W, H, D
+
+
+def lut():
+    """The classification table in LUT.bin layout (24 800 int32)."""
+    out = np.zeros(24800, np.int32)
+    n = lib().extremz_lut(_p(out), out.size)
+    return out[:n]
+
+
+def classify(field):
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    t = np.empty(f.shape, np.uint8)
+    m = np.empty(f.shape, np.uint16)
+    _check(lib().extremz_classify(_p(f), *_dims(f), _p(t), _p(m)))
+    return t, m
+
+
+def labels(field):
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    ds = np.empty(f.shape, np.int32)
+    asc = np.empty(f.shape, np.int32)
+    _check(lib().extremz_labels(_p(f), *_dims(f), _p(ds), _p(asc)))
+    return ds, asc
+
+
+def correct(orig, decp, xi, max_c_iters=0, max_outer=0, timing=False):
+    """Corrects decp against orig within xi.  Returns (fixed, counts, stats).
+
+    orig may be float32 or float64 (promoted exactly); inputs are not modified.
+    """
+    o = np.ascontiguousarray(orig)
+    if o.dtype == np.float32:
+        dt = 0
+    else:
+        o = np.ascontiguousarray(o, dtype=np.float64)
+        dt = 1
+    d = np.array(decp, dtype=np.float64, order="C", copy=True)
+    if d.shape != o.shape:
+        raise ValueError("orig and decp shapes differ")
+    counts = np.zeros(o.shape, np.uint8)
+    st = Stats()
+    opts = Opts(int(max_c_iters), int(max_outer), int(bool(timing)), 0)
+    rc = lib().extremz_correct(_p(o), dt, _p(d), *_dims(o), float(xi), ctypes.byref(opts),
+                               ctypes.byref(st), _p(counts))
+    _check(rc)
+    return d, counts, st.as_dict()
+
+
+class Context:
+    """A device context: one HIP stream plus work buffers (extremz_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().extremz_ctx_create(int(device), ctypes.byref(h)))
+        self.handle = h
+
+    def correct_device(self, d_orig, d_decp, shape, xi, d_counts=0, max_c_iters=0, max_outer=0,
+                       timing=False):
+        """Device pointers (ints): d_orig float64, d_decp float64 edited in place."""
+        W, H, D = shape
+        st = Stats()
+        opts = Opts(int(max_c_iters), int(max_outer), int(bool(timing)), 0)
+        rc = lib().extremz_correct_device(self.handle, ctypes.c_void_p(d_orig),
+                                          ctypes.c_void_p(d_decp), W, H, D, float(xi),
+                                          ctypes.byref(opts), ctypes.byref(st),
+                                          ctypes.c_void_p(d_counts or None))
+        _check(rc)
+        return st.as_dict()
+
+    def close(self):
+        if self.handle:
+            lib().extremz_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def encode(orig, decp_before, decp_after, counts, xi):
+    """cost() streams: (index_bitmap uint8, edits f64, mantissas u64, exponents i32)."""
+    o = np.ascontiguousarray(orig, dtype=np.float64).ravel()
+    b = np.ascontiguousarray(decp_before, dtype=np.float64).ravel()
+    a = np.ascontiguousarray(decp_after, dtype=np.float64).ravel()
+    c = np.ascontiguousarray(counts, dtype=np.uint8).ravel()
+    n = o.size
+    bm = np.zeros((3 * n + 7) // 8, np.uint8)
+    edits = np.zeros(n, np.float64)
+    mant = np.zeros(n, np.uint64)
+    expo = np.zeros(n, np.int32)
+    ne = ctypes.c_int64(0)
+    _check(lib().extremz_encode(_p(o), _p(b), _p(a), _p(c), n, float(xi), _p(bm), _p(edits),
+                                _p(mant), _p(expo), ctypes.byref(ne)))
+    k = ne.value
+    return bm, edits[:k].copy(), mant[:k].copy(), expo[:k].copy()
+
+
+def getdata(filename, shape, er):
+    """Raw float64 field + bound (max-min)*er, getdata of src/ExtreM.cpp:95-138.
+
+    shape is (W, H, D); a size mismatch raises (the reference prints and
+    returns, src/ExtreM.cpp:108-111).
+    """
+    W, H, D = shape
+    a = np.fromfile(filename, dtype="<f8")
+    if a.size != W * H * D:
+        raise ValueError("File size does not match expected data size.")
+    a = a.reshape(D, H, W)
+    return a, float((a.max() - a.min()) * er)
+
+
+def saveArrayToBin(arr, filename):
+    """Raw little-endian float64, no header (src/ExtreM.cpp:2918-2928)."""
+    np.ascontiguousarray(arr, dtype="<f8").tofile(filename)

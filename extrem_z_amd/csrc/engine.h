@@ -1,0 +1,137 @@
+// engine.h -- device state shared by the kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace ez {
+
+constexpr int kNbr = 14;
+constexpr int kNone = -1;
+
+// Device counters (one int array, read back with one copy).
+enum Ctr {
+  C_TOUCHED = 0,  // touched list length (vertices with a pending delta)
+  C_DIRTY,        // dirty list length (decp changed since the last check)
+  C_FRONT_A,      // frontier buffers
+  C_FRONT_B,
+  C_CHANGED,      // pointer jumping moved something
+  C_ACT_A,        // greedy active lists
+  C_ACT_B,
+  C_NFC,          // number_of_false_cases
+  C_NFC1,         // number_of_false_cases1
+  C_WMC,          // wrong_max_counter
+  C_WMC2,         // wrong_max_counter_2
+  C_WMINC,        // wrong_min_counter
+  C_WMINC2,       // wrong_min_counter_2
+  C_WSC,          // wrong_saddle_counter
+  C_WSCJ,         // wrong_saddle_counter_join
+  C_GMIN,         // original global min / max (pre-snap)
+  C_GMAX,
+  C_DGMAX,        // global max / min of the CP lists under decp
+  C_DGMIN,
+  C_N2,           // list lengths
+  C_N1,
+  C_NMAX,
+  C_NMIN,
+  C_ANY_CHANGE,   // decp changed since the last CP computation
+  C_FAIL,
+  C_EDITED,
+  C_COUNT_
+};
+
+struct Geo {
+  int W, H, D, WH, N;
+  int off[kNbr];  // index offset of each stencil direction
+};
+
+// Everything a kernel needs, passed by value.
+struct St {
+  Geo g;
+  double xi;
+  long long sent;       // ordered key of the -4*xi sentinel
+  const uint8_t* lut;
+  const double* orig;
+  double* decp;
+  long long* delta;     // ordered-int64 delta buffer (d_deltaBuffer)
+  uint8_t* cnt;         // delta_counter
+  uint8_t* otype;       // vertex_type
+  uint16_t* olow;       // original lower-star masks
+  int *oDS, *oAS, *dDS, *dAS;
+  int* stamp;
+  uint8_t* dflag;
+  int *touched, *dirty;
+  int* ctr;
+  // CP lists (original types; equal to the decp types wherever used)
+  int *S2, *S1, *MX, *MN;
+  int n2, n1, nmax, nmin;
+  int *ormap2, *ormap1;           // 14 original roots per saddle
+  int *of2, *ol2, *of1, *ol1;     // original triplet first / last
+  int *df2, *dl2, *df1, *dl1;     // decp triplet first / last
+  int *olarg, *osmall, *dlarg, *dsmall;  // per extremum (vertex-indexed)
+  uint16_t* cmask;                // greedy candidate masks per saddle
+  uint8_t *wn2, *wn1;             // wrong_neighbors(_ds) flags
+  int *wnl2, *wnl1;               // wrong_neighbors(_ds)_index
+  int *minsad2, *minsad1;         // greedy reservations
+  int *win_a, *win_b;             // first-saddle-wins reservations
+};
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& m) : std::runtime_error(m) {}
+};
+
+inline void hip_check(hipError_t e, const char* what = "hip") {
+  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void upload_constants(const int* dx, const int* dy, const int* dz, const int* cls_key,
+                      const int* lut_off);
+
+// Kernels (kernels.hip).
+__global__ void k_fill_i64(long long* p, long long v, long long n);
+__global__ void k_fill_i32(int* p, int v, long long n);
+__global__ void k_ctr_set(int* ctr, int idx, int v);
+__global__ void k_ctr_set2(int* ctr, int a, int b, int v);
+__global__ void k_f32_to_f64(const float* in, double* out, long long n);
+__global__ void k_check_finite(const double* a, long long n, int* ctr);
+__global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* type,
+                           uint16_t* lower);
+__global__ void k_check_full(St s, int* front, int fctr, int epoch);
+__global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int* front,
+                             int fctr, int epoch);
+__global__ void k_clear_dirty(St s);
+__global__ void k_propose_c(St s, const int* front, int fctr);
+__global__ void k_apply(St s);
+__global__ void k_steepest(Geo g, const double* f, int* ds, int* as);
+__global__ void k_jump(int n, int* ds, int* as, int* ctr);
+__global__ void k_list_best(const int* list, int n, const double* f, int* slot, int want_max);
+__global__ void k_list_fill(const int* list, int n, int* arr, int val);
+__global__ void k_saddles(St s, int side, const int* list, int n, const double* f,
+                          const int* L, int* first, int* last, int* slots, int glob_idx,
+                          int* ormap);
+__global__ void k_wn_cand(St s, int n2, int n1, int* active, int actr);
+__global__ void k_wn_register(St s, int n2, const int* active, int actr);
+__global__ void k_wn_commit(St s, int n2, const int* active, int actr);
+__global__ void k_wn_next(St s, int n2, const int* active, int actr, int* next, int nctr);
+__global__ void k_wn_reset(St s);
+__global__ void k_fixpath(St s);
+__global__ void k_wrong_index(St s, int side, int phase, int do_fix);
+__global__ void k_wrong_saddle(St s, int side, int do_fix);
+__global__ void k_lists(St s);
+__global__ void k_presnap(St s);
+__global__ void k_argext(const double* f, int n, int* slot_min, int* slot_max);
+__global__ void k_count_edited(const uint8_t* cnt, long long n, int* ctr);
+__global__ void k_bitmap3(const double* before, const double* after, const uint8_t* cnt,
+                          long long n, uint8_t* out, long long nbytes);
+__global__ void k_edit_flags(const double* before, const double* after, const uint8_t* cnt,
+                             long long n, int* flag);
+__global__ void k_edit_write(const double* orig, const double* before, const int* pos,
+                             const int* flag, long long n, double xi, double* edits,
+                             unsigned long long* mant, int* expo);
+__global__ void k_scan_block(const int* in, int* out, long long n, int* block_sums);
+__global__ void k_scan_add(int* out, long long n, const int* offs);
+
+}  // namespace ez

@@ -1,0 +1,944 @@
+// engine.hip -- host side of the correction engine and its C ABI.
+//
+// The control flow restates main.cpp:65-446 step by step; every step is a
+// sequence of kernel launches on the context stream.  The host reads the
+// small counter array back only where main.cpp branches on a counter.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/extremz.h"
+#include "engine.h"
+
+namespace ez {
+
+// ------------------------------------------------------------------ LUT ----
+// 14-neighbour Freudenthal stencil, src/ExtreM_cuda.cu:155-162.
+static const int kDx[kNbr] = {1, -1, 0, 0, 0, 0, -1, 1, 0, 0, -1, 1, -1, 1};
+static const int kDy[kNbr] = {0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1};
+static const int kDz[kNbr] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1};
+// Segment order of LUT.bin, src/ExtreM_cuda.cu:84-91.
+static const int kKeys[27] = {277,  153,   1050,  2085,  102,   554,  413,  1179, 2357,
+                              1594, 2151,  622,   5462,  10921, 5463, 5470, 5494, 10937,
+                              10925, 10923, 5599, 7543,  11197, 6014, 11963, 10991, 16383};
+
+static bool stencil_has(int dx, int dy, int dz) {
+  for (int d = 0; d < kNbr; d++)
+    if (kDx[d] == dx && kDy[d] == dy && kDz[d] == dz) return true;
+  return false;
+}
+
+// Number of lower / upper link components for one neighbour-existence key and
+// one upper config, then the type rule of src/ExtreM_cuda.cu:458-464.
+static int link_component_type(int key, int cfg) {
+  int dir[kNbr], m = 0;
+  for (int d = 0; d < kNbr; d++)
+    if ((key >> d) & 1) dir[m++] = d;
+  int comp[kNbr];
+  for (int k = 0; k < m; k++) comp[k] = k;
+  bool again = true;
+  while (again) {  // label propagation to the smallest index
+    again = false;
+    for (int a = 0; a < m; a++)
+      for (int b = 0; b < m; b++) {
+        if (a == b || (((cfg >> a) ^ (cfg >> b)) & 1)) continue;
+        int da = dir[a], db = dir[b];
+        if (!stencil_has(kDx[db] - kDx[da], kDy[db] - kDy[da], kDz[db] - kDz[da])) continue;
+        if (comp[b] < comp[a]) {
+          comp[a] = comp[b];
+          again = true;
+        }
+      }
+  }
+  int lower = 0, upper = 0;
+  for (int k = 0; k < m; k++)
+    if (comp[k] == k) (((cfg >> k) & 1) ? upper : lower)++;
+  if (lower == 0 && upper == 1) return 0;
+  if (lower == 1 && upper == 0) return 4;
+  if (lower == 1 && upper == 1) return 5;
+  if (lower > 1 && upper > 1) return 3;
+  if (lower > 1) return 1;
+  if (upper > 1) return 2;
+  return 5;
+}
+
+struct LutTable {
+  std::vector<uint8_t> table;
+  int ref_entries = 0;
+  int cls_key[64], cls_off[64];
+  LutTable() {
+    std::vector<int> key_off(1 << kNbr, -1);
+    auto add = [&](int key) {
+      key_off[key] = (int)table.size();
+      int m = __builtin_popcount(key);
+      for (int c = 0; c < (1 << m); c++) table.push_back((uint8_t)link_component_type(key, c));
+    };
+    for (int k : kKeys) add(k);
+    ref_entries = (int)table.size();
+    for (int cls = 0; cls < 64; cls++) {
+      int st[3] = {cls & 3, (cls >> 2) & 3, (cls >> 4) & 3};
+      const int* comp[3] = {kDx, kDy, kDz};
+      int key = 0;
+      for (int d = 0; d < kNbr; d++) {
+        bool ok = true;
+        for (int a = 0; a < 3; a++) {
+          int c = comp[a][d];
+          if (c < 0 && !(st[a] == 0 || st[a] == 2)) ok = false;
+          if (c > 0 && !(st[a] == 0 || st[a] == 1)) ok = false;
+        }
+        if (ok) key |= 1 << d;
+      }
+      if (key_off[key] < 0) add(key);  // axes of length 1: extension classes
+      cls_key[cls] = key;
+      cls_off[cls] = key_off[key];
+    }
+  }
+};
+
+static const LutTable& lut_table() {
+  static LutTable t;
+  return t;
+}
+
+// --------------------------------------------------------------- context ----
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void ensure(size_t n) {
+    if (n <= cap) return;
+    if (p) hip_check(hipFree(p), "hipFree");
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw std::bad_alloc();
+    }
+    cap = n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct EventPair {
+  hipEvent_t a, b;
+  int fam;
+};
+
+}  // namespace ez
+
+struct extremz_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  ez::DevBuf<uint8_t> lut;
+  ez::DevBuf<long long> delta;
+  ez::DevBuf<uint8_t> cnt, otype, dflag, wn2, wn1;
+  ez::DevBuf<uint16_t> olow, cmask;
+  ez::DevBuf<int> oDS, oAS, dDS, dAS, stamp, touched, dirty, frontA, frontB;
+  ez::DevBuf<int> S2, S1, MX, MN, olarg, osmall, dlarg, dsmall, minsad2, minsad1, win_a, win_b;
+  ez::DevBuf<int> ormap2, ormap1, of2, ol2, of1, ol1, df2, dl2, df1, dl1, actA, actB, wnl2, wnl1;
+  ez::DevBuf<int> ctr;
+  ez::DevBuf<double> orig_tmp, decp_tmp;
+  ez::DevBuf<uint8_t> counts_tmp;
+  int* h_ctr = nullptr;
+  std::vector<ez::EventPair> ev_pool;
+  size_t ev_used = 0;
+  bool timing = false;
+  int64_t host_syncs = 0;
+};
+
+namespace ez {
+
+static thread_local std::string g_err;
+
+class Run {
+ public:
+  Run(extremz_ctx* c, const extremz_opts* o) : c_(c) {
+    max_c_ = (o && o->max_c_iters > 0) ? o->max_c_iters : 100000;
+    max_outer_ = (o && o->max_outer > 0) ? o->max_outer : 10000;
+    c_->timing = o && o->timing;
+    c_->ev_used = 0;
+  }
+
+  int correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
+              extremz_stats* stats, uint8_t* d_counts);
+  void classify(const double* d_f, int W, int H, int D, uint8_t* d_t, uint16_t* d_l);
+  int labels(const double* d_f, int W, int H, int D, int* d_ds, int* d_as);
+
+  int64_t counts_[EXTREMZ_STAT_COUNT_] = {};
+
+ private:
+  extremz_ctx* c_;
+  St s_{};
+  int max_c_, max_outer_;
+  int epoch_ = 0;
+  bool clean_ = false;
+  int front_ = C_FRONT_A;  // counter index of the live frontier
+  int* front_buf_ = nullptr;
+  int* back_buf_ = nullptr;
+
+  hipStream_t st() const { return c_->stream; }
+
+  void setup_geo(int W, int H, int D) {
+    s_.g.W = W;
+    s_.g.H = H;
+    s_.g.D = D;
+    s_.g.WH = W * H;
+    s_.g.N = W * H * D;
+    for (int d = 0; d < kNbr; d++) s_.g.off[d] = kDx[d] + W * (kDy[d] + H * kDz[d]);
+  }
+  dim3 grid3() const {
+    return dim3((s_.g.W + 63) / 64, (s_.g.H + 3) / 4, s_.g.D);
+  }
+  static int blocks_for(long long n) {
+    long long b = (n + 255) / 256;
+    return (int)std::max<long long>(1, std::min<long long>(b, 2048));
+  }
+
+  // ---- timing
+  void tbegin(int fam) {
+    if (!c_->timing) return;
+    if (c_->ev_used == c_->ev_pool.size()) {
+      EventPair p;
+      ez::hip_check(hipEventCreate(&p.a), "hipEventCreate");
+      ez::hip_check(hipEventCreate(&p.b), "hipEventCreate");
+      c_->ev_pool.push_back(p);
+    }
+    EventPair& p = c_->ev_pool[c_->ev_used];
+    p.fam = fam;
+    ez::hip_check(hipEventRecord(p.a, st()), "hipEventRecord");
+  }
+  void tend() {
+    if (!c_->timing) return;
+    ez::hip_check(hipEventRecord(c_->ev_pool[c_->ev_used].b, st()), "hipEventRecord");
+    c_->ev_used++;
+  }
+
+  // ---- counters
+  void set_ctr(int idx, int v) { k_ctr_set<<<1, 64, 0, st()>>>(s_.ctr, idx, v); }
+  void set_ctr2(int a, int b, int v) { k_ctr_set2<<<1, 64, 0, st()>>>(s_.ctr, a, b, v); }
+  void pull() {
+    ez::hip_check(hipGetLastError(), "kernel launch");
+    ez::hip_check(hipMemcpyAsync(c_->h_ctr, s_.ctr, sizeof(int) * C_COUNT_, hipMemcpyDeviceToHost,
+                             st()),
+              "hipMemcpyAsync");
+    ez::hip_check(hipStreamSynchronize(st()), "hipStreamSynchronize");
+    c_->host_syncs++;
+  }
+  int h(int idx) const { return c_->h_ctr[idx]; }
+
+  void allocate(long long N);
+  void alloc_saddles();
+  void apply();
+  void steepest_labels(const double* f, int* ds, int* as);
+  void c_loops();
+  void cp_dec(bool force);
+  void wrong_neighbors();
+  void r_loops();
+  void s_loops(int side, int do_fix);
+  void saddle_loops(int side, int do_fix);
+};
+
+void Run::allocate(long long N) {
+  extremz_ctx* c = c_;
+  c->delta.ensure(N);
+  c->cnt.ensure(N);
+  c->otype.ensure(N);
+  c->dflag.ensure(N);
+  c->wn2.ensure(N);
+  c->wn1.ensure(N);
+  c->olow.ensure(N);
+  for (DevBuf<int>* b : {&c->oDS, &c->oAS, &c->dDS, &c->dAS, &c->stamp, &c->touched, &c->dirty,
+                         &c->frontA, &c->frontB, &c->S2, &c->S1, &c->MX, &c->MN, &c->olarg,
+                         &c->osmall, &c->dlarg, &c->dsmall, &c->minsad2, &c->minsad1,
+                         &c->win_a, &c->win_b})
+    b->ensure(N);
+  c->ctr.ensure(C_COUNT_);
+}
+
+void Run::alloc_saddles() {
+  extremz_ctx* c = c_;
+  long long n2 = std::max(1, s_.n2), n1 = std::max(1, s_.n1);
+  c->ormap2.ensure(n2 * kNbr);
+  c->ormap1.ensure(n1 * kNbr);
+  for (DevBuf<int>* b : {&c->of2, &c->ol2, &c->df2, &c->dl2}) b->ensure(n2);
+  for (DevBuf<int>* b : {&c->of1, &c->ol1, &c->df1, &c->dl1}) b->ensure(n1);
+  c->cmask.ensure(n2 + n1);
+  c->actA.ensure(n2 + n1);
+  c->actB.ensure(n2 + n1);
+  c->wnl2.ensure(s_.g.N);
+  c->wnl1.ensure(s_.g.N);
+  s_.ormap2 = c->ormap2.p;
+  s_.ormap1 = c->ormap1.p;
+  s_.of2 = c->of2.p;
+  s_.ol2 = c->ol2.p;
+  s_.df2 = c->df2.p;
+  s_.dl2 = c->dl2.p;
+  s_.of1 = c->of1.p;
+  s_.ol1 = c->ol1.p;
+  s_.df1 = c->df1.p;
+  s_.dl1 = c->dl1.p;
+  s_.cmask = c->cmask.p;
+  s_.wnl2 = c->wnl2.p;
+  s_.wnl1 = c->wnl1.p;
+}
+
+// apply_delta over the touched list, then the list is emptied.
+void Run::apply() {
+  tbegin(EXTREMZ_K_APPLY);
+  k_apply<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
+  tend();
+  set_ctr(C_TOUCHED, 0);
+}
+
+// ComputeDescendingManifold + ComputeAscendingManifold on field f (D7).
+void Run::steepest_labels(const double* f, int* ds, int* as) {
+  tbegin(EXTREMZ_K_STEEPEST);
+  k_steepest<<<grid3(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as);
+  tend();
+  counts_[EXTREMZ_STAT_LABEL_PASSES]++;
+  int nb = (s_.g.N + 255) / 256;
+  for (int guard = 0; guard < 64; guard++) {
+    tbegin(EXTREMZ_K_JUMP);
+    k_jump<<<nb, 256, 0, st()>>>(s_.g.N, ds, as, s_.ctr);
+    tend();
+    set_ctr(C_CHANGED, 0);
+    tbegin(EXTREMZ_K_JUMP);
+    k_jump<<<nb, 256, 0, st()>>>(s_.g.N, ds, as, s_.ctr);
+    tend();
+    counts_[EXTREMZ_STAT_JUMP_ROUNDS] += 2;
+    pull();
+    if (!h(C_CHANGED)) return;
+  }
+  throw HipError("pointer jumping did not converge");
+}
+
+// c_loops, src/ExtreM.cpp:1505-1583.  The first call scans the full grid; later
+// calls start from the vertices edited since the last clean state.
+void Run::c_loops() {
+  counts_[EXTREMZ_STAT_C_LOOPS_CALLS]++;
+  front_ = C_FRONT_A;
+  front_buf_ = c_->frontA.p;
+  back_buf_ = c_->frontB.p;
+  set_ctr2(C_FRONT_A, C_FRONT_B, 0);
+  epoch_++;
+  if (!clean_) {
+    tbegin(EXTREMZ_K_CHECK_FULL);
+    k_check_full<<<grid3(), dim3(64, 4, 1), 0, st()>>>(s_, front_buf_, front_, epoch_);
+    tend();
+  } else {
+    tbegin(EXTREMZ_K_CHECK_INCR);
+    k_check_list<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, s_.dirty, C_DIRTY, 1, front_buf_,
+                                                        front_, epoch_);
+    tend();
+  }
+  k_clear_dirty<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
+  set_ctr(C_DIRTY, 0);
+  while (true) {
+    pull();
+    int nf = h(front_);
+    if (nf == 0) break;
+    counts_[EXTREMZ_STAT_FRONTIER_ITEMS] += nf;
+    if (++counts_[EXTREMZ_STAT_C_LOOP_ITERS] > max_c_) throw std::range_error("c_loops");
+    tbegin(EXTREMZ_K_PROPOSE);
+    k_propose_c<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_);
+    tend();
+    apply();
+    int back = front_ == C_FRONT_A ? C_FRONT_B : C_FRONT_A;
+    set_ctr(back, 0);
+    epoch_++;
+    tbegin(EXTREMZ_K_CHECK_INCR);
+    k_check_list<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_, 0, back_buf_, back,
+                                                    epoch_);
+    k_check_list<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, s_.dirty, C_DIRTY, 1, back_buf_,
+                                                        back, epoch_);
+    tend();
+    k_clear_dirty<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
+    set_ctr(C_DIRTY, 0);
+    std::swap(front_buf_, back_buf_);
+    front_ = back;
+  }
+  clean_ = true;
+}
+
+// Manifolds + computeCP + largest/smallest saddle maps of decp
+// (main.cpp:290-304).  Skipped when decp has not changed since the last call:
+// every structure it computes depends on decp only.
+void Run::cp_dec(bool force) {
+  if (!force) {
+    pull();
+    if (!h(C_ANY_CHANGE)) return;
+  }
+  steepest_labels(s_.decp, s_.dDS, s_.dAS);
+  set_ctr2(C_DGMAX, C_DGMIN, kNone);
+  tbegin(EXTREMZ_K_SADDLE);
+  k_list_best<<<blocks_for(s_.nmax), 256, 0, st()>>>(s_.MX, s_.nmax, s_.decp, s_.ctr + C_DGMAX, 1);
+  k_list_best<<<blocks_for(s_.nmin), 256, 0, st()>>>(s_.MN, s_.nmin, s_.decp, s_.ctr + C_DGMIN, 0);
+  k_list_fill<<<blocks_for(s_.nmax), 256, 0, st()>>>(s_.MX, s_.nmax, s_.dlarg, kNone);
+  k_list_fill<<<blocks_for(s_.nmin), 256, 0, st()>>>(s_.MN, s_.nmin, s_.dsmall, kNone);
+  if (s_.n2)
+    k_saddles<<<blocks_for(s_.n2), 256, 0, st()>>>(s_, 2, s_.S2, s_.n2, s_.decp, s_.dDS, s_.df2,
+                                                    s_.dl2, s_.dlarg, C_DGMAX, nullptr);
+  if (s_.n1)
+    k_saddles<<<blocks_for(s_.n1), 256, 0, st()>>>(s_, 1, s_.S1, s_.n1, s_.decp, s_.dAS, s_.df1,
+                                                    s_.dl1, s_.dsmall, C_DGMIN, nullptr);
+  tend();
+  set_ctr(C_ANY_CHANGE, 0);
+}
+
+// get_wrong_split_neighbors / get_wrong_join_neighbors over all saddles in
+// saddle-id order (main.cpp:309-323), as rounds of deterministic reservations.
+void Run::wrong_neighbors() {
+  k_wn_reset<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
+  set_ctr2(C_NFC, C_NFC1, 0);
+  set_ctr2(C_ACT_A, C_ACT_B, 0);
+  int* act = c_->actA.p;
+  int* nxt = c_->actB.p;
+  int actr = C_ACT_A, nctr = C_ACT_B;
+  int tot = s_.n2 + s_.n1;
+  if (tot == 0) {
+    pull();
+    return;
+  }
+  k_wn_cand<<<blocks_for(tot), 256, 0, st()>>>(s_, s_.n2, s_.n1, act, actr);
+  // each round commits at least the lowest active saddle, so rounds <= tot
+  for (int round = 0;; round++) {
+    pull();
+    int na = h(actr);
+    if (na == 0) break;
+    if (round > tot) throw HipError("wrong-neighbour reservations did not finish");
+    set_ctr(nctr, 0);
+    int b = blocks_for(na);
+    k_wn_register<<<b, 256, 0, st()>>>(s_, s_.n2, act, actr);
+    k_wn_commit<<<b, 256, 0, st()>>>(s_, s_.n2, act, actr);
+    k_wn_next<<<b, 256, 0, st()>>>(s_, s_.n2, act, actr, nxt, nctr);
+    std::swap(act, nxt);
+    std::swap(actr, nctr);
+  }
+}
+
+// r_loops, src/ExtreM.cpp:2577-2593.
+void Run::r_loops() {
+  counts_[EXTREMZ_STAT_R_LOOP_CALLS]++;
+  tbegin(EXTREMZ_K_PROPOSE);
+  k_fixpath<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
+  tend();
+  apply();
+}
+
+// s_loops / s_loops_join (src/ExtreM.cpp:2838-2886) with do_fix, or the bare
+// get_wrong_index_max / _min of main.cpp:434,441.
+void Run::s_loops(int side, int do_fix) {
+  int n = side == 2 ? s_.n2 : s_.n1;
+  if (side == 2) set_ctr2(C_WMC, C_WMC2, 0);
+  else set_ctr2(C_WMINC, C_WMINC2, 0);
+  if (do_fix) counts_[EXTREMZ_STAT_S_LOOP_CALLS]++;
+  if (n) {
+    int b = blocks_for(n);
+    k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 0, 0);
+    tbegin(EXTREMZ_K_PROPOSE);
+    k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 1, do_fix);
+    tend();
+    k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 2, 0);
+  }
+  if (do_fix) apply();
+}
+
+// saddle_loops / saddle_loops_join (src/ExtreM.cpp:2888-2916) with do_fix,
+// or the bare get_wrong_index_saddles(_join) of main.cpp:435,442.
+void Run::saddle_loops(int side, int do_fix) {
+  int n = side == 2 ? s_.nmax : s_.nmin;
+  set_ctr(side == 2 ? C_WSC : C_WSCJ, 0);
+  if (do_fix) counts_[EXTREMZ_STAT_SADDLE_LOOP_CALLS]++;
+  if (n) {
+    tbegin(EXTREMZ_K_PROPOSE);
+    k_wrong_saddle<<<blocks_for(n), 256, 0, st()>>>(s_, side, do_fix);
+    tend();
+  }
+  if (do_fix) apply();
+}
+
+void Run::classify(const double* d_f, int W, int H, int D, uint8_t* d_t, uint16_t* d_l) {
+  setup_geo(W, H, D);
+  if (c_->lut.p == nullptr) throw HipError("context has no LUT");
+  tbegin(EXTREMZ_K_CLASSIFY);
+  k_classify<<<grid3(), dim3(64, 4, 1), 0, st()>>>(s_.g, c_->lut.p, d_f, d_t, d_l);
+  tend();
+  ez::hip_check(hipGetLastError(), "k_classify");
+  ez::hip_check(hipStreamSynchronize(st()), "sync");
+}
+
+int Run::labels(const double* d_f, int W, int H, int D, int* d_ds, int* d_as) {
+  setup_geo(W, H, D);
+  c_->ctr.ensure(C_COUNT_);
+  s_.ctr = c_->ctr.p;
+  steepest_labels(d_f, d_ds, d_as);
+  return 0;
+}
+
+int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
+                 extremz_stats* stats, uint8_t* d_counts) {
+  setup_geo(W, H, D);
+  const long long N = s_.g.N;
+  allocate(N);
+  extremz_ctx* c = c_;
+  s_.xi = xi;
+  {
+    double sd = -4.0 * xi;
+    long long b;
+    std::memcpy(&b, &sd, 8);
+    s_.sent = b >= 0 ? b : (b ^ 0x7fffffffffffffffLL);
+  }
+  s_.lut = c->lut.p;
+  s_.orig = d_orig;
+  s_.decp = d_decp;
+  s_.delta = c->delta.p;
+  s_.cnt = d_counts ? d_counts : c->cnt.p;
+  s_.otype = c->otype.p;
+  s_.olow = c->olow.p;
+  s_.oDS = c->oDS.p;
+  s_.oAS = c->oAS.p;
+  s_.dDS = c->dDS.p;
+  s_.dAS = c->dAS.p;
+  s_.stamp = c->stamp.p;
+  s_.dflag = c->dflag.p;
+  s_.touched = c->touched.p;
+  s_.dirty = c->dirty.p;
+  s_.ctr = c->ctr.p;
+  s_.S2 = c->S2.p;
+  s_.S1 = c->S1.p;
+  s_.MX = c->MX.p;
+  s_.MN = c->MN.p;
+  s_.olarg = c->olarg.p;
+  s_.osmall = c->osmall.p;
+  s_.dlarg = c->dlarg.p;
+  s_.dsmall = c->dsmall.p;
+  s_.wn2 = c->wn2.p;
+  s_.wn1 = c->wn1.p;
+  s_.minsad2 = c->minsad2.p;
+  s_.minsad1 = c->minsad1.p;
+  s_.win_a = c->win_a.p;
+  s_.win_b = c->win_b.p;
+
+  hipStream_t q = st();
+  const int bN = blocks_for(N);
+  ez::hip_check(hipMemsetAsync(s_.ctr, 0, sizeof(int) * C_COUNT_, q), "memset");
+  k_check_finite<<<bN, 256, 0, q>>>(d_orig, N, s_.ctr);
+  k_check_finite<<<bN, 256, 0, q>>>(d_decp, N, s_.ctr);
+  k_fill_i64<<<bN, 256, 0, q>>>(s_.delta, s_.sent, N);
+  ez::hip_check(hipMemsetAsync(s_.cnt, 0, N, q), "memset");
+  ez::hip_check(hipMemsetAsync(s_.stamp, 0, sizeof(int) * N, q), "memset");
+  ez::hip_check(hipMemsetAsync(s_.dflag, 0, N, q), "memset");
+  ez::hip_check(hipMemsetAsync(s_.wn2, 0, N, q), "memset");
+  ez::hip_check(hipMemsetAsync(s_.wn1, 0, N, q), "memset");
+  for (int* p : {s_.minsad2, s_.minsad1, s_.win_a, s_.win_b})
+    k_fill_i32<<<bN, 256, 0, q>>>(p, 0x7fffffff, N);
+  for (int* p : {s_.olarg, s_.osmall, s_.dlarg, s_.dsmall}) k_fill_i32<<<bN, 256, 0, q>>>(p, kNone, N);
+  set_ctr2(C_GMIN, C_GMAX, kNone);
+  pull();
+  if (h(C_FAIL)) return EXTREMZ_EINVAL;
+
+  // main.cpp:67-75
+  k_argext<<<bN, 256, 0, q>>>(d_orig, N, s_.ctr + C_GMIN, s_.ctr + C_GMAX);
+  // main.cpp:85-89
+  tbegin(EXTREMZ_K_CLASSIFY);
+  k_classify<<<grid3(), dim3(64, 4, 1), 0, q>>>(s_.g, s_.lut, d_orig, s_.otype, s_.olow);
+  tend();
+  steepest_labels(d_orig, s_.oDS, s_.oAS);
+  // main.cpp:100-119: CP lists, triplets, largest/smallest saddle maps of orig
+  k_lists<<<(N + 255) / 256, 256, 0, q>>>(s_);
+  pull();
+  s_.n2 = h(C_N2);
+  s_.n1 = h(C_N1);
+  s_.nmax = h(C_NMAX);
+  s_.nmin = h(C_NMIN);
+  counts_[EXTREMZ_STAT_N_SADDLE2] = s_.n2;
+  counts_[EXTREMZ_STAT_N_SADDLE1] = s_.n1;
+  counts_[EXTREMZ_STAT_N_MAX] = s_.nmax;
+  counts_[EXTREMZ_STAT_N_MIN] = s_.nmin;
+  alloc_saddles();
+  tbegin(EXTREMZ_K_SADDLE);
+  if (s_.n2)
+    k_saddles<<<blocks_for(s_.n2), 256, 0, q>>>(s_, 2, s_.S2, s_.n2, d_orig, s_.oDS, s_.of2,
+                                                 s_.ol2, s_.olarg, C_GMAX, s_.ormap2);
+  if (s_.n1)
+    k_saddles<<<blocks_for(s_.n1), 256, 0, q>>>(s_, 1, s_.S1, s_.n1, d_orig, s_.oAS, s_.of1,
+                                                 s_.ol1, s_.osmall, C_GMIN, s_.ormap1);
+  tend();
+  // main.cpp:74-75, 122-131
+  k_presnap<<<blocks_for(s_.n2 + s_.n1 + 2), 256, 0, q>>>(s_);
+
+  c_loops();     // main.cpp:139
+  cp_dec(true);  // main.cpp:172-191
+  wrong_neighbors();  // main.cpp:210-226
+  int nfc = h(C_NFC), nfc1 = h(C_NFC1);
+  int wmc = 0, wminc = 0;
+  if (nfc == 0) {  // main.cpp:230-254
+    cp_dec(false);
+    s_loops(2, 1);
+    pull();
+    wmc = h(C_WMC);
+    if (wmc == 0) saddle_loops(2, 1);
+  }
+  if (nfc1 == 0) {  // main.cpp:255-274
+    cp_dec(false);
+    s_loops(1, 1);
+    pull();
+    wminc = h(C_WMINC);
+    if (wminc == 0) saddle_loops(1, 1);
+  }
+  auto any = [&]() {
+    pull();
+    return h(C_NFC) > 0 || h(C_NFC1) > 0 || h(C_WMC) > 0 || h(C_WMC2) > 0 || h(C_WSC) > 0 ||
+           h(C_WMINC) > 0 || h(C_WMINC2) > 0 || h(C_WSCJ) > 0;
+  };
+  while (any()) {  // main.cpp:277-446
+    if (++counts_[EXTREMZ_STAT_OUTER_ITERS] > max_outer_) throw std::range_error("outer");
+    r_loops();
+    for (int rep = 0; rep < 3; rep++) {
+      c_loops();
+      cp_dec(false);
+      wrong_neighbors();
+      nfc = h(C_NFC);
+      nfc1 = h(C_NFC1);
+      if (rep < 2) {
+        if (nfc == 0) {
+          s_loops(2, 1);
+          pull();
+          if (h(C_WMC) == 0) saddle_loops(2, 1);
+        }
+        if (nfc1 == 0) {
+          s_loops(1, 1);
+          pull();
+          if (h(C_WMINC) == 0) saddle_loops(1, 1);
+        }
+      } else {
+        if (nfc == 0) {
+          s_loops(2, 0);
+          saddle_loops(2, 0);
+        }
+        if (nfc1 == 0) {
+          s_loops(1, 0);
+          saddle_loops(1, 0);
+        }
+      }
+    }
+  }
+  set_ctr(C_EDITED, 0);
+  k_count_edited<<<bN, 256, 0, q>>>(s_.cnt, N, s_.ctr);
+  pull();
+  counts_[EXTREMZ_STAT_EDITED] = h(C_EDITED);
+  (void)stats;
+  return 0;
+}
+
+static std::mutex g_cache_mu;
+static std::vector<extremz_ctx*> g_cache;
+
+static extremz_ctx* new_ctx(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  ez::hip_check(hipSetDevice(device), "hipSetDevice");
+  hipDeviceProp_t prop;
+  ez::hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return nullptr;
+  extremz_ctx* c = new extremz_ctx();
+  c->device = device;
+  ez::hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  ez::hip_check(hipHostMalloc((void**)&c->h_ctr, sizeof(int) * C_COUNT_, hipHostMallocDefault),
+            "hipHostMalloc");
+  const LutTable& L = lut_table();
+  c->lut.ensure(L.table.size());
+  ez::hip_check(hipMemcpy(c->lut.p, L.table.data(), L.table.size(), hipMemcpyHostToDevice), "lut");
+  upload_constants(kDx, kDy, kDz, L.cls_key, L.cls_off);
+  return c;
+}
+
+static void free_ctx(extremz_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& p : c->ev_pool) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (ez::DevBuf<int>* b :
+       {&c->oDS, &c->oAS, &c->dDS, &c->dAS, &c->stamp, &c->touched, &c->dirty, &c->frontA,
+        &c->frontB, &c->S2, &c->S1, &c->MX, &c->MN, &c->olarg, &c->osmall, &c->dlarg, &c->dsmall,
+        &c->minsad2, &c->minsad1, &c->win_a, &c->win_b, &c->ormap2, &c->ormap1, &c->of2, &c->ol2,
+        &c->of1, &c->ol1, &c->df2, &c->dl2, &c->df1, &c->dl1, &c->actA, &c->actB, &c->wnl2,
+        &c->wnl1, &c->ctr})
+    b->release();
+  for (ez::DevBuf<uint8_t>* b : {&c->lut, &c->cnt, &c->otype, &c->dflag, &c->wn2, &c->wn1,
+                                 &c->counts_tmp})
+    b->release();
+  c->delta.release();
+  c->olow.release();
+  c->cmask.release();
+  c->orig_tmp.release();
+  c->decp_tmp.release();
+  if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static void fill_stats(extremz_ctx* c, const Run& r, int status, double wall_ms,
+                       extremz_stats* stats) {
+  if (!stats) return;
+  std::memset(stats, 0, sizeof(*stats));
+  for (int i = 0; i < EXTREMZ_STAT_COUNT_; i++) stats->counts[i] = r.counts_[i];
+  stats->counts[EXTREMZ_STAT_STATUS] = status;
+  stats->counts[EXTREMZ_STAT_HOST_SYNCS] = c->host_syncs;
+  stats->wall_ms = wall_ms;
+  if (c->timing && c->ev_used) {
+    (void)hipStreamSynchronize(c->stream);
+    for (size_t i = 0; i < c->ev_used; i++) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, c->ev_pool[i].a, c->ev_pool[i].b) == hipSuccess) {
+        stats->kernel_ms[c->ev_pool[i].fam] += ms;
+        stats->kernel_launches[c->ev_pool[i].fam]++;
+      }
+    }
+  }
+}
+
+static bool valid_shape(int W, int H, int D) {
+  if (W <= 0 || H <= 0 || D <= 0 || D > 65535) return false;
+  long long N = (long long)W * H * D;
+  return N < (1LL << 31) - 64;
+}
+
+// Runs fn with error translation; fn returns a status.
+template <class F>
+static int guarded(F&& fn) {
+  g_err.clear();
+  try {
+    return fn();
+  } catch (const std::bad_alloc&) {
+    g_err = "device allocation failed";
+    return EXTREMZ_ENOMEM;
+  } catch (const std::range_error& e) {
+    g_err = std::string("iteration cap reached in ") + e.what();
+    return EXTREMZ_ENOTCONV;
+  } catch (const HipError& e) {
+    g_err = e.what();
+    return EXTREMZ_EHIP;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return EXTREMZ_EHIP;
+  }
+}
+
+static extremz_ctx* cached_ctx(int device) {
+  for (extremz_ctx* c : g_cache)
+    if (c->device == device) return c;
+  extremz_ctx* c = new_ctx(device);
+  if (c) g_cache.push_back(c);
+  return c;
+}
+
+static int run_correct(extremz_ctx* c, const double* d_orig, double* d_decp, int W, int H, int D,
+                       double xi, const extremz_opts* opts, extremz_stats* stats,
+                       uint8_t* d_counts) {
+  auto t0 = std::chrono::steady_clock::now();
+  ez::hip_check(hipSetDevice(c->device), "hipSetDevice");
+  c->host_syncs = 0;
+  Run r(c, opts);
+  int status = guarded([&] { return r.correct(d_orig, d_decp, W, H, D, xi, stats, d_counts); });
+  double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(c, r, status, ms, stats);
+  return status;
+}
+
+}  // namespace ez
+
+// ================================================================ C ABI ====
+extern "C" {
+
+const char* extremz_last_error(void) { return ez::g_err.c_str(); }
+
+int extremz_lut(int32_t* out, int cap) {
+  const ez::LutTable& L = ez::lut_table();
+  if (cap < L.ref_entries || out == nullptr) return L.ref_entries;
+  for (int i = 0; i < L.ref_entries; i++) out[i] = L.table[i];
+  return L.ref_entries;
+}
+
+int extremz_ctx_create(int device, extremz_ctx** out) {
+  if (!out) return EXTREMZ_EINVAL;
+  *out = nullptr;
+  return ez::guarded([&] {
+    extremz_ctx* c = ez::new_ctx(device);
+    if (!c) {
+      ez::g_err = "no gfx950 device with that ordinal";
+      return EXTREMZ_ENODEV;
+    }
+    *out = c;
+    return EXTREMZ_OK;
+  });
+}
+
+int extremz_ctx_destroy(extremz_ctx* ctx) {
+  ez::free_ctx(ctx);
+  return EXTREMZ_OK;
+}
+
+int extremz_correct_device(extremz_ctx* ctx, const double* d_orig, double* d_decp, int W, int H,
+                           int D, double xi, const extremz_opts* opts, extremz_stats* stats,
+                           uint8_t* d_counts) {
+  if (!ctx || !d_orig || !d_decp || !ez::valid_shape(W, H, D) || !(xi > 0) || !std::isfinite(xi))
+    return EXTREMZ_EINVAL;
+  return ez::run_correct(ctx, d_orig, d_decp, W, H, D, xi, opts, stats, d_counts);
+}
+
+int extremz_correct(const void* orig, int orig_dtype, double* decp, int W, int H, int D, double xi,
+                    const extremz_opts* opts, extremz_stats* stats, uint8_t* counts) {
+  if (!orig || !decp || !ez::valid_shape(W, H, D) || !(xi > 0) || !std::isfinite(xi) ||
+      (orig_dtype != EXTREMZ_DTYPE_F32 && orig_dtype != EXTREMZ_DTYPE_F64))
+    return EXTREMZ_EINVAL;
+  std::lock_guard<std::mutex> lk(ez::g_cache_mu);
+  return ez::guarded([&] {
+    extremz_ctx* c = ez::cached_ctx(0);
+    if (!c) {
+      ez::g_err = "no gfx950 device";
+      return EXTREMZ_ENODEV;
+    }
+    ez::hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const size_t N = (size_t)W * H * D;
+    c->orig_tmp.ensure(N);
+    c->decp_tmp.ensure(N);
+    c->counts_tmp.ensure(N);
+    if (orig_dtype == EXTREMZ_DTYPE_F64) {
+      ez::hip_check(hipMemcpyAsync(c->orig_tmp.p, orig, N * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+    } else {
+      // stage float32 in the decp buffer, then promote exactly
+      ez::hip_check(hipMemcpyAsync(c->decp_tmp.p, orig, N * 4, hipMemcpyHostToDevice, c->stream), "h2d");
+      ez::k_f32_to_f64<<<2048, 256, 0, c->stream>>>((const float*)c->decp_tmp.p, c->orig_tmp.p,
+                                                    (long long)N);
+    }
+    ez::hip_check(hipMemcpyAsync(c->decp_tmp.p, decp, N * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+    int rc = ez::run_correct(c, c->orig_tmp.p, c->decp_tmp.p, W, H, D, xi, opts, stats,
+                             c->counts_tmp.p);
+    ez::hip_check(hipMemcpyAsync(decp, c->decp_tmp.p, N * 8, hipMemcpyDeviceToHost, c->stream), "d2h");
+    if (counts)
+      ez::hip_check(hipMemcpyAsync(counts, c->counts_tmp.p, N, hipMemcpyDeviceToHost, c->stream), "d2h");
+    ez::hip_check(hipStreamSynchronize(c->stream), "sync");
+    return rc;
+  });
+}
+
+int extremz_classify(const double* field, int W, int H, int D, uint8_t* types,
+                     uint16_t* lower_masks) {
+  if (!field || !types || !lower_masks || !ez::valid_shape(W, H, D)) return EXTREMZ_EINVAL;
+  std::lock_guard<std::mutex> lk(ez::g_cache_mu);
+  return ez::guarded([&] {
+    extremz_ctx* c = ez::cached_ctx(0);
+    if (!c) return EXTREMZ_ENODEV;
+    ez::hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const size_t N = (size_t)W * H * D;
+    c->orig_tmp.ensure(N);
+    c->otype.ensure(N);
+    c->olow.ensure(N);
+    ez::hip_check(hipMemcpyAsync(c->orig_tmp.p, field, N * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+    ez::Run r(c, nullptr);
+    r.classify(c->orig_tmp.p, W, H, D, c->otype.p, c->olow.p);
+    ez::hip_check(hipMemcpyAsync(types, c->otype.p, N, hipMemcpyDeviceToHost, c->stream), "d2h");
+    ez::hip_check(hipMemcpyAsync(lower_masks, c->olow.p, N * 2, hipMemcpyDeviceToHost, c->stream), "d2h");
+    ez::hip_check(hipStreamSynchronize(c->stream), "sync");
+    return EXTREMZ_OK;
+  });
+}
+
+int extremz_labels(const double* field, int W, int H, int D, int32_t* ds, int32_t* as) {
+  if (!field || !ds || !as || !ez::valid_shape(W, H, D)) return EXTREMZ_EINVAL;
+  std::lock_guard<std::mutex> lk(ez::g_cache_mu);
+  return ez::guarded([&] {
+    extremz_ctx* c = ez::cached_ctx(0);
+    if (!c) return EXTREMZ_ENODEV;
+    ez::hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const size_t N = (size_t)W * H * D;
+    c->orig_tmp.ensure(N);
+    c->oDS.ensure(N);
+    c->oAS.ensure(N);
+    ez::hip_check(hipMemcpyAsync(c->orig_tmp.p, field, N * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+    ez::Run r(c, nullptr);
+    r.labels(c->orig_tmp.p, W, H, D, c->oDS.p, c->oAS.p);
+    ez::hip_check(hipMemcpyAsync(ds, c->oDS.p, N * 4, hipMemcpyDeviceToHost, c->stream), "d2h");
+    ez::hip_check(hipMemcpyAsync(as, c->oAS.p, N * 4, hipMemcpyDeviceToHost, c->stream), "d2h");
+    ez::hip_check(hipStreamSynchronize(c->stream), "sync");
+    return EXTREMZ_OK;
+  });
+}
+
+int extremz_encode(const double* orig, const double* decp_before, const double* decp_after,
+                   const uint8_t* counts, int64_t n, double xi, uint8_t* index_bitmap,
+                   double* edits, uint64_t* mantissas, int32_t* exponents, int64_t* n_edits) {
+  if (!orig || !decp_before || !decp_after || !counts || n <= 0 || n >= (1LL << 31) - 64 ||
+      !index_bitmap)
+    return EXTREMZ_EINVAL;
+  std::lock_guard<std::mutex> lk(ez::g_cache_mu);
+  return ez::guarded([&] {
+    extremz_ctx* c = ez::cached_ctx(0);
+    if (!c) return EXTREMZ_ENODEV;
+    ez::hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hipStream_t q = c->stream;
+    const long long N = n;
+    const long long nbytes = (3 * N + 7) / 8;
+    const long long nblk = (N + 1023) / 1024;
+    ez::DevBuf<double> o, b, a, e;
+    ez::DevBuf<uint8_t> cn, bm;
+    ez::DevBuf<int> flag, pos, bs, ex;
+    ez::DevBuf<unsigned long long> mt;
+    o.ensure(N); b.ensure(N); a.ensure(N); cn.ensure(N); bm.ensure(nbytes);
+    flag.ensure(N); pos.ensure(N); bs.ensure(nblk);
+    ez::hip_check(hipMemcpyAsync(o.p, orig, N * 8, hipMemcpyHostToDevice, q), "h2d");
+    ez::hip_check(hipMemcpyAsync(b.p, decp_before, N * 8, hipMemcpyHostToDevice, q), "h2d");
+    ez::hip_check(hipMemcpyAsync(a.p, decp_after, N * 8, hipMemcpyHostToDevice, q), "h2d");
+    ez::hip_check(hipMemcpyAsync(cn.p, counts, N, hipMemcpyHostToDevice, q), "h2d");
+    ez::k_bitmap3<<<2048, 256, 0, q>>>(b.p, a.p, cn.p, N, bm.p, nbytes);
+    ez::k_edit_flags<<<2048, 256, 0, q>>>(b.p, a.p, cn.p, N, flag.p);
+    ez::k_scan_block<<<(unsigned)nblk, 1024, 0, q>>>(flag.p, pos.p, N, bs.p);
+    std::vector<int> hb(nblk);
+    ez::hip_check(hipMemcpyAsync(hb.data(), bs.p, nblk * 4, hipMemcpyDeviceToHost, q), "d2h");
+    ez::hip_check(hipMemcpyAsync(index_bitmap, bm.p, nbytes, hipMemcpyDeviceToHost, q), "d2h");
+    ez::hip_check(hipStreamSynchronize(q), "sync");
+    long long total = 0;
+    for (long long i = 0; i < nblk; i++) {
+      long long v = hb[i];
+      hb[i] = (int)total;
+      total += v;
+    }
+    if (n_edits) *n_edits = total;
+    if (total > 0 && (edits || mantissas || exponents)) {
+      ez::hip_check(hipMemcpyAsync(bs.p, hb.data(), nblk * 4, hipMemcpyHostToDevice, q), "h2d");
+      ez::k_scan_add<<<(unsigned)nblk, 1024, 0, q>>>(pos.p, N, bs.p);
+      e.ensure(total);
+      mt.ensure(total);
+      ex.ensure(total);
+      ez::k_edit_write<<<2048, 256, 0, q>>>(o.p, b.p, pos.p, flag.p, N, xi, e.p, mt.p, ex.p);
+      if (edits) ez::hip_check(hipMemcpyAsync(edits, e.p, total * 8, hipMemcpyDeviceToHost, q), "d2h");
+      if (mantissas)
+        ez::hip_check(hipMemcpyAsync(mantissas, mt.p, total * 8, hipMemcpyDeviceToHost, q), "d2h");
+      if (exponents)
+        ez::hip_check(hipMemcpyAsync(exponents, ex.p, total * 4, hipMemcpyDeviceToHost, q), "d2h");
+    }
+    ez::hip_check(hipStreamSynchronize(q), "sync");
+    return EXTREMZ_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,145 @@
+/* extremz.h -- C ABI of the MI355X ExtreM correction engine.
+ *
+ * Plain pointers and sizes only.  Fields are raw little-endian arrays with x
+ * fastest (idx = x + W*(y + H*z)), the layout of decp.bin / fixed.bin
+ * (src/ExtreM_cuda.cu:1002, src/ExtreM.cpp:2918-2928).  All arithmetic is
+ * IEEE float64 without contraction, as the reference (src/ExtreM.cpp:33).
+ *
+ * Every entry point returns 0 or a negative EXTREMZ_E* code.  A context owns
+ * one HIP stream and the device work buffers; calls on one context must not
+ * overlap, calls on different contexts may.
+ */
+#ifndef EXTREMZ_H
+#define EXTREMZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXTREMZ_OK 0
+#define EXTREMZ_ENOTCONV (-3)  /* an iteration cap was reached */
+#define EXTREMZ_EHIP (-5)      /* a HIP runtime call failed */
+#define EXTREMZ_ENOMEM (-12)   /* device allocation failed */
+#define EXTREMZ_ENODEV (-19)   /* no usable gfx950 device */
+#define EXTREMZ_EINVAL (-22)   /* bad shape, bound, dtype or non-finite input */
+
+#define EXTREMZ_DTYPE_F32 0
+#define EXTREMZ_DTYPE_F64 1
+
+/* Indices into extremz_stats.counts. */
+enum {
+  EXTREMZ_STAT_STATUS = 0,
+  EXTREMZ_STAT_C_LOOP_ITERS,    /* iterations of the c_loops inner while */
+  EXTREMZ_STAT_OUTER_ITERS,     /* iterations of main.cpp:277 while */
+  EXTREMZ_STAT_R_LOOP_CALLS,
+  EXTREMZ_STAT_S_LOOP_CALLS,
+  EXTREMZ_STAT_SADDLE_LOOP_CALLS,
+  EXTREMZ_STAT_C_LOOPS_CALLS,
+  EXTREMZ_STAT_EDITED,          /* vertices with a non-zero edit count */
+  EXTREMZ_STAT_N_MAX,
+  EXTREMZ_STAT_N_MIN,
+  EXTREMZ_STAT_N_SADDLE1,
+  EXTREMZ_STAT_N_SADDLE2,
+  EXTREMZ_STAT_LABEL_PASSES,    /* full-grid manifold label computations */
+  EXTREMZ_STAT_JUMP_ROUNDS,     /* pointer-jumping rounds, all passes */
+  EXTREMZ_STAT_HOST_SYNCS,      /* device->host counter reads */
+  EXTREMZ_STAT_FRONTIER_ITEMS,  /* false-critical-point frontier entries */
+  EXTREMZ_STAT_COUNT_ = 16
+};
+
+/* Kernel families timed with HIP events on the context stream. */
+enum {
+  EXTREMZ_K_CLASSIFY = 0, /* full-grid classification of the original */
+  EXTREMZ_K_CHECK_FULL,   /* full-grid false-critical-point scan */
+  EXTREMZ_K_CHECK_INCR,   /* frontier / neighbourhood re-check */
+  EXTREMZ_K_STEEPEST,     /* full-grid steepest ascent+descent pointers */
+  EXTREMZ_K_JUMP,         /* one pointer-jumping round, both manifolds */
+  EXTREMZ_K_SADDLE,       /* per-saddle triplet / extremum-map pass */
+  EXTREMZ_K_PROPOSE,      /* correction proposals (all loops) */
+  EXTREMZ_K_APPLY,        /* apply_delta over touched vertices */
+  EXTREMZ_K_FAMILIES_ = 8
+};
+
+typedef struct extremz_opts {
+  int max_c_iters; /* cap on c_loops inner iterations per run; 0 = 100000 */
+  int max_outer;   /* cap on outer iterations; 0 = 10000 */
+  int timing;      /* 1 = record HIP events per kernel family */
+  int reserved;
+} extremz_opts;
+
+typedef struct extremz_stats {
+  int64_t counts[EXTREMZ_STAT_COUNT_];
+  double kernel_ms[EXTREMZ_K_FAMILIES_];     /* summed event time */
+  int64_t kernel_launches[EXTREMZ_K_FAMILIES_];
+  double wall_ms;                            /* host wall time of the call */
+} extremz_stats;
+
+typedef struct extremz_ctx extremz_ctx;
+
+/* Creates a context on a HIP device.  No reference counterpart: the
+ * reference keeps all state in globals (include/api/ExtreM.h:17-41). */
+int extremz_ctx_create(int device, extremz_ctx** out);
+int extremz_ctx_destroy(extremz_ctx* ctx);
+
+/* The correction call: replaces the sequence of main.cpp:65-446 (getdata's
+ * field + bound, pre-snap, classifyAllVertices, manifolds, computeCP,
+ * c_loops / r_loops / s_loops / saddle_loops until every counter is zero)
+ * whose result main.cpp:507 writes to fixed.bin.  orig is float32 or
+ * float64 (orig_dtype), promoted exactly; decp is edited in place; xi is
+ * the absolute bound (src/ExtreM.cpp:138: (max-min)*er).  counts (nullable)
+ * receives delta_counter (main.cpp:57) as uint8.  Host pointers. */
+int extremz_correct(const void* orig, int orig_dtype, double* decp, int W, int H,
+                    int D, double xi, const extremz_opts* opts,
+                    extremz_stats* stats, uint8_t* counts);
+
+/* Same on device-resident buffers (orig float64), on the context stream.
+ * Returns when the result is in d_decp. */
+int extremz_correct_device(extremz_ctx* ctx, const double* d_orig, double* d_decp,
+                           int W, int H, int D, double xi,
+                           const extremz_opts* opts, extremz_stats* stats,
+                           uint8_t* d_counts);
+
+/* Phase (a): vertex types (0 min, 1 1-saddle, 2 2-saddle, 3 both, 4 max,
+ * 5 regular; src/ExtreM.cpp:880) and lower-star direction masks of a float64
+ * field; replaces classifyAllVertices (src/ExtreM.cpp:1088-1097) with the
+ * LUT rule of src/ExtreM_cuda.cu:3796-3853.  Host pointers. */
+int extremz_classify(const double* field, int W, int H, int D, uint8_t* types,
+                     uint16_t* lower_masks);
+
+/* Phase (b): fully compressed steepest-ascent (ds) and steepest-descent (as)
+ * root labels; replaces ComputeDescendingManifold / ComputeAscendingManifold
+ * (src/ExtreM.cpp:144-183, 502-541).  Host pointers. */
+int extremz_labels(const double* field, int W, int H, int D, int32_t* ds,
+                   int32_t* as);
+
+/* The classification table in the layout of LUT.bin (24 800 int32 over the
+ * 27 boundary keys of src/ExtreM_cuda.cu:84-110), generated from the link
+ * definition; loadLUTToGPU (src/ExtreM_cuda.cu:244-260) reads the same data
+ * from disk.  Returns the entry count (or the required size if cap is too
+ * small).  Runs on the host; no GPU needed. */
+int extremz_lut(int32_t* out, int cap);
+
+/* Phase (d): edit encoding, cost() of src/ExtreM.cpp:272-389.
+ * index_bitmap receives the 3-bit LSB-first packing of per-vertex edit
+ * counts (0 for unedited vertices; encodeTo3BitBitmap, src/ExtreM.cpp:220-250),
+ * ceil(3N/8) bytes.  For every vertex whose count is 6 (snapped) the edit
+ * -(decp0 - (orig - xi)) is written in index order to edits (float64), its
+ * 52-bit mantissa to mantissas and unbiased exponent to exponents (the split
+ * of src/ExtreM_cuda.cu:4466-4476).  n_edits receives the number of edits;
+ * the three edit arrays (nullable) need room for N entries.  Host pointers;
+ * runs on the GPU. */
+int extremz_encode(const double* orig, const double* decp_before,
+                   const double* decp_after, const uint8_t* counts, int64_t n,
+                   double xi, uint8_t* index_bitmap, double* edits,
+                   uint64_t* mantissas, int32_t* exponents, int64_t* n_edits);
+
+/* Last error message of the calling thread ("" if none). */
+const char* extremz_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EXTREMZ_H */

@@ -33,8 +33,13 @@
 //   D11 an empty triplet gives NONE (-1) for its first/last entry.
 //   D12 a largest/smallest-saddle index equal to the list size (no adjacent
 //       saddle; always the case for the global extremum) reads as NONE
-//       instead of one element past the end of the vector.
-//   D13 fix_wrong_index_saddle is skipped when either index is NONE.
+//       instead of one element past the end of the vector, and a pair with a
+//       NONE side is no mismatch: it is neither counted nor fixed (the
+//       reference reads past both vectors there; counting it would stall
+//       the loop on a pair that no fix addresses, e.g. when decp's highest
+//       maximum is not the original global maximum).
+//   D13 fix_wrong_index_saddle is skipped when either index is NONE (kept as
+//       a guard; D12 no longer produces such pairs).
 //   LUT extension: fields with an axis of length 1 give boundary classes the
 //   reference has no LUT segment for; they use the same link-component rule.
 //   Iteration caps turn the reference's unbounded loops into an error code.
@@ -696,7 +701,7 @@ struct Engine {
       int oi = omap[ext];  // operator[] as the reference
       int dv = (di >= 0 && di < (int)dsad.size()) ? dsad[di] : NONE;
       int ov = (oi >= 0 && oi < (int)osad.size()) ? osad[oi] : NONE;
-      if (dv != ov) {
+      if (dv != ov && dv != NONE && ov != NONE) {  // D12
         out.push_back(ov);
         out.push_back(dv);
         cnt++;

@@ -1,0 +1,127 @@
+"""GPU parity: the HIP engine against the CPU oracle on the same inputs.
+
+Integer outputs (types, masks, labels, counts) and the corrected float64
+field must be bit-identical; iteration counters must match.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests.conftest import make_field
+from tests.topo import violations
+
+pytestmark = pytest.mark.gpu
+
+
+def ez():
+    import extrem_z_amd as m
+    return m
+
+
+CASES = [
+    ((16, 16, 16), 3, "gauss", 1e-3),
+    ((20, 14, 10), 4, "noise", 2e-2),
+    ((12, 18, 9), 5, "gauss", 1e-2),
+    ((33, 17, 5), 6, "noise", 5e-2),
+    ((40, 40, 40), 8, "gauss", 1e-3),
+    ((7, 1, 9), 9, "noise", 1e-1),
+]
+
+
+@pytest.mark.parametrize("shape,seed,kind,er", CASES)
+def test_classify_parity(shape, seed, kind, er):
+    orig, decp, xi = make_field(shape, seed, kind, er)
+    for f in (orig, decp):
+        t0, m0 = oracle.classify(f)
+        t1, m1 = ez().classify(f)
+        assert np.array_equal(t0, t1)
+        assert np.array_equal(m0, m1)
+
+
+@pytest.mark.parametrize("shape,seed,kind,er", CASES)
+def test_labels_parity(shape, seed, kind, er):
+    orig, decp, xi = make_field(shape, seed, kind, er)
+    for f in (orig, decp):
+        a0, b0 = oracle.labels(f)
+        a1, b1 = ez().labels(f)
+        assert np.array_equal(a0, a1)
+        assert np.array_equal(b0, b1)
+
+
+COUNTERS = ("status", "c_loop_iters", "outer_iters", "r_loop_calls", "s_loop_calls",
+            "saddle_loop_calls", "c_loops_calls", "edited", "n_max", "n_min", "n_saddle1",
+            "n_saddle2")
+
+
+@pytest.mark.parametrize("shape,seed,kind,er", CASES)
+def test_correct_parity(shape, seed, kind, er):
+    orig, decp, xi = make_field(shape, seed, kind, er)
+    f0, c0, s0 = oracle.correct(orig, decp, xi)
+    f1, c1, s1 = ez().correct(orig.astype(np.float32), decp, xi)
+    assert s0["status"] == 0
+    assert f0.tobytes() == f1.tobytes()
+    assert np.array_equal(c0, c1)
+    assert {k: s0[k] for k in COUNTERS} == {k: s1[k] for k in COUNTERS}
+
+
+def test_known_answer_on_gpu():
+    orig = np.array([0.0, 0.1, 0.2, 0.3, 0.4]).reshape(1, 1, 5)
+    decp = np.array([0.0, 0.18, 0.12, 0.3, 0.4]).reshape(1, 1, 5)
+    fixed, counts, st = ez().correct(orig, decp, 0.1)
+    assert fixed.ravel().tolist() == [-0.1, 0.09, 0.12, 0.3, 0.30000000000000004]
+    assert counts.ravel().tolist() == [0, 1, 0, 0, 0]
+
+
+def test_config1_parity_64cube():
+    from extrem_z_amd import synth
+    orig, decp, xi = synth.config(1)
+    f0, c0, s0 = oracle.correct(orig, decp, xi)
+    f1, c1, s1 = ez().correct(orig.astype(np.float32), decp, xi)
+    assert f0.tobytes() == f1.tobytes()
+    assert np.array_equal(c0, c1)
+    assert {k: s0[k] for k in COUNTERS} == {k: s1[k] for k in COUNTERS}
+
+
+def test_repeat_calls_are_identical():
+    orig, decp, xi = make_field((24, 20, 16), 11, "gauss", 1e-3)
+    a = ez().correct(orig, decp, xi)
+    big_o, big_d, big_xi = make_field((30, 30, 30), 12, "noise", 1e-2)
+    ez().correct(big_o, big_d, big_xi)          # a larger field in between
+    b = ez().correct(orig, decp, xi)
+    assert a[0].tobytes() == b[0].tobytes() and np.array_equal(a[1], b[1])
+
+
+def test_encode_parity():
+    orig, decp, xi = make_field((24, 20, 16), 13, "noise", 2e-2)
+    fixed, counts, _ = oracle.correct(orig, decp, xi)
+    bm, edits, mant, expo = ez().encode(orig, decp, fixed, counts, xi)
+    vals = np.where(decp.ravel() != fixed.ravel(), counts.ravel(), 0).astype(np.uint8)
+    assert np.array_equal(bm, oracle.bitmap3(vals))
+    sel = (decp.ravel() != fixed.ravel()) & (counts.ravel() == 6)
+    want = -(decp.ravel()[sel] - (orig.ravel()[sel] - xi))
+    assert edits.tobytes() == want.tobytes()
+    bits = want.view(np.uint64)
+    assert np.array_equal(mant, bits & np.uint64((1 << 52) - 1))
+    assert np.array_equal(expo, ((bits >> np.uint64(52)) & np.uint64(0x7FF)).astype(np.int32) - 1023)
+
+
+def test_invariants_at_128cube():
+    orig, decp, xi = make_field((128, 128, 128), 21, "gauss", 1e-3, n_gauss=16)
+    fixed, counts, st = ez().correct(orig.astype(np.float32), decp, xi)
+    assert st["status"] == 0
+    tol = xi + np.spacing(np.abs(orig) + xi)
+    assert np.all(np.abs(fixed - orig) <= tol)
+    t0, m0 = ez().classify(orig)
+    t1, m1 = ez().classify(fixed)
+    assert np.array_equal(t0, t1)
+    sad = (t0 >= 1) & (t0 <= 3)
+    assert np.array_equal(m0[sad], m1[sad])
+
+
+def test_nonfinite_input_rejected():
+    o = np.zeros((4, 4, 4))
+    d = o.copy()
+    d[1, 1, 1] = np.nan
+    with pytest.raises(ez().ExtremzError) as e:
+        ez().correct(o, d, 0.1)
+    assert e.value.code == -22

@@ -1,0 +1,123 @@
+"""Phase (a) pins: the LUT against the reference's LUT.bin, and vertex
+classification against a brute-force link-component count and closed forms."""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests.topo import DIRS
+
+LUT_SHA256 = "10af892bf6e660f3a52841a1d59330344386f3bd14db39c10742e60e6077a912"
+
+
+def test_lut_fixture_is_the_reference_table(golden):
+    import hashlib
+    data = open(os.path.join(golden, "LUT.bin"), "rb").read()
+    assert len(data) == 99200
+    assert hashlib.sha256(data).hexdigest() == LUT_SHA256
+
+
+def test_oracle_lut_matches_reference(golden):
+    ref = np.fromfile(os.path.join(golden, "LUT.bin"), dtype="<i4")
+    assert np.array_equal(oracle.lut(), ref)
+    # value histogram of LUT.bin
+    assert np.bincount(ref).tolist() == [27, 6859, 6859, 200, 27, 10828]
+
+
+def test_product_lut_matches_reference(golden):
+    import extrem_z_amd as ez
+    ref = np.fromfile(os.path.join(golden, "LUT.bin"), dtype="<i4")
+    assert np.array_equal(ez.lut(), ref)
+
+
+def brute_type(vals, centre_idx, centre, nbr_ids, present):
+    """Union-find over the link of one vertex, straight from the definition:
+    link vertices are the existing stencil neighbours, link edges join
+    neighbours a, b whose offset difference is itself a stencil direction."""
+    lower = []
+    upper = []
+    for d in range(14):
+        if not present[d]:
+            continue
+        v, i = vals[d], nbr_ids[d]
+        is_lower = v < centre or (v == centre and i < centre_idx)
+        (lower if is_lower else upper).append(d)
+    def comps(S):
+        parent = {d: d for d in S}
+        def f(a):
+            while parent[a] != a:
+                a = parent[a]
+            return a
+        for a, b in itertools.combinations(S, 2):
+            diff = tuple(DIRS[b][k] - DIRS[a][k] for k in range(3))
+            if diff in DIRS:
+                parent[f(a)] = f(b)
+        return len({f(d) for d in S})
+    lc, uc = comps(lower), comps(upper)
+    if not upper:
+        return 4
+    if not lower:
+        return 0
+    if lc > 1 and uc > 1:
+        return 3
+    if lc > 1:
+        return 1
+    if uc > 1:
+        return 2
+    return 5
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("shape", [(5, 4, 3), (3, 3, 3), (6, 2, 4), (7, 1, 3)])
+def test_classification_matches_bruteforce(shape, seed):
+    W, H, D = shape
+    rng = np.random.default_rng(seed)
+    # few distinct values -> many ties broken by index
+    f = rng.integers(0, 4, size=(D, H, W)).astype(np.float64)
+    types, masks = oracle.classify(f)
+    flat = f.ravel()
+    for z, y, x in itertools.product(range(D), range(H), range(W)):
+        v = x + W * (y + H * z)
+        present, vals, ids = [], [], []
+        for dx, dy, dz in DIRS:
+            nx, ny, nz = x + dx, y + dy, z + dz
+            ok = 0 <= nx < W and 0 <= ny < H and 0 <= nz < D
+            present.append(ok)
+            i = nx + W * (ny + H * nz) if ok else -1
+            ids.append(i)
+            vals.append(flat[i] if ok else 0.0)
+        assert types.ravel()[v] == brute_type(vals, v, flat[v], ids, present)
+        want = sum(1 << d for d in range(14) if present[d] and
+                   (vals[d] < flat[v] or (vals[d] == flat[v] and ids[d] < v)))
+        assert masks.ravel()[v] == want
+
+
+def grid(shape):
+    W, H, D = shape
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    return x.astype(np.float64), y.astype(np.float64), z.astype(np.float64)
+
+
+def test_ramp_has_one_min_one_max():
+    x, y, z = grid((6, 5, 4))
+    t, _ = oracle.classify(x + 2 * y + 4 * z)
+    assert np.bincount(t.ravel(), minlength=6).tolist() == [1, 0, 0, 0, 1, 6 * 5 * 4 - 2]
+    assert t[0, 0, 0] == 0 and t[-1, -1, -1] == 4
+
+
+def test_gaussian_bump_has_one_max():
+    x, y, z = grid((9, 9, 9))
+    f = np.exp(-((x - 4) ** 2 + (y - 4) ** 2 + (z - 4) ** 2) / 8.0)
+    t, _ = oracle.classify(f)
+    assert (t == 4).sum() == 1 and t[4, 4, 4] == 4
+
+
+def test_saddles_closed_form():
+    x, y, z = grid((5, 5, 5))
+    f = (x - 2) ** 2 + (y - 2) ** 2 - 2.5 * (z - 2) ** 2
+    t, _ = oracle.classify(f)
+    assert t[2, 2, 2] == 1          # two lower cones, one upper ring
+    t, _ = oracle.classify(-f)
+    assert t[2, 2, 2] == 2
