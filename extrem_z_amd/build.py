@@ -30,6 +30,10 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    cli = os.path.join(OUT_DIR, "extremz_cli")
+    subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(SRC, "cli.cpp"), "-o", cli + ".tmp",
+                    "-L" + OUT_DIR, "-lextremz", "-Wl,-rpath,$ORIGIN"], check=True)
+    os.replace(cli + ".tmp", cli)
     return LIB
 
 

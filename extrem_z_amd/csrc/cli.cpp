@@ -1,0 +1,128 @@
+// extremz_cli -- command-line driver with the reference's argument form.
+//
+//   extremz_cli <path,W,H,D> <er> <compressor_id> [--decp decp.bin]
+//               [--orig-f32] [--out fixed.bin] [--counts counts.bin]
+//
+// Same positional arguments as main.cpp:13-37: a raw float64 field (getdata,
+// src/ExtreM.cpp:95-118; --orig-f32 for float32), the relative bound er
+// (xi = (max - min) * er, src/ExtreM.cpp:136-138) and a compressor id that
+// only names outputs.  The reference produces decp with SZ3 inside getdata
+// (src/ExtreM.cpp:121-131); here decp is read from --decp, or, without it,
+// made by the uniform quantiser decp = 2*xi*rint(orig/(2*xi)).  The corrected
+// field is written as raw float64 to fixed.bin (main.cpp:507).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/extremz.h"
+
+static bool read_raw(const std::string& path, void* dst, size_t bytes) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f) {
+    std::fprintf(stderr, "Cannot open file: %s\n", path.c_str());
+    return false;
+  }
+  if ((size_t)f.tellg() != bytes) {
+    std::fprintf(stderr, "File size does not match expected data size.\n");
+    return false;
+  }
+  f.seekg(0);
+  f.read(static_cast<char*>(dst), (std::streamsize)bytes);
+  return (bool)f;
+}
+
+static bool write_raw(const std::string& path, const void* src, size_t bytes) {
+  std::ofstream f(path, std::ios::binary);
+  if (!f) {
+    std::fprintf(stderr, "Error opening file for writing: %s\n", path.c_str());
+    return false;
+  }
+  f.write(static_cast<const char*>(src), (std::streamsize)bytes);
+  return (bool)f;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 4) {
+    std::fprintf(stderr,
+                 "usage: %s <path,W,H,D> <er> <compressor_id> [--decp decp.bin] [--orig-f32] "
+                 "[--out fixed.bin] [--counts counts.bin]\n",
+                 argv[0]);
+    return 2;
+  }
+  std::string dims = argv[1], path;
+  std::istringstream iss(dims);
+  int W = 0, H = 0, D = 0;
+  char sep;
+  if (!std::getline(iss, path, ',') || !(iss >> W >> sep) || sep != ',' || !(iss >> H >> sep) ||
+      sep != ',' || !(iss >> D)) {
+    std::fprintf(stderr, "Parsing error for dimensions\n");
+    return 2;
+  }
+  double er = std::strtod(argv[2], nullptr);
+  std::string compressor_id = argv[3];
+  std::string decp_path, out_path = "fixed.bin", counts_path;
+  bool f32 = false;
+  for (int i = 4; i < argc; i++) {
+    std::string a = argv[i];
+    if (a == "--decp" && i + 1 < argc) decp_path = argv[++i];
+    else if (a == "--out" && i + 1 < argc) out_path = argv[++i];
+    else if (a == "--counts" && i + 1 < argc) counts_path = argv[++i];
+    else if (a == "--orig-f32") f32 = true;
+    else {
+      std::fprintf(stderr, "unknown argument %s\n", a.c_str());
+      return 2;
+    }
+  }
+  const size_t N = (size_t)W * H * D;
+  std::printf("Filepath: %s\nWidth: %d\nHeight: %d\nDepth: %d\nsize is:%zu\n", path.c_str(), W, H,
+              D, N);
+  std::vector<double> orig(N);
+  if (f32) {
+    std::vector<float> tmp(N);
+    if (!read_raw(path, tmp.data(), N * 4)) return 1;
+    for (size_t i = 0; i < N; i++) orig[i] = tmp[i];
+  } else if (!read_raw(path, orig.data(), N * 8)) {
+    return 1;
+  }
+  double mn = orig[0], mx = orig[0];
+  for (double v : orig) {
+    mn = v < mn ? v : mn;
+    mx = v > mx ? v : mx;
+  }
+  const double xi = (mx - mn) * er;
+  std::vector<double> decp(N);
+  if (!decp_path.empty()) {
+    if (!read_raw(decp_path, decp.data(), N * 8)) return 1;
+  } else {
+    const double q = 2.0 * xi;
+    for (size_t i = 0; i < N; i++) {
+      double d = q * std::rint(orig[i] / q);
+      if (std::fabs(d - orig[i]) > xi) d -= (d > orig[i] ? q : -q);
+      decp[i] = d;
+    }
+  }
+  std::vector<double> before = decp;
+  std::vector<uint8_t> counts(N);
+  extremz_stats st;
+  extremz_opts opts = {0, 0, 0, 0};
+  int rc = extremz_correct(orig.data(), EXTREMZ_DTYPE_F64, decp.data(), W, H, D, xi, &opts, &st,
+                           counts.data());
+  if (rc != 0) {
+    std::fprintf(stderr, "extremz_correct failed (%d): %s\n", rc, extremz_last_error());
+    return 1;
+  }
+  size_t changed = 0;
+  for (size_t i = 0; i < N; i++) changed += decp[i] != before[i];
+  std::printf("bound: %.17g\nouter iterations: %lld\nc_loop iterations: %lld\n", xi,
+              (long long)st.counts[EXTREMZ_STAT_OUTER_ITERS],
+              (long long)st.counts[EXTREMZ_STAT_C_LOOP_ITERS]);
+  std::printf("edit ratio: %.16f\nwall ms: %.3f\n", (double)changed / (double)N, st.wall_ms);
+  if (!write_raw(out_path, decp.data(), N * 8)) return 1;
+  if (!counts_path.empty() && !write_raw(counts_path, counts.data(), N)) return 1;
+  return 0;
+}
