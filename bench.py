@@ -29,8 +29,9 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_VOXEL = {
     "classify": 8 + 1 + 2,        # read f64 field, write u8 type + u16 mask
     "check_full": 8 + 2,          # read f64 decp + u16 original mask
-    "steepest": 8 + 4 + 4,        # read f64 field, write two i32 pointers
+    "steepest": 8 + 4 + 4,        # label tiles: read f64 field, write two i32 pointers
     "jump": 4 * 4,                # read + write both i32 label arrays
+    "saddle": 1 + 8 + 8 + 2 + 8,  # saddle pass: type, decp, two decp roots, mask, two orig roots
 }
 
 

@@ -40,6 +40,8 @@ enum Ctr {
   C_ANY_CHANGE,   // decp changed since the last CP computation
   C_FAIL,
   C_EDITED,
+  C_ACT0,         // wrong-neighbour candidates of the last CP pass
+  C_PAIRS,        // (extremum, saddle) pairs of the slot pass
   C_COUNT_
 };
 
@@ -60,6 +62,7 @@ struct St {
   uint8_t* cnt;         // delta_counter
   uint8_t* otype;       // vertex_type
   uint16_t* olow;       // original lower-star masks
+  uint16_t* dlow;       // decp lower-star masks (written by the label pass)
   int *oDS, *oAS, *dDS, *dAS;
   int* stamp;
   uint8_t* dflag;
@@ -68,11 +71,11 @@ struct St {
   // CP lists (original types; equal to the decp types wherever used)
   int *S2, *S1, *MX, *MN;
   int n2, n1, nmax, nmin;
-  int *ormap2, *ormap1;           // 14 original roots per saddle
-  int *of2, *ol2, *of1, *ol1;     // original triplet first / last
-  int *df2, *dl2, *df1, *dl1;     // decp triplet first / last
-  int *olarg, *osmall, *dlarg, *dsmall;  // per extremum (vertex-indexed)
-  uint16_t* cmask;                // greedy candidate masks per saddle
+  int *of2, *ol2, *of1, *ol1;     // original triplet first / last (per vertex)
+  int *df2, *dl2, *df1, *dl1;     // decp triplet first / last (per vertex)
+  int *olarg, *osmall, *dlarg, *dsmall;  // per extremum, at its list position
+  int* ext_id;                    // 2*position (maxima) / 2*position+1 (minima), else -1
+  int2* act0;                     // {2*saddle + side, candidate mask}
   uint8_t *wn2, *wn1;             // wrong_neighbors(_ds) flags
   int *wnl2, *wnl1;               // wrong_neighbors(_ds)_index
   int *minsad2, *minsad1;         // greedy reservations
@@ -99,23 +102,31 @@ __global__ void k_f32_to_f64(const float* in, double* out, long long n);
 __global__ void k_check_finite(const double* a, long long n, int* ctr);
 __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* type,
                            uint16_t* lower);
-__global__ void k_check_full(St s, int* front, int fctr, int epoch);
+__global__ void k_check_full(St s, int* front, int fctr);
 __global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int* front,
                              int fctr, int epoch);
 __global__ void k_clear_dirty(St s);
 __global__ void k_propose_c(St s, const int* front, int fctr);
 __global__ void k_apply(St s);
-__global__ void k_steepest(Geo g, const double* f, int* ds, int* as);
-__global__ void k_jump(int n, int* ds, int* as, int* ctr);
-__global__ void k_list_best(const int* list, int n, const double* f, int* slot, int want_max);
+__global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t* lower);
+__global__ void k_jump(int n, int* ds, int* as);
+__global__ void k_chase(int n, int* ds, int* as);
 __global__ void k_list_fill(const int* list, int n, int* arr, int val);
-__global__ void k_saddles(St s, int side, const int* list, int n, const double* f,
-                          const int* L, int* first, int* last, int* slots, int glob_idx,
-                          int* ormap);
-__global__ void k_wn_cand(St s, int n2, int n1, int* active, int actr);
-__global__ void k_wn_register(St s, int n2, const int* active, int actr);
-__global__ void k_wn_commit(St s, int n2, const int* active, int actr);
-__global__ void k_wn_next(St s, int n2, const int* active, int actr, int* next, int nctr);
+__global__ void k_saddle_grid(St s, int dec, const double* f, const int* L2, const int* L1,
+                              const uint16_t* flow, int* F2, int* La2, int* F1, int* La1,
+                              int2* pairs, long long* kmax, long long* kmin, int* bmax,
+                              int* bmin);
+__global__ void k_slot_ids(St s, const int2* pairs, const double* f, const long long* kmax,
+                           const long long* kmin, int* slot_max, int* slot_min);
+__global__ void k_ext_index(St s);
+__global__ void k_fill_pair(long long* a, long long va, int* b, int vb, int n);
+__global__ void k_list_fill64(const int* list, int n, long long* arr, long long val);
+__global__ void k_ext_reduce(const int* inmax, const int* inmin, int n, const double* f,
+                             int* outmax, int* outmin);
+__global__ void k_act_copy(St s, int2* dst, int dctr);
+__global__ void k_wn_register(St s, int2* active, int actr);
+__global__ void k_wn_commit(St s, int2* active, int actr);
+__global__ void k_wn_next(St s, const int2* active, int actr, int2* next, int nctr);
 __global__ void k_wn_reset(St s);
 __global__ void k_fixpath(St s);
 __global__ void k_wrong_index(St s, int side, int phase, int do_fix);

@@ -142,12 +142,14 @@ struct extremz_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   ez::DevBuf<uint8_t> lut;
-  ez::DevBuf<long long> delta;
+  ez::DevBuf<long long> delta, kmax, kmin;
   ez::DevBuf<uint8_t> cnt, otype, dflag, wn2, wn1;
-  ez::DevBuf<uint16_t> olow, cmask;
+  ez::DevBuf<uint16_t> olow, dlow;
   ez::DevBuf<int> oDS, oAS, dDS, dAS, stamp, touched, dirty, frontA, frontB;
-  ez::DevBuf<int> S2, S1, MX, MN, olarg, osmall, dlarg, dsmall, minsad2, minsad1, win_a, win_b;
-  ez::DevBuf<int> ormap2, ormap1, of2, ol2, of1, ol1, df2, dl2, df1, dl1, actA, actB, wnl2, wnl1;
+  ez::DevBuf<int> S2, S1, MX, MN, olarg, osmall, dlarg, dsmall, minsad2, minsad1, win_a, win_b,
+      ext_id;
+  ez::DevBuf<int> of2, ol2, of1, ol1, df2, dl2, df1, dl1, wnl2, wnl1, bmax, bmin, bmax2, bmin2;
+  ez::DevBuf<int2> act0, actA, actB, pairs;
   ez::DevBuf<int> ctr;
   ez::DevBuf<double> orig_tmp, decp_tmp;
   ez::DevBuf<uint8_t> counts_tmp;
@@ -172,7 +174,7 @@ class Run {
   }
 
   int correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
-              extremz_stats* stats, uint8_t* d_counts);
+              uint8_t* d_counts);
   void classify(const double* d_f, int W, int H, int D, uint8_t* d_t, uint16_t* d_l);
   int labels(const double* d_f, int W, int H, int D, int* d_ds, int* d_as);
 
@@ -198,8 +200,11 @@ class Run {
     s_.g.N = W * H * D;
     for (int d = 0; d < kNbr; d++) s_.g.off[d] = kDx[d] + W * (kDy[d] + H * kDz[d]);
   }
-  dim3 grid3() const {
-    return dim3((s_.g.W + 63) / 64, (s_.g.H + 3) / 4, s_.g.D);
+  // 64x4 row blocks (k_classify, k_check_full, k_saddle_grid)
+  int row_blocks() const { return ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4) * s_.g.D; }
+  // 64x4x4 label tiles
+  int tile_blocks() const {
+    return ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4) * ((s_.g.D + 3) / 4);
   }
   static int blocks_for(long long n) {
     long long b = (n + 255) / 256;
@@ -230,18 +235,19 @@ class Run {
   void set_ctr2(int a, int b, int v) { k_ctr_set2<<<1, 64, 0, st()>>>(s_.ctr, a, b, v); }
   void pull() {
     ez::hip_check(hipGetLastError(), "kernel launch");
-    ez::hip_check(hipMemcpyAsync(c_->h_ctr, s_.ctr, sizeof(int) * C_COUNT_, hipMemcpyDeviceToHost,
-                             st()),
-              "hipMemcpyAsync");
+    ez::hip_check(hipMemcpyAsync(c_->h_ctr, s_.ctr, sizeof(int) * C_COUNT_,
+                                 hipMemcpyDeviceToHost, st()),
+                  "hipMemcpyAsync");
     ez::hip_check(hipStreamSynchronize(st()), "hipStreamSynchronize");
     c_->host_syncs++;
   }
   int h(int idx) const { return c_->h_ctr[idx]; }
 
   void allocate(long long N);
-  void alloc_saddles();
+  void slot_pass(int dec, const double* f, const int* L2, const int* L1, const uint16_t* flow,
+                 int* F2, int* La2, int* F1, int* La1, int* smax, int* smin);
   void apply();
-  void steepest_labels(const double* f, int* ds, int* as);
+  void label_pass(const double* f, int* ds, int* as, uint16_t* lower);
   void c_loops();
   void cp_dec(bool force);
   void wrong_neighbors();
@@ -253,45 +259,52 @@ class Run {
 void Run::allocate(long long N) {
   extremz_ctx* c = c_;
   c->delta.ensure(N);
-  c->cnt.ensure(N);
-  c->otype.ensure(N);
-  c->dflag.ensure(N);
-  c->wn2.ensure(N);
-  c->wn1.ensure(N);
+  c->kmax.ensure(N);
+  c->kmin.ensure(N);
+  for (DevBuf<uint8_t>* b : {&c->cnt, &c->otype, &c->dflag, &c->wn2, &c->wn1}) b->ensure(N);
   c->olow.ensure(N);
+  c->dlow.ensure(N);
   for (DevBuf<int>* b : {&c->oDS, &c->oAS, &c->dDS, &c->dAS, &c->stamp, &c->touched, &c->dirty,
                          &c->frontA, &c->frontB, &c->S2, &c->S1, &c->MX, &c->MN, &c->olarg,
-                         &c->osmall, &c->dlarg, &c->dsmall, &c->minsad2, &c->minsad1,
-                         &c->win_a, &c->win_b})
+                         &c->osmall, &c->dlarg, &c->dsmall, &c->minsad2, &c->minsad1, &c->ext_id,
+                         &c->win_a, &c->win_b, &c->of2, &c->ol2, &c->of1, &c->ol1, &c->df2,
+                         &c->dl2, &c->df1, &c->dl1, &c->wnl2, &c->wnl1})
     b->ensure(N);
+  c->bmax.ensure(row_blocks());
+  c->bmin.ensure(row_blocks());
+  c->bmax2.ensure(256);
+  c->bmin2.ensure(256);
+  c->act0.ensure(2 * N);
+  c->actA.ensure(2 * N);
+  c->actB.ensure(2 * N);
   c->ctr.ensure(C_COUNT_);
 }
 
-void Run::alloc_saddles() {
-  extremz_ctx* c = c_;
-  long long n2 = std::max(1, s_.n2), n1 = std::max(1, s_.n1);
-  c->ormap2.ensure(n2 * kNbr);
-  c->ormap1.ensure(n1 * kNbr);
-  for (DevBuf<int>* b : {&c->of2, &c->ol2, &c->df2, &c->dl2}) b->ensure(n2);
-  for (DevBuf<int>* b : {&c->of1, &c->ol1, &c->df1, &c->dl1}) b->ensure(n1);
-  c->cmask.ensure(n2 + n1);
-  c->actA.ensure(n2 + n1);
-  c->actB.ensure(n2 + n1);
-  c->wnl2.ensure(s_.g.N);
-  c->wnl1.ensure(s_.g.N);
-  s_.ormap2 = c->ormap2.p;
-  s_.ormap1 = c->ormap1.p;
-  s_.of2 = c->of2.p;
-  s_.ol2 = c->ol2.p;
-  s_.df2 = c->df2.p;
-  s_.dl2 = c->dl2.p;
-  s_.of1 = c->of1.p;
-  s_.ol1 = c->ol1.p;
-  s_.df1 = c->df1.p;
-  s_.dl1 = c->dl1.p;
-  s_.cmask = c->cmask.p;
-  s_.wnl2 = c->wnl2.p;
-  s_.wnl1 = c->wnl1.p;
+// computeCP + triplets + largest/smallest saddle maps of field f in one
+// full-grid pass (plus the slot tie-break pass); with dec, also the
+// wrong-neighbour candidates and decp's global maximum / minimum.
+void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, const uint16_t* flow,
+                    int* F2, int* La2, int* F1, int* La1, int* smax, int* smin) {
+  tbegin(EXTREMZ_K_SADDLE);
+  long long* kmax = c_->kmax.p;
+  long long* kmin = c_->kmin.p;
+  k_fill_pair<<<blocks_for(s_.nmax), 256, 0, st()>>>(kmax, (long long)0x8000000000000000ULL, smax,
+                                                     kNone, s_.nmax);
+  k_fill_pair<<<blocks_for(s_.nmin), 256, 0, st()>>>(kmin, 0x7fffffffffffffffLL, smin, 0x7fffffff,
+                                                     s_.nmin);
+  set_ctr2(C_ACT0, C_PAIRS, 0);
+  const int rb = row_blocks();
+  k_saddle_grid<<<rb, dim3(64, 4, 1), 0, st()>>>(s_, dec, f, L2, L1, flow, F2, La2, F1, La1,
+                                                  c_->pairs.p, kmax, kmin, c_->bmax.p,
+                                                  c_->bmin.p);
+  k_slot_ids<<<blocks_for(14LL * (s_.n2 + s_.n1)), 256, 0, st()>>>(s_, c_->pairs.p, f, kmax, kmin,
+                                                                   smax, smin);
+  if (dec) {
+    k_ext_reduce<<<256, 256, 0, st()>>>(c_->bmax.p, c_->bmin.p, rb, f, c_->bmax2.p, c_->bmin2.p);
+    k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, c_->bmin2.p, 256, f, s_.ctr + C_DGMAX,
+                                      s_.ctr + C_DGMIN);
+  }
+  tend();
 }
 
 // apply_delta over the touched list, then the list is emptied.
@@ -302,26 +315,22 @@ void Run::apply() {
   set_ctr(C_TOUCHED, 0);
 }
 
-// ComputeDescendingManifold + ComputeAscendingManifold on field f (D7).
-void Run::steepest_labels(const double* f, int* ds, int* as) {
-  tbegin(EXTREMZ_K_STEEPEST);
-  k_steepest<<<grid3(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as);
-  tend();
+// ComputeDescendingManifold + ComputeAscendingManifold on field f (D7): tile
+// resolution, two jumping rounds, then a chase of whatever is left.  No host
+// round trip.
+void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower) {
   counts_[EXTREMZ_STAT_LABEL_PASSES]++;
+  tbegin(EXTREMZ_K_STEEPEST);
+  k_labels_tile<<<tile_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as, lower);
+  tend();
   int nb = (s_.g.N + 255) / 256;
-  for (int guard = 0; guard < 64; guard++) {
-    tbegin(EXTREMZ_K_JUMP);
-    k_jump<<<nb, 256, 0, st()>>>(s_.g.N, ds, as, s_.ctr);
-    tend();
-    set_ctr(C_CHANGED, 0);
-    tbegin(EXTREMZ_K_JUMP);
-    k_jump<<<nb, 256, 0, st()>>>(s_.g.N, ds, as, s_.ctr);
-    tend();
-    counts_[EXTREMZ_STAT_JUMP_ROUNDS] += 2;
-    pull();
-    if (!h(C_CHANGED)) return;
-  }
-  throw HipError("pointer jumping did not converge");
+  tbegin(EXTREMZ_K_JUMP);
+  k_jump<<<nb, 256, 0, st()>>>(s_.g.N, ds, as);
+  tend();
+  tbegin(EXTREMZ_K_JUMP);
+  k_chase<<<nb, 256, 0, st()>>>(s_.g.N, ds, as);
+  tend();
+  counts_[EXTREMZ_STAT_JUMP_ROUNDS] += 2;
 }
 
 // c_loops, src/ExtreM.cpp:1505-1583.  The first call scans the full grid; later
@@ -335,7 +344,7 @@ void Run::c_loops() {
   epoch_++;
   if (!clean_) {
     tbegin(EXTREMZ_K_CHECK_FULL);
-    k_check_full<<<grid3(), dim3(64, 4, 1), 0, st()>>>(s_, front_buf_, front_, epoch_);
+    k_check_full<<<row_blocks(), dim3(64, 4, 1), 0, st()>>>(s_, front_buf_, front_);
     tend();
   } else {
     tbegin(EXTREMZ_K_CHECK_INCR);
@@ -372,57 +381,43 @@ void Run::c_loops() {
   clean_ = true;
 }
 
-// Manifolds + computeCP + largest/smallest saddle maps of decp
-// (main.cpp:290-304).  Skipped when decp has not changed since the last call:
-// every structure it computes depends on decp only.
+// Manifolds + computeCP + largest/smallest saddle maps + wrong-neighbour
+// candidates of decp (main.cpp:290-304 and the comparisons of 313-323).
+// Skipped when decp has not changed since the last call: everything it
+// computes depends on decp only.
 void Run::cp_dec(bool force) {
   if (!force) {
     pull();
     if (!h(C_ANY_CHANGE)) return;
   }
-  steepest_labels(s_.decp, s_.dDS, s_.dAS);
-  set_ctr2(C_DGMAX, C_DGMIN, kNone);
-  tbegin(EXTREMZ_K_SADDLE);
-  k_list_best<<<blocks_for(s_.nmax), 256, 0, st()>>>(s_.MX, s_.nmax, s_.decp, s_.ctr + C_DGMAX, 1);
-  k_list_best<<<blocks_for(s_.nmin), 256, 0, st()>>>(s_.MN, s_.nmin, s_.decp, s_.ctr + C_DGMIN, 0);
-  k_list_fill<<<blocks_for(s_.nmax), 256, 0, st()>>>(s_.MX, s_.nmax, s_.dlarg, kNone);
-  k_list_fill<<<blocks_for(s_.nmin), 256, 0, st()>>>(s_.MN, s_.nmin, s_.dsmall, kNone);
-  if (s_.n2)
-    k_saddles<<<blocks_for(s_.n2), 256, 0, st()>>>(s_, 2, s_.S2, s_.n2, s_.decp, s_.dDS, s_.df2,
-                                                    s_.dl2, s_.dlarg, C_DGMAX, nullptr);
-  if (s_.n1)
-    k_saddles<<<blocks_for(s_.n1), 256, 0, st()>>>(s_, 1, s_.S1, s_.n1, s_.decp, s_.dAS, s_.df1,
-                                                    s_.dl1, s_.dsmall, C_DGMIN, nullptr);
-  tend();
+  label_pass(s_.decp, s_.dDS, s_.dAS, s_.dlow);
+  slot_pass(1, s_.decp, s_.dDS, s_.dAS, s_.dlow, s_.df2, s_.dl2, s_.df1, s_.dl1, s_.dlarg,
+            s_.dsmall);
   set_ctr(C_ANY_CHANGE, 0);
 }
 
 // get_wrong_split_neighbors / get_wrong_join_neighbors over all saddles in
-// saddle-id order (main.cpp:309-323), as rounds of deterministic reservations.
+// saddle-id order (main.cpp:309-323), as rounds of deterministic reservations
+// on the candidates of the last CP pass.
 void Run::wrong_neighbors() {
   k_wn_reset<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
   set_ctr2(C_NFC, C_NFC1, 0);
-  set_ctr2(C_ACT_A, C_ACT_B, 0);
-  int* act = c_->actA.p;
-  int* nxt = c_->actB.p;
+  int2* act = c_->actA.p;
+  int2* nxt = c_->actB.p;
   int actr = C_ACT_A, nctr = C_ACT_B;
-  int tot = s_.n2 + s_.n1;
-  if (tot == 0) {
-    pull();
-    return;
-  }
-  k_wn_cand<<<blocks_for(tot), 256, 0, st()>>>(s_, s_.n2, s_.n1, act, actr);
-  // each round commits at least the lowest active saddle, so rounds <= tot
-  for (int round = 0;; round++) {
+  k_act_copy<<<blocks_for(2LL * s_.g.N), 256, 0, st()>>>(s_, act, actr);
+  long long guard = (long long)s_.n2 + s_.n1 + 2;
+  for (long long round = 0;; round++) {
     pull();
     int na = h(actr);
     if (na == 0) break;
-    if (round > tot) throw HipError("wrong-neighbour reservations did not finish");
+    // each round commits at least the lowest active saddle
+    if (round > guard) throw HipError("wrong-neighbour reservations did not finish");
     set_ctr(nctr, 0);
     int b = blocks_for(na);
-    k_wn_register<<<b, 256, 0, st()>>>(s_, s_.n2, act, actr);
-    k_wn_commit<<<b, 256, 0, st()>>>(s_, s_.n2, act, actr);
-    k_wn_next<<<b, 256, 0, st()>>>(s_, s_.n2, act, actr, nxt, nctr);
+    k_wn_register<<<b, 256, 0, st()>>>(s_, act, actr);
+    k_wn_commit<<<b, 256, 0, st()>>>(s_, act, actr);
+    k_wn_next<<<b, 256, 0, st()>>>(s_, act, actr, nxt, nctr);
     std::swap(act, nxt);
     std::swap(actr, nctr);
   }
@@ -473,7 +468,7 @@ void Run::classify(const double* d_f, int W, int H, int D, uint8_t* d_t, uint16_
   setup_geo(W, H, D);
   if (c_->lut.p == nullptr) throw HipError("context has no LUT");
   tbegin(EXTREMZ_K_CLASSIFY);
-  k_classify<<<grid3(), dim3(64, 4, 1), 0, st()>>>(s_.g, c_->lut.p, d_f, d_t, d_l);
+  k_classify<<<row_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, c_->lut.p, d_f, d_t, d_l);
   tend();
   ez::hip_check(hipGetLastError(), "k_classify");
   ez::hip_check(hipStreamSynchronize(st()), "sync");
@@ -483,12 +478,14 @@ int Run::labels(const double* d_f, int W, int H, int D, int* d_ds, int* d_as) {
   setup_geo(W, H, D);
   c_->ctr.ensure(C_COUNT_);
   s_.ctr = c_->ctr.p;
-  steepest_labels(d_f, d_ds, d_as);
+  label_pass(d_f, d_ds, d_as, nullptr);
+  ez::hip_check(hipGetLastError(), "labels");
+  ez::hip_check(hipStreamSynchronize(st()), "sync");
   return 0;
 }
 
 int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
-                 extremz_stats* stats, uint8_t* d_counts) {
+                 uint8_t* d_counts) {
   setup_geo(W, H, D);
   const long long N = s_.g.N;
   allocate(N);
@@ -507,6 +504,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   s_.cnt = d_counts ? d_counts : c->cnt.p;
   s_.otype = c->otype.p;
   s_.olow = c->olow.p;
+  s_.dlow = c->dlow.p;
   s_.oDS = c->oDS.p;
   s_.oAS = c->oAS.p;
   s_.dDS = c->dDS.p;
@@ -520,12 +518,24 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   s_.S1 = c->S1.p;
   s_.MX = c->MX.p;
   s_.MN = c->MN.p;
+  s_.of2 = c->of2.p;
+  s_.ol2 = c->ol2.p;
+  s_.of1 = c->of1.p;
+  s_.ol1 = c->ol1.p;
+  s_.df2 = c->df2.p;
+  s_.dl2 = c->dl2.p;
+  s_.df1 = c->df1.p;
+  s_.dl1 = c->dl1.p;
   s_.olarg = c->olarg.p;
   s_.osmall = c->osmall.p;
   s_.dlarg = c->dlarg.p;
   s_.dsmall = c->dsmall.p;
+  s_.act0 = c->act0.p;
+  s_.ext_id = c->ext_id.p;
   s_.wn2 = c->wn2.p;
   s_.wn1 = c->wn1.p;
+  s_.wnl2 = c->wnl2.p;
+  s_.wnl1 = c->wnl1.p;
   s_.minsad2 = c->minsad2.p;
   s_.minsad1 = c->minsad1.p;
   s_.win_a = c->win_a.p;
@@ -544,7 +554,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   ez::hip_check(hipMemsetAsync(s_.wn1, 0, N, q), "memset");
   for (int* p : {s_.minsad2, s_.minsad1, s_.win_a, s_.win_b})
     k_fill_i32<<<bN, 256, 0, q>>>(p, 0x7fffffff, N);
-  for (int* p : {s_.olarg, s_.osmall, s_.dlarg, s_.dsmall}) k_fill_i32<<<bN, 256, 0, q>>>(p, kNone, N);
+  k_fill_i32<<<bN, 256, 0, q>>>(s_.ext_id, -1, N);
   set_ctr2(C_GMIN, C_GMAX, kNone);
   pull();
   if (h(C_FAIL)) return EXTREMZ_EINVAL;
@@ -553,9 +563,9 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   k_argext<<<bN, 256, 0, q>>>(d_orig, N, s_.ctr + C_GMIN, s_.ctr + C_GMAX);
   // main.cpp:85-89
   tbegin(EXTREMZ_K_CLASSIFY);
-  k_classify<<<grid3(), dim3(64, 4, 1), 0, q>>>(s_.g, s_.lut, d_orig, s_.otype, s_.olow);
+  k_classify<<<row_blocks(), dim3(64, 4, 1), 0, q>>>(s_.g, s_.lut, d_orig, s_.otype, s_.olow);
   tend();
-  steepest_labels(d_orig, s_.oDS, s_.oAS);
+  label_pass(d_orig, s_.oDS, s_.oAS, nullptr);
   // main.cpp:100-119: CP lists, triplets, largest/smallest saddle maps of orig
   k_lists<<<(N + 255) / 256, 256, 0, q>>>(s_);
   pull();
@@ -567,15 +577,11 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   counts_[EXTREMZ_STAT_N_SADDLE1] = s_.n1;
   counts_[EXTREMZ_STAT_N_MAX] = s_.nmax;
   counts_[EXTREMZ_STAT_N_MIN] = s_.nmin;
-  alloc_saddles();
-  tbegin(EXTREMZ_K_SADDLE);
-  if (s_.n2)
-    k_saddles<<<blocks_for(s_.n2), 256, 0, q>>>(s_, 2, s_.S2, s_.n2, d_orig, s_.oDS, s_.of2,
-                                                 s_.ol2, s_.olarg, C_GMAX, s_.ormap2);
-  if (s_.n1)
-    k_saddles<<<blocks_for(s_.n1), 256, 0, q>>>(s_, 1, s_.S1, s_.n1, d_orig, s_.oAS, s_.of1,
-                                                 s_.ol1, s_.osmall, C_GMIN, s_.ormap1);
-  tend();
+  // every saddle records at most one pair per distinct root of its star
+  c->pairs.ensure(14LL * (s_.n2 + s_.n1) + 1);
+  k_ext_index<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, q>>>(s_);
+  slot_pass(0, d_orig, s_.oDS, s_.oAS, s_.olow, s_.of2, s_.ol2, s_.of1, s_.ol1, s_.olarg,
+            s_.osmall);
   // main.cpp:74-75, 122-131
   k_presnap<<<blocks_for(s_.n2 + s_.n1 + 2), 256, 0, q>>>(s_);
 
@@ -583,20 +589,17 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   cp_dec(true);  // main.cpp:172-191
   wrong_neighbors();  // main.cpp:210-226
   int nfc = h(C_NFC), nfc1 = h(C_NFC1);
-  int wmc = 0, wminc = 0;
   if (nfc == 0) {  // main.cpp:230-254
     cp_dec(false);
     s_loops(2, 1);
     pull();
-    wmc = h(C_WMC);
-    if (wmc == 0) saddle_loops(2, 1);
+    if (h(C_WMC) == 0) saddle_loops(2, 1);
   }
   if (nfc1 == 0) {  // main.cpp:255-274
     cp_dec(false);
     s_loops(1, 1);
     pull();
-    wminc = h(C_WMINC);
-    if (wminc == 0) saddle_loops(1, 1);
+    if (h(C_WMINC) == 0) saddle_loops(1, 1);
   }
   auto any = [&]() {
     pull();
@@ -639,7 +642,6 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   k_count_edited<<<bN, 256, 0, q>>>(s_.cnt, N, s_.ctr);
   pull();
   counts_[EXTREMZ_STAT_EDITED] = h(C_EDITED);
-  (void)stats;
   return 0;
 }
 
@@ -678,17 +680,21 @@ static void free_ctx(extremz_ctx* c) {
   }
   for (ez::DevBuf<int>* b :
        {&c->oDS, &c->oAS, &c->dDS, &c->dAS, &c->stamp, &c->touched, &c->dirty, &c->frontA,
-        &c->frontB, &c->S2, &c->S1, &c->MX, &c->MN, &c->olarg, &c->osmall, &c->dlarg, &c->dsmall,
-        &c->minsad2, &c->minsad1, &c->win_a, &c->win_b, &c->ormap2, &c->ormap1, &c->of2, &c->ol2,
-        &c->of1, &c->ol1, &c->df2, &c->dl2, &c->df1, &c->dl1, &c->actA, &c->actB, &c->wnl2,
-        &c->wnl1, &c->ctr})
+        &c->frontB, &c->S2, &c->S1, &c->MX, &c->MN, &c->olarg, &c->osmall, &c->dlarg,
+        &c->dsmall, &c->minsad2, &c->minsad1, &c->win_a, &c->win_b, &c->ext_id, &c->of2, &c->ol2,
+        &c->of1,
+        &c->ol1, &c->df2, &c->dl2, &c->df1, &c->dl1, &c->wnl2, &c->wnl1, &c->bmax, &c->bmin,
+        &c->bmax2, &c->bmin2, &c->ctr})
     b->release();
   for (ez::DevBuf<uint8_t>* b : {&c->lut, &c->cnt, &c->otype, &c->dflag, &c->wn2, &c->wn1,
                                  &c->counts_tmp})
     b->release();
+  for (ez::DevBuf<int2>* b : {&c->act0, &c->actA, &c->actB, &c->pairs}) b->release();
   c->delta.release();
+  c->kmax.release();
+  c->kmin.release();
   c->olow.release();
-  c->cmask.release();
+  c->dlow.release();
   c->orig_tmp.release();
   c->decp_tmp.release();
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
@@ -717,9 +723,9 @@ static void fill_stats(extremz_ctx* c, const Run& r, int status, double wall_ms,
 }
 
 static bool valid_shape(int W, int H, int D) {
-  if (W <= 0 || H <= 0 || D <= 0 || D > 65535) return false;
+  if (W <= 0 || H <= 0 || D <= 0) return false;
   long long N = (long long)W * H * D;
-  return N < (1LL << 31) - 64;
+  return N <= (1LL << 30);  // entries 2*v + side must fit an int
 }
 
 // Runs fn with error translation; fn returns a status.
@@ -758,7 +764,7 @@ static int run_correct(extremz_ctx* c, const double* d_orig, double* d_decp, int
   ez::hip_check(hipSetDevice(c->device), "hipSetDevice");
   c->host_syncs = 0;
   Run r(c, opts);
-  int status = guarded([&] { return r.correct(d_orig, d_decp, W, H, D, xi, stats, d_counts); });
+  int status = guarded([&] { return r.correct(d_orig, d_decp, W, H, D, xi, d_counts); });
   double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   fill_stats(c, r, status, ms, stats);
   return status;

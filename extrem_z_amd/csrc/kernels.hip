@@ -10,6 +10,11 @@
 namespace ez {
 
 __constant__ int c_dx[kNbr], c_dy[kNbr], c_dz[kNbr];
+// The same stencil as compile-time tables, so unrolled loops get immediates
+// (src/ExtreM_cuda.cu:155-162).
+__device__ constexpr int kSX[kNbr] = {1, -1, 0, 0, 0, 0, -1, 1, 0, 0, -1, 1, -1, 1};
+__device__ constexpr int kSY[kNbr] = {0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1};
+__device__ constexpr int kSZ[kNbr] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1};
 __constant__ int c_cls_key[64];
 __constant__ int c_lut_off[64];
 
@@ -64,6 +69,36 @@ __device__ __forceinline__ void wave_append(int* list, int* counter, bool pred, 
   if (lane == leader) base = atomicAdd(counter, __popcll(m));
   base = __shfl(base, leader);
   if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Returns a logical block index such that every XCD walks one contiguous
+// range of tiles (so z +- 1 neighbour rows are in its own L2).
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Appends val when pred with one atomic per 256-thread block; every thread
+// of the block must call.
+template <class T>
+__device__ __forceinline__ void block_append(T* list, int* counter, bool pred, T val) {
+  __shared__ int s_cnt[4];
+  __shared__ int s_base;
+  int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  int wave = tid >> 6, lane = tid & 63;
+  unsigned long long m = __ballot(pred);
+  if (lane == 0) s_cnt[wave] = __popcll(m);
+  __syncthreads();
+  if (tid == 0) {
+    int tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    s_base = tot ? atomicAdd(counter, tot) : 0;
+  }
+  __syncthreads();
+  int off = s_base;
+  for (int w = 0; w < wave; w++) off += s_cnt[w];
+  if (pred) list[off + __popcll(m & ((1ull << lane) - 1ull))] = val;
+  __syncthreads();
 }
 
 // Lower-star mask (direction space), compact upper config and the vertex type
@@ -140,16 +175,78 @@ __device__ __forceinline__ bool mismatch_at(const St& s, int v) {
   return dt != ot || is_saddle_type(ot);
 }
 
+// A list append staged in LDS: each push costs one LDS atomic; flush() moves
+// the block's entries to the global list with one global atomic.  Entries
+// beyond K go straight to the global list.  Every thread of the block must
+// reach flush().
+template <class T, int K>
+struct StagedList {
+  T* buf;
+  int* n;
+  int* base;
+  T* glist;
+  int* gcount;
+  // Callable from divergent code: the active lanes of the wavefront share
+  // one LDS atomic.
+  __device__ __forceinline__ void push(T v) {
+    unsigned long long m = __ballot(1);
+    int lane = __lane_id();
+    int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(n, __popcll(m));
+    base = __shfl(base, leader);
+    int i = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (i < K) buf[i] = v;
+    else glist[atomicAdd(gcount, 1)] = v;
+  }
+  __device__ __forceinline__ void flush() {
+    __syncthreads();
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nt = blockDim.x * blockDim.y;
+    const int cnt = *n < K ? *n : K;
+    if (tid == 0) *base = cnt ? atomicAdd(gcount, cnt) : 0;
+    __syncthreads();
+    for (int i = tid; i < cnt; i += nt) glist[*base + i] = buf[i];
+  }
+};
+#define STAGED_LIST(name, T, K, glist, gcount)                         \
+  __shared__ T name##_buf[K];                                          \
+  __shared__ int name##_n, name##_base;                                \
+  if (threadIdx.x == 0 && threadIdx.y == 0) name##_n = 0;              \
+  __syncthreads();                                                     \
+  StagedList<T, K> name{name##_buf, &name##_n, &name##_base, (glist), (gcount)}
+
+// A counter staged in LDS (one global atomic per block at flush()).
+struct StagedCount {
+  int* n;
+  int* gcount;
+  // Adds one per calling lane; callable from divergent code (one LDS atomic
+  // per wavefront).
+  __device__ __forceinline__ void inc() {
+    unsigned long long m = __ballot(1);
+    if (__lane_id() == __ffsll((long long)m) - 1) atomicAdd(n, __popcll(m));
+  }
+  __device__ __forceinline__ void flush() {
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0 && *n) atomicAdd(gcount, *n);
+  }
+};
+#define STAGED_COUNT(name, gcount)                                     \
+  __shared__ int name##_n;                                             \
+  if (threadIdx.x == 0 && threadIdx.y == 0) name##_n = 0;              \
+  __syncthreads();                                                     \
+  StagedCount name{&name##_n, (gcount)}
+
+constexpr int kStage = 1024;
+using Touched = StagedList<int, kStage>;
+
 // swap + the caller's `d > oldValue` gate, sequentially a max (D5).  The
 // first proposal above the sentinel lists the vertex as touched.
-__device__ __forceinline__ void propose(const St& s, int t, double d) {
+__device__ __forceinline__ void propose(const St& s, int t, double d, Touched& tl) {
   long long k = okey(d);
   if (k <= s.sent) return;
   long long old = atomicMax(&s.delta[t], k);
-  if (old == s.sent) {
-    int i = atomicAdd(&s.ctr[C_TOUCHED], 1);
-    s.touched[i] = t;
-  }
+  if (old == s.sent) tl.push(t);
 }
 __device__ __forceinline__ double lower_mid(const St& s, int v) {
   return ((s.orig[v] - s.xi) + s.decp[v]) / 2.0 - s.decp[v];
@@ -194,32 +291,44 @@ __global__ void k_check_finite(const double* a, long long n, int* ctr) {
 }
 
 // ---------------------------------------------------- classification ----
-// Full-grid classification of a field: block (64, 4), grid (ceil(W/64),
-// ceil(H/4), D).  classifyAllVertices, src/ExtreM.cpp:1088-1097.
+// Row blocks: 64x4 voxels per 256-thread block (one 64-voxel row per
+// wavefront), 1-D grid in XCD-aware order.
+struct RowPos {
+  int x, y, z;
+  bool in;
+};
+__device__ __forceinline__ RowPos row_pos(const Geo& g) {
+  const int nbx = (g.W + 63) / 64, nby = (g.H + 3) / 4;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  RowPos p;
+  p.x = (L % nbx) * 64 + threadIdx.x;
+  p.y = ((L / nbx) % nby) * 4 + threadIdx.y;
+  p.z = L / (nbx * nby);
+  p.in = p.x < g.W && p.y < g.H;
+  return p;
+}
+
+// Full-grid classification of a field (classifyAllVertices,
+// src/ExtreM.cpp:1088-1097).
 __global__ void __launch_bounds__(256) k_classify(Geo g, const uint8_t* lut, const double* f,
                                                   uint8_t* type, uint16_t* lower) {
-  int x = blockIdx.x * 64 + threadIdx.x;
-  int y = blockIdx.y * 4 + threadIdx.y;
-  int z = blockIdx.z;
-  if (x >= g.W || y >= g.H) return;
-  int v = x + g.W * (y + g.H * z);
+  RowPos p = row_pos(g);
+  if (!p.in) return;
+  int v = p.x + g.W * (p.y + g.H * p.z);
   unsigned lm;
-  int t = classify_at(g, lut, f, v, class_of(g, x, y, z), &lm);
+  int t = classify_at(g, lut, f, v, class_of(g, p.x, p.y, p.z), &lm);
   type[v] = (uint8_t)t;
   lower[v] = (uint16_t)lm;
 }
 
 // Full-grid false-critical-point scan (the first get_false_criticle_points
 // of c_loops, src/ExtreM.cpp:1513-1519); appends to the frontier.
-__global__ void __launch_bounds__(256) k_check_full(St s, int* front, int fctr, int epoch) {
-  int x = blockIdx.x * 64 + threadIdx.x;
-  int y = blockIdx.y * 4 + threadIdx.y;
-  int z = blockIdx.z;
-  bool in = x < s.g.W && y < s.g.H;
-  int v = in ? x + s.g.W * (y + s.g.H * z) : 0;
+__global__ void __launch_bounds__(256) k_check_full(St s, int* front, int fctr) {
+  RowPos p = row_pos(s.g);
+  int v = p.in ? p.x + s.g.W * (p.y + s.g.H * p.z) : 0;
   bool mis = false;
-  if (in) {
-    int cls = class_of(s.g, x, y, z);
+  if (p.in) {
+    int cls = class_of(s.g, p.x, p.y, p.z);
     unsigned dl = lower_mask_at(s.g, s.decp, v, cls);
     unsigned ol = s.olow[v];
     if (dl != ol) {
@@ -228,8 +337,7 @@ __global__ void __launch_bounds__(256) k_check_full(St s, int* front, int fctr, 
       mis = dt != ot || is_saddle_type(ot);
     }
   }
-  (void)epoch;
-  wave_append(front, &s.ctr[fctr], mis, v);
+  block_append(front, &s.ctr[fctr], mis, v);
 }
 
 // Re-check after edits: every list entry (and, with nbrs, its 14 neighbours)
@@ -240,6 +348,7 @@ __global__ void __launch_bounds__(256) k_check_full(St s, int* front, int fctr, 
 __global__ void __launch_bounds__(256) k_check_list(St s, const int* list, int lctr,
                                                     int with_nbrs, int* front, int fctr,
                                                     int epoch) {
+  STAGED_LIST(fl, int, kStage, front, &s.ctr[fctr]);
   int n = s.ctr[lctr];
   GRID_STRIDE(i, n) {
     int v = list[i];
@@ -256,12 +365,10 @@ __global__ void __launch_bounds__(256) k_check_list(St s, const int* list, int l
     for (int j = 0; j < m; j++) {
       int u = cand[j];
       if (atomicExch(&s.stamp[u], epoch) == epoch) continue;
-      if (mismatch_at(s, u)) {
-        int p = atomicAdd(&s.ctr[fctr], 1);
-        front[p] = u;
-      }
+      if (mismatch_at(s, u)) fl.push(u);
     }
   }
+  fl.flush();
 }
 
 // Clears the dirty flags of the listed vertices and the list counter.
@@ -274,6 +381,7 @@ __global__ void k_clear_dirty(St s) {
 // c_loop (src/ExtreM.cpp:1152-1236) for false maxima / minima and
 // get_false_stars (src/ExtreM.cpp:1412-1445) for saddle-list entries.
 __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int fctr) {
+  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[C_TOUCHED]);
   int n = s.ctr[fctr];
   GRID_STRIDE(i, n) {
     int v = front[i];
@@ -287,7 +395,7 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
     int ot = s.otype[v];
     if ((dt == 4) != (ot == 4)) {  // all_max, direction 0
       if (ot != 4) {
-        propose(s, v, lower_mid(s, v));
+        propose(s, v, lower_mid(s, v), tl);
       } else {
         int largest = v;
         double fl = s.decp[v];
@@ -301,7 +409,7 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
           }
         }
         double fv = s.decp[v];
-        if (!(fv > fl || (fv == fl && v > largest))) propose(s, largest, lower_mid(s, largest));
+        if (!(fv > fl || (fv == fl && v > largest))) propose(s, largest, lower_mid(s, largest), tl);
       }
     }
     if ((dt == 0) != (ot == 0)) {  // all_min, direction 1
@@ -319,9 +427,9 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
         }
         double dv = s.decp[v], ds = s.decp[smallest];
         if (!(dv > ds || (dv == ds && v > smallest)))
-          propose(s, smallest, lower_mid(s, smallest));
+          propose(s, smallest, lower_mid(s, smallest), tl);
       } else {
-        propose(s, v, lower_mid(s, v));
+        propose(s, v, lower_mid(s, v), tl);
       }
     }
     bool sad = (dt != ot) ? (is_saddle_type(dt) || is_saddle_type(ot))
@@ -331,26 +439,28 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
       for (int d = 0; d < kNbr; d++)
         if ((a >> d) & 1) {
           int r = v + s.g.off[d];
-          propose(s, r, (s.orig[r] - s.xi) - s.decp[r]);
+          propose(s, r, (s.orig[r] - s.xi) - s.decp[r], tl);
         }
-      if (dl & ~ol) propose(s, v, (s.orig[v] - s.xi) - s.decp[v]);
+      if (dl & ~ol) propose(s, v, (s.orig[v] - s.xi) - s.decp[v], tl);
     }
   }
+  tl.flush();
 }
 
 // apply_delta, split (src/ExtreM.cpp:1486-1503), over the touched list only;
 // resets the delta slot and records changed vertices as dirty.
 __global__ void __launch_bounds__(256) k_apply(St s) {
+  STAGED_LIST(dl, int, kStage, s.dirty, &s.ctr[C_DIRTY]);
   int n = s.ctr[C_TOUCHED];
   GRID_STRIDE(i, n) {
     int t = s.touched[i];
-    double dl = okey_inv(s.delta[t]);
+    double dd = okey_inv(s.delta[t]);
     s.delta[t] = s.sent;
     double o = s.orig[t], x = s.decp[t];
     int c = s.cnt[t];
     double nx;
-    if (fabs(dl) > 1e-15 && c < 5 && fabs(o - x - dl) <= s.xi) {
-      nx = x + dl;
+    if (fabs(dd) > 1e-15 && c < 5 && fabs(o - x - dd) <= s.xi) {
+      nx = x + dd;
       c += 1;
     } else {
       nx = o - s.xi;
@@ -362,63 +472,203 @@ __global__ void __launch_bounds__(256) k_apply(St s) {
       s.ctr[C_ANY_CHANGE] = 1;
       if (!s.dflag[t]) {
         s.dflag[t] = 1;
-        int p = atomicAdd(&s.ctr[C_DIRTY], 1);
-        s.dirty[p] = t;
+        dl.push(t);
       }
     }
   }
+  dl.flush();
 }
 
 // ----------------------------------------------------- manifold labels ----
-// Steepest ascent (ds) and descent (as) pointer of every vertex, first loops
-// of src/ExtreM.cpp:146-157 and 504-515.
-__global__ void __launch_bounds__(256) k_steepest(Geo g, const double* f, int* ds, int* as) {
-  int x = blockIdx.x * 64 + threadIdx.x;
-  int y = blockIdx.y * 4 + threadIdx.y;
-  int z = blockIdx.z;
-  if (x >= g.W || y >= g.H) return;
-  int v = x + g.W * (y + g.H * z);
-  int key = c_cls_key[class_of(g, x, y, z)];
-  double c = f[v];
-  int up = v, dn = v;
-  double fu = c, fd = c;
+// Tiles of 64x4x4 voxels, marched along z through a ring of three LDS planes
+// (66x6 values each, halo included; the next plane is prefetched into
+// registers while the current one is processed).  Steepest ascent / descent
+// pointers (first loops of src/ExtreM.cpp:146-157 and 504-515) are computed
+// from LDS and resolved inside the tile by pointer doubling; a pointer then
+// names the root or the first vertex outside the tile on the steepest path.
+// With lower != nullptr the lower-star mask of every voxel is written too.
+//
+// Fast path: for an interior voxel of a field with W >= 3 the 15 candidate
+// ids (self + 14 neighbours) have the fixed order of their offsets
+//   d9 < d13 < d5 < d11 < d3 < d7 < d1 < self < d0 < d6 < d2 < d10 < d4 < d12 < d8
+// so every (value, id) comparison of src/ExtreM.cpp:153 / 511 is one compare:
+// scanning in id order, ">=" keeps the highest (value, id), "<" the lowest.
+constexpr int kLTX = 64, kLTY = 4, kLTZ = 4, kLTN = kLTX * kLTY * kLTZ;
+constexpr int kPX = kLTX + 2, kPY = kLTY + 2, kPN = kPX * kPY;
+constexpr int kPPer = (kPN + 255) / 256;
+constexpr int kPairBuf = 2048;  // (root, saddle) pairs staged in LDS per block
+
+__global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int* ds, int* as,
+                                                     uint16_t* lower) {
+  __shared__ double ring[3][kPN];
+  __shared__ int sup[kLTN];
+  __shared__ int sdn[kLTN];
+  const int ntx = (g.W + kLTX - 1) / kLTX, nty = (g.H + kLTY - 1) / kLTY;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int x0 = (L % ntx) * kLTX, y0 = ((L / ntx) % nty) * kLTY, z0 = (L / (ntx * nty)) * kLTZ;
+  const int lx = threadIdx.x, ly = threadIdx.y, tid = lx + 64 * ly;
+  const bool ordered = g.W >= 3;
+  double pre[kPPer];
+  auto fetch = [&](int z) {
 #pragma unroll
-  for (int d = 0; d < kNbr; d++) {
-    if (!((key >> d) & 1)) continue;
-    int r = v + g.off[d];
-    double fr = f[r];
-    if (fu < fr || (fu == fr && up < r)) {
-      up = r;
-      fu = fr;
+    for (int j = 0; j < kPPer; j++) {
+      int i = tid + 256 * j;
+      double val = 0.0;
+      if (i < kPN) {
+        int px = i % kPX, py = i / kPX, gx = x0 + px - 1, gy = y0 + py - 1;
+        if (z >= 0 && z < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
+          val = f[gx + g.W * (gy + g.H * z)];
+      }
+      pre[j] = val;
     }
-    if (fd > fr || (fd == fr && dn > r)) {
-      dn = r;
-      fd = fr;
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int j = 0; j < kPPer; j++) {
+      int i = tid + 256 * j;
+      if (i < kPN) ring[slot][i] = pre[j];
     }
+  };
+  fetch(z0 - 1);
+  store(0);
+  fetch(z0);
+  store(1);
+  fetch(z0 + 1);
+  for (int lz = 0; lz < kLTZ; lz++) {
+    store((lz + 2) % 3);
+    __syncthreads();
+    if (lz + 1 < kLTZ) fetch(z0 + lz + 2);
+    const double* P0 = ring[lz % 3];        // z - 1
+    const double* P1 = ring[(lz + 1) % 3];  // z
+    const double* P2 = ring[(lz + 2) % 3];  // z + 1
+    const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    const int li = lx + kLTX * (ly + kLTY * lz);
+    int pu = li, pd = li;
+    if (x < g.W && y < g.H && z < g.D) {
+      const int v = x + g.W * (y + g.H * z);
+      const int key = c_cls_key[class_of(g, x, y, z)];
+      const int c0 = (lx + 1) + kPX * (ly + 1);
+      const double c = P1[c0];
+      int du = -1, dd = -1;
+      unsigned lm = 0;
+      if (ordered && key == 0x3FFF) {
+        const double n0 = P1[c0 + 1], n1 = P1[c0 - 1], n2 = P1[c0 + kPX], n3 = P1[c0 - kPX];
+        const double n4 = P2[c0], n5 = P0[c0], n6 = P1[c0 - 1 + kPX], n7 = P1[c0 + 1 - kPX];
+        const double n8 = P2[c0 + kPX], n9 = P0[c0 - kPX], n10 = P2[c0 - 1], n11 = P0[c0 + 1];
+        const double n12 = P2[c0 - 1 + kPX], n13 = P0[c0 + 1 - kPX];
+        // lower: neighbours with a smaller id count on ties
+        lm = (unsigned)(n0 < c) | (unsigned)(n1 <= c) << 1 | (unsigned)(n2 < c) << 2 |
+             (unsigned)(n3 <= c) << 3 | (unsigned)(n4 < c) << 4 | (unsigned)(n5 <= c) << 5 |
+             (unsigned)(n6 < c) << 6 | (unsigned)(n7 <= c) << 7 | (unsigned)(n8 < c) << 8 |
+             (unsigned)(n9 <= c) << 9 | (unsigned)(n10 < c) << 10 | (unsigned)(n11 <= c) << 11 |
+             (unsigned)(n12 < c) << 12 | (unsigned)(n13 <= c) << 13;
+        double bu = n9, bd = n9;
+        du = 9;
+        dd = 9;
+#define EZ_SCAN(val, dir)      \
+  if ((val) >= bu) {           \
+    bu = (val);                \
+    du = (dir);                \
+  }                            \
+  if ((val) < bd) {            \
+    bd = (val);                \
+    dd = (dir);                \
   }
-  ds[v] = up;
-  as[v] = dn;
+        EZ_SCAN(n13, 13) EZ_SCAN(n5, 5) EZ_SCAN(n11, 11) EZ_SCAN(n3, 3) EZ_SCAN(n7, 7)
+        EZ_SCAN(n1, 1) EZ_SCAN(c, -1) EZ_SCAN(n0, 0) EZ_SCAN(n6, 6) EZ_SCAN(n2, 2)
+        EZ_SCAN(n10, 10) EZ_SCAN(n4, 4) EZ_SCAN(n12, 12) EZ_SCAN(n8, 8)
+#undef EZ_SCAN
+      } else {
+        int up = v, dn = v;
+        double fu = c, fd = c;
+#pragma unroll
+        for (int d = 0; d < kNbr; d++) {
+          if (!((key >> d) & 1)) continue;
+          const double* P = kSZ[d] < 0 ? P0 : (kSZ[d] > 0 ? P2 : P1);
+          double fr = P[c0 + kSX[d] + kPX * kSY[d]];
+          int r = v + g.off[d];
+          if (fr < c || (fr == c && r < v)) lm |= 1u << d;
+          if (fu < fr || (fu == fr && up < r)) { up = r; fu = fr; du = d; }
+          if (fd > fr || (fd == fr && dn > r)) { dn = r; fd = fr; dd = d; }
+        }
+      }
+      if (lower) lower[v] = (uint16_t)lm;
+      if (du >= 0) {
+        int ux = lx + c_dx[du], uy = ly + c_dy[du], uz = lz + c_dz[du];
+        bool inside = ux >= 0 && ux < kLTX && uy >= 0 && uy < kLTY && uz >= 0 && uz < kLTZ;
+        pu = inside ? ux + kLTX * (uy + kLTY * uz) : ~(v + g.off[du]);
+      }
+      if (dd >= 0) {
+        int ux = lx + c_dx[dd], uy = ly + c_dy[dd], uz = lz + c_dz[dd];
+        bool inside = ux >= 0 && ux < kLTX && uy >= 0 && uy < kLTY && uz >= 0 && uz < kLTZ;
+        pd = inside ? ux + kLTX * (uy + kLTY * uz) : ~(v + g.off[dd]);
+      }
+    }
+    sup[li] = pu;
+    sdn[li] = pd;
+    __syncthreads();
+  }
+  for (int round = 0; round < 11; round++) {
+    int ch = 0;
+#pragma unroll
+    for (int k = 0; k < kLTN / 256; k++) {
+      int li = tid + 256 * k;
+      int p = sup[li];
+      if (p >= 0 && p != li) {
+        int q = sup[p];
+        if (q != p) { sup[li] = q; ch = 1; }
+      }
+      p = sdn[li];
+      if (p >= 0 && p != li) {
+        int q = sdn[p];
+        if (q != p) { sdn[li] = q; ch = 1; }
+      }
+    }
+    if (!__syncthreads_or(ch)) break;
+  }
+#pragma unroll
+  for (int k = 0; k < kLTN / 256; k++) {
+    int li = tid + 256 * k;
+    int x = x0 + li % kLTX, y = y0 + (li / kLTX) % kLTY, z = z0 + li / (kLTX * kLTY);
+    if (x >= g.W || y >= g.H || z >= g.D) continue;
+    int v = x + g.W * (y + g.H * z);
+    int p = sup[li];
+    ds[v] = p >= 0 ? (x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY))) : ~p;
+    p = sdn[li];
+    as[v] = p >= 0 ? (x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY))) : ~p;
+  }
 }
 
-// One in-place pointer-jumping round on both label arrays (D7: every vertex
-// is compressed to its root).  Sets ctr[C_CHANGED] when anything moved.
-__global__ void __launch_bounds__(256) k_jump(int n, int* ds, int* as, int* ctr) {
+// One in-place pointer-jumping round on both label arrays.
+__global__ void __launch_bounds__(256) k_jump(int n, int* ds, int* as) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
-  bool ch = false;
-  if (v < n) {
-    int r = ds[v], rr = ds[r];
-    if (r != rr) {
-      ds[v] = rr;
-      ch = true;
-    }
-    r = as[v];
-    rr = as[r];
-    if (r != rr) {
-      as[v] = rr;
-      ch = true;
-    }
+  if (v >= n) return;
+  int r = ds[v], rr = ds[r];
+  if (r != rr) ds[v] = rr;
+  r = as[v];
+  rr = as[r];
+  if (r != rr) as[v] = rr;
+}
+
+// Follows every pointer that is not yet at a root to its root (D7: every
+// vertex is compressed).  Other lanes' writes only ever replace a pointer by
+// one further along the same path, so any value read is an ancestor; the
+// walk is bounded by N.
+__device__ __forceinline__ void chase_one(int* S, int v, int n) {
+  int r = S[v];
+  int rr = __hip_atomic_load(&S[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (rr == r) return;
+  for (int step = 0; step < n && rr != r; step++) {
+    r = rr;
+    rr = __hip_atomic_load(&S[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  if (__ballot(ch) && __lane_id() == 0) ctr[C_CHANGED] = 1;
+  S[v] = r;
+}
+__global__ void __launch_bounds__(256) k_chase(int n, int* ds, int* as) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  chase_one(ds, v, n);
+  chase_one(as, v, n);
 }
 
 // ------------------------------------------------------- CP structures ----
@@ -437,150 +687,317 @@ __device__ __forceinline__ void cas_best(int* slot, int cand, const double* f, b
   }
 }
 
-// The extremum of a list with the highest (want_max) or lowest (value, id).
-__global__ void __launch_bounds__(256) k_list_best(const int* list, int n, const double* f,
-                                                   int* slot, int want_max) {
-  int best = kNone;
-  double fb = 0.0;
-  GRID_STRIDE(i, n) {
-    int v = list[i];
-    double fv = f[v];
-    if (best == kNone || (want_max ? larger(fv, v, fb, best) : smaller(fv, v, fb, best))) {
-      best = v;
-      fb = fv;
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    int ob = __shfl_xor(best, o);
-    double of = __shfl_xor(fb, o);
-    if (ob != kNone &&
-        (best == kNone || (want_max ? larger(of, ob, fb, best) : smaller(of, ob, fb, best)))) {
-      best = ob;
-      fb = of;
-    }
-  }
-  if (__lane_id() == 0 && best != kNone) cas_best(slot, best, f, want_max != 0);
-}
-
 __global__ void k_list_fill(const int* list, int n, int* arr, int val) {
   GRID_STRIDE(i, n) arr[list[i]] = val;
 }
 
-// computeCP + findAscPaths/findDesPaths + computelargestSaddlesForMax /
-// computesmallestSaddlesForMin for one saddle list (src/ExtreM.cpp:1597-1679,
-// 1920-2040, 2309-2371), without sorting:
-//  side 2: upper neighbours, ascent roots L; first = lowest (value,id) root,
-//          last = highest; each root that is a listed maximum other than the
-//          global one keeps the highest (value,id) saddle reaching it.
-//  side 1: lower neighbours, descent roots L; first = highest root, last =
-//          lowest; each listed minimum other than the global one keeps the
-//          lowest saddle.
-// With ormap, also stores the roots in neighbour order (compute_Max_for_Saddle,
-// src/ExtreM.cpp:2373-2403; 14 slots, D3).
-__global__ void __launch_bounds__(256) k_saddles(St s, int side, const int* list, int n,
-                                                 const double* f, const int* L, int* first,
-                                                 int* last, int* slots, int glob_idx,
-                                                 int* ormap) {
-  int glob = s.ctr[glob_idx];
-  int ext_type = side == 2 ? 4 : 0;
-  GRID_STRIDE(i, n) {
-    int sv = list[i];
-    int x, y, z;
-    coords(s.g, sv, x, y, z);
-    int key = c_cls_key[class_of(s.g, x, y, z)];
-    double fs = f[sv];
-    int lo = kNone, hi = kNone, c = 0;
-    double flo = 0.0, fhi = 0.0;
-    int roots[kNbr];
-    for (int d = 0; d < kNbr; d++) {
-      if (!((key >> d) & 1)) continue;
-      int r = sv + s.g.off[d];
-      double fr = f[r];
-      bool take = side == 2 ? larger(fr, r, fs, sv) : smaller(fr, r, fs, sv);
-      if (!take) continue;
-      int root = L[r];
-      double fo = f[root];
-      roots[c++] = root;
-      if (lo == kNone || smaller(fo, root, flo, lo)) {
-        lo = root;
-        flo = fo;
-      }
-      if (hi == kNone || larger(fo, root, fhi, hi)) {
-        hi = root;
-        fhi = fo;
-      }
-    }
-    if (side == 2) {
-      first[i] = lo;
-      last[i] = hi;
-    } else {
-      first[i] = hi;
-      last[i] = lo;
-    }
-    if (ormap) {
-      for (int k = 0; k < kNbr; k++) ormap[(long long)i * kNbr + k] = k < c ? roots[k] : kNone;
-    }
-    for (int k = 0; k < c; k++) {
-      int r = roots[k];
-      if (r == glob || s.otype[r] != ext_type) continue;
-      bool dup = false;
-      for (int j = 0; j < k; j++) dup |= roots[j] == r;
-      if (dup) continue;
-      cas_best(&slots[r], sv, f, side == 2);
+// Wavefront-wide best (value, id) over one (id, value) per lane; every lane
+// receives the winner (and its value in fc).
+__device__ __forceinline__ int wave_best(int cand, double& fc, bool want_max) {
+  for (int o = 32; o > 0; o >>= 1) {
+    int ob = __shfl_xor(cand, o);
+    double of = __shfl_xor(fc, o);
+    if (ob != kNone &&
+        (cand == kNone || (want_max ? larger(of, ob, fc, cand) : smaller(of, ob, fc, cand)))) {
+      cand = ob;
+      fc = of;
     }
   }
+  return cand;
+}
+
+// Ordered int64 key of a value with -0.0 folded into +0.0, so that equal
+// values (the reference's `abs(a - b) == 0`) get equal keys.
+__device__ __forceinline__ long long okey_c(double d) { return okey(d + 0.0); }
+
+// Block-wide best (value, id) of one candidate per thread; returns it on
+// thread 0.  All threads of the 256-thread block must call.
+__device__ __forceinline__ int block_best(int cand, double fc, const double* f, bool want_max) {
+  __shared__ int s_best[4];
+  cand = wave_best(cand, fc, want_max);
+  int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  if ((tid & 63) == 0) s_best[tid >> 6] = cand;
+  __syncthreads();
+  int best = kNone;
+  if (tid == 0) {
+    double fb = 0.0;
+    for (int w = 0; w < 4; w++) {
+      int c = s_best[w];
+      if (c == kNone) continue;
+      double fv = f[c];
+      if (best == kNone || (want_max ? larger(fv, c, fb, best) : smaller(fv, c, fb, best))) {
+        best = c;
+        fb = fv;
+      }
+    }
+  }
+  __syncthreads();
+  return best;
+}
+
+// One full-grid pass over saddle voxels (rows of 64 per wavefront, XCD-aware
+// block order), replacing computeCP + findAscPaths / findDesPaths +
+// computelargestSaddlesForMax / computesmallestSaddlesForMin
+// (src/ExtreM.cpp:1597-1679, 1920-2040, 2309-2371) without sorting.  The
+// upper / lower stars come from flow, the lower-star masks of field f.
+//  2-saddles: upper neighbours, ascent roots L2; F2 = lowest (value, id)
+//   root (triplet[0]), La2 = highest (triplet[44]-1); every listed maximum
+//   ends with the highest (value, id) saddle reaching it.
+//  1-saddles: lower neighbours, descent roots L1; F1 = highest root, La1 =
+//   lowest; every listed minimum ends with the lowest saddle.
+// Per-extremum results live at the extremum's list position (ext_id): slot
+// pass A raises the position's ordered value key (atomics without return)
+// and records a (position, saddle) pair; k_slot_ids settles the id among the
+// saddles holding the best key.  A root outside the lists (stale types, D10)
+// has no position and is skipped; the global extremum is excluded where the
+// slots are read.  With dec != 0, also the candidate masks of
+// get_wrong_split_neighbors / get_wrong_join_neighbors
+// (src/ExtreM.cpp:2405-2450): bit d set when the d-th neighbour is in the
+// decp upper (lower) star and its root differs from the original root of the
+// same-rank original upper (lower) neighbour; and the per-block best maximum
+// / minimum under decp.  Stencil loops are unrolled and all loads of a stage
+// are issued before any is used.
+template <int SIDE>
+__device__ __forceinline__ void saddle_side(const St& s, int dec, const double* f, const int* L,
+                                            const int* Lo, int v, unsigned star, unsigned ostar,
+                                            const int* s_off, int* F, int* La, unsigned& cand,
+                                            int (&R)[kNbr], unsigned& uniq) {
+  const Geo& g = s.g;
+  int O[kNbr];
+#pragma unroll
+  for (int d = 0; d < kNbr; d++) {
+    const bool in = (star >> d) & 1;
+    R[d] = in ? L[v + g.off[d]] : kNone;
+    O[d] = (dec && in) ? Lo[v + g.off[d]] : kNone;
+  }
+  uniq = 0;
+#pragma unroll
+  for (int d = 0; d < kNbr; d++) {
+    bool dup = R[d] == kNone;
+#pragma unroll
+    for (int e = 0; e < d; e++) dup |= R[e] == R[d];
+    if (!dup) uniq |= 1u << d;
+  }
+  double V[kNbr];
+#pragma unroll
+  for (int d = 0; d < kNbr; d++) V[d] = ((uniq >> d) & 1) ? f[R[d]] : 0.0;
+  int lo = kNone, hi = kNone;
+  double flo = 0.0, fhi = 0.0;
+#pragma unroll
+  for (int d = 0; d < kNbr; d++) {
+    if (!((uniq >> d) & 1)) continue;
+    if (lo == kNone || smaller(V[d], R[d], flo, lo)) { lo = R[d]; flo = V[d]; }
+    if (hi == kNone || larger(V[d], R[d], fhi, hi)) { hi = R[d]; fhi = V[d]; }
+  }
+  if (SIDE == 2) {
+    F[v] = lo;
+    La[v] = hi;
+  } else {
+    F[v] = hi;
+    La[v] = lo;
+  }
+  cand = 0;
+  if (!dec) return;
+  if (star == ostar) {
+    // decp star == original star (always so after c_loops): the k-th decp
+    // neighbour is the k-th original neighbour
+#pragma unroll
+    for (int d = 0; d < kNbr; d++)
+      if (R[d] != kNone && R[d] != O[d]) cand |= 1u << d;
+  } else {
+    unsigned st = star, os = ostar;
+    while (st) {
+      int d = __ffs(st) - 1;
+      st &= st - 1;
+      int od = os ? __ffs(os) - 1 : -1;
+      os &= os - 1;
+      int oroot = od < 0 ? kNone : Lo[v + s_off[od]];
+      if (L[v + s_off[d]] != oroot) cand |= 1u << d;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_saddle_grid(St s, int dec, const double* f,
+                                                     const int* L2, const int* L1,
+                                                     const uint16_t* flow, int* F2, int* La2,
+                                                     int* F1, int* La1, int2* pairs,
+                                                     long long* kmax, long long* kmin,
+                                                     int* bmax, int* bmin) {
+  __shared__ int2 s_pairs[kPairBuf];
+  __shared__ int s_off[kNbr];
+  __shared__ int s_np, s_pbase;
+  const Geo& g = s.g;
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  if (tid < kNbr) s_off[tid] = c_dx[tid] + g.W * (c_dy[tid] + g.H * c_dz[tid]);
+  if (tid == 0) s_np = 0;
+  __syncthreads();
+  RowPos p = row_pos(g);
+  const int v = p.in ? p.x + g.W * (p.y + g.H * p.z) : 0;
+  const int t = p.in ? s.otype[v] : 5;
+  const bool s2 = t == 2 || t == 3, s1 = t == 1 || t == 3;
+  const double fs = (p.in && (s2 || s1 || (dec && (t == 4 || t == 0)))) ? f[v] : 0.0;
+  int R2[kNbr], R1[kNbr];
+  unsigned u2 = 0, u1 = 0, m2 = 0, m1 = 0;
+  if (s2 || s1) {
+    const unsigned key = c_cls_key[class_of(g, p.x, p.y, p.z)];
+    const unsigned fl = flow[v];
+    const unsigned ol = s.olow[v];
+    if (s2)
+      saddle_side<2>(s, dec, f, L2, s.oDS, v, key & ~fl, key & ~ol, s_off, F2, La2, m2, R2, u2);
+    if (s1) saddle_side<1>(s, dec, f, L1, s.oAS, v, key & fl, ol, s_off, F1, La1, m1, R1, u1);
+  }
+  // slot pass A on list positions: maxima have even ext_id, minima odd
+  int E2[kNbr], E1[kNbr];
+#pragma unroll
+  for (int d = 0; d < kNbr; d++) {
+    E2[d] = ((u2 >> d) & 1) ? s.ext_id[R2[d]] : -1;
+    E1[d] = ((u1 >> d) & 1) ? s.ext_id[R1[d]] : -1;
+  }
+  const long long kv = okey_c(fs);
+  int cnt = 0;
+#pragma unroll
+  for (int d = 0; d < kNbr; d++) {
+    if (E2[d] >= 0 && !(E2[d] & 1)) {
+      atomicMax(&kmax[E2[d] >> 1], kv);
+      cnt++;
+    } else {
+      E2[d] = -1;
+    }
+    if (E1[d] >= 0 && (E1[d] & 1)) {
+      atomicMin(&kmin[E1[d] >> 1], kv);
+      cnt++;
+    } else {
+      E1[d] = -1;
+    }
+  }
+  // (position, saddle) pairs: one LDS atomic per wavefront, then plain stores
+  int pre = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(pre, o);
+    if (__lane_id() >= o) pre += y;
+  }
+  const int total = __shfl(pre, 63);
+  pre -= cnt;
+  int base = 0;
+  if (__lane_id() == 0 && total) base = atomicAdd(&s_np, total);
+  base = __shfl(base, 0) + pre;
+#pragma unroll
+  for (int d = 0; d < kNbr; d++) {
+    if (E2[d] >= 0) {
+      int2 pr = make_int2(E2[d], v);  // even: maximum
+      if (base < kPairBuf) s_pairs[base] = pr;
+      else pairs[atomicAdd(&s.ctr[C_PAIRS], 1)] = pr;
+      base++;
+    }
+    if (E1[d] >= 0) {
+      int2 pr = make_int2(E1[d], v);  // odd: minimum
+      if (base < kPairBuf) s_pairs[base] = pr;
+      else pairs[atomicAdd(&s.ctr[C_PAIRS], 1)] = pr;
+      base++;
+    }
+  }
+  __syncthreads();
+  const int np = s_np < kPairBuf ? s_np : kPairBuf;
+  if (tid == 0) s_pbase = np ? atomicAdd(&s.ctr[C_PAIRS], np) : 0;
+  __syncthreads();
+  for (int i = tid; i < np; i += 256) pairs[s_pbase + i] = s_pairs[i];
+  if (dec) {
+    block_append(s.act0, &s.ctr[C_ACT0], m2 != 0, make_int2(2 * v, (int)m2));
+    block_append(s.act0, &s.ctr[C_ACT0], m1 != 0, make_int2(2 * v + 1, (int)m1));
+    int bm = block_best(t == 4 ? v : kNone, fs, f, true);
+    int bn = block_best(t == 0 ? v : kNone, fs, f, false);
+    if (tid == 0) {
+      bmax[blockIdx.x] = bm;
+      bmin[blockIdx.x] = bn;
+    }
+  }
+}
+
+// Reduction of per-block extrema: out[block] = best of its grid-stride share.
+__global__ void __launch_bounds__(256) k_ext_reduce(const int* inmax, const int* inmin, int n,
+                                                    const double* f, int* outmax, int* outmin) {
+  int cmax = kNone, cmin = kNone;
+  double fmax = 0, fmin = 0;
+  GRID_STRIDE(i, n) {
+    int a = inmax[i];
+    if (a != kNone) {
+      double fa = f[a];
+      if (cmax == kNone || larger(fa, a, fmax, cmax)) { cmax = a; fmax = fa; }
+    }
+    int b = inmin[i];
+    if (b != kNone) {
+      double fb = f[b];
+      if (cmin == kNone || smaller(fb, b, fmin, cmin)) { cmin = b; fmin = fb; }
+    }
+  }
+  int gm = block_best(cmax, fmax, f, true);
+  int gn = block_best(cmin, fmin, f, false);
+  if (threadIdx.x == 0) {
+    outmax[blockIdx.x] = gm;
+    outmin[blockIdx.x] = gn;
+  }
+}
+
+// Slot pass B: among the recorded (position, saddle) pairs, the saddles
+// whose value equals the position's best value key settle the id (highest id
+// for a maximum's slot, lowest for a minimum's), giving the (value, id) order
+// of computelargestSaddlesForMax / computesmallestSaddlesForMin.
+__global__ void __launch_bounds__(256) k_slot_ids(St s, const int2* pairs, const double* f,
+                                                  const long long* kmax, const long long* kmin,
+                                                  int* slot_max, int* slot_min) {
+  int n = s.ctr[C_PAIRS];
+  GRID_STRIDE(i, n) {
+    int2 pr = pairs[i];
+    int e = pr.x >> 1;
+    long long k = okey_c(f[pr.y]);
+    if (pr.x & 1) {
+      if (k == kmin[e]) atomicMin(&slot_min[e], pr.y);
+    } else {
+      if (k == kmax[e]) atomicMax(&slot_max[e], pr.y);
+    }
+  }
+}
+
+// ext_id[v] = 2 * (position in MX) for maxima, 2 * (position in MN) + 1 for
+// minima (set once per run; -1 elsewhere).
+__global__ void k_ext_index(St s) {
+  GRID_STRIDE(i, s.nmax) s.ext_id[s.MX[i]] = 2 * i;
+  GRID_STRIDE(i, s.nmin) s.ext_id[s.MN[i]] = 2 * i + 1;
+}
+
+__global__ void k_fill_pair(long long* a, long long va, int* b, int vb, int n) {
+  GRID_STRIDE(i, n) {
+    a[i] = va;
+    b[i] = vb;
+  }
+}
+
+__global__ void k_list_fill64(const int* list, int n, long long* arr, long long val) {
+  GRID_STRIDE(i, n) arr[list[i]] = val;
 }
 
 // ------------------------------------------- wrong neighbours (greedy) ----
-// Candidate masks of get_wrong_split_neighbors / get_wrong_join_neighbors
-// (src/ExtreM.cpp:2405-2450): bit d set when the d-th neighbour is in the
-// saddle's upper (lower) star of decp and its root differs from the root of
-// the same-rank original neighbour.  Entries e < n2 are 2-saddles, the rest
-// 1-saddles.
-__global__ void __launch_bounds__(256) k_wn_cand(St s, int n2, int n1, int* active, int actr) {
-  int n = n2 + n1;
-  GRID_STRIDE(e, n) {
-    int side2 = e < n2;
-    int i = side2 ? e : e - n2;
-    int sv = side2 ? s.S2[i] : s.S1[i];
-    const int* L = side2 ? s.dDS : s.dAS;
-    const int* om = side2 ? s.ormap2 : s.ormap1;
-    int x, y, z;
-    coords(s.g, sv, x, y, z);
-    int key = c_cls_key[class_of(s.g, x, y, z)];
-    double fs = s.decp[sv];
-    unsigned mask = 0;
-    int c = 0;
-    for (int d = 0; d < kNbr; d++) {
-      if (!((key >> d) & 1)) continue;
-      int r = sv + s.g.off[d];
-      double fr = s.decp[r];
-      bool take = side2 ? larger(fr, r, fs, sv) : smaller(fr, r, fs, sv);
-      if (!take) continue;
-      int orl = c < kNbr ? om[(long long)i * kNbr + c] : kNone;
-      if (L[r] != orl) mask |= 1u << d;
-      c++;
-    }
-    s.cmask[e] = (uint16_t)mask;
-    if (mask) {
-      int p = atomicAdd(&s.ctr[actr], 1);
-      active[p] = e;
-    }
-  }
+// Deterministic reservations reproducing the serial greedy of
+// get_wrong_split_neighbors / get_wrong_join_neighbors in saddle-id order
+// (main.cpp:309-323).  Entries are {2*saddle + side, candidate mask}; side 0
+// is a 2-saddle (upper star), side 1 a 1-saddle (lower star).
+__global__ void k_act_copy(St s, int2* dst, int dctr) {
+  int n = s.ctr[C_ACT0];
+  GRID_STRIDE(i, n) dst[i] = s.act0[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr[dctr] = n;
 }
 
-// Deterministic reservations reproducing the serial greedy in saddle-id
-// order: drop flagged candidates, then every remaining candidate records the
-// smallest saddle id that may still take it.
-__global__ void __launch_bounds__(256) k_wn_register(St s, int n2, const int* active, int actr) {
+// Drop flagged candidates; every remaining candidate records the smallest
+// saddle id that may still take it.
+__global__ void __launch_bounds__(256) k_wn_register(St s, int2* active, int actr) {
   int n = s.ctr[actr];
   GRID_STRIDE(a, n) {
-    int e = active[a];
-    int side2 = e < n2;
-    int sv = side2 ? s.S2[e] : s.S1[e - n2];
-    const uint8_t* fl = side2 ? s.wn2 : s.wn1;
-    int* ms = side2 ? s.minsad2 : s.minsad1;
-    unsigned mask = s.cmask[e];
+    int2 e = active[a];
+    int sv = e.x >> 1, side = e.x & 1;
+    const uint8_t* fl = side ? s.wn1 : s.wn2;
+    int* ms = side ? s.minsad1 : s.minsad2;
+    unsigned mask = (unsigned)e.y;
     for (int d = 0; d < kNbr; d++) {
       if (!((mask >> d) & 1)) continue;
       int r = sv + s.g.off[d];
@@ -590,55 +1007,52 @@ __global__ void __launch_bounds__(256) k_wn_register(St s, int n2, const int* ac
       }
       atomicMin(&ms[r], sv);
     }
-    s.cmask[e] = (uint16_t)mask;
+    active[a].y = (int)mask;
   }
 }
 
 // A saddle whose first candidate is reserved by itself takes it (flag +
 // append, as get_wrong_split_neighbors does before its return).
-__global__ void __launch_bounds__(256) k_wn_commit(St s, int n2, const int* active, int actr) {
+__global__ void __launch_bounds__(256) k_wn_commit(St s, int2* active, int actr) {
+  STAGED_LIST(l2, int, kStage, s.wnl2, &s.ctr[C_NFC]);
+  STAGED_LIST(l1, int, kStage, s.wnl1, &s.ctr[C_NFC1]);
   int n = s.ctr[actr];
   GRID_STRIDE(a, n) {
-    int e = active[a];
-    unsigned mask = s.cmask[e];
+    int2 e = active[a];
+    unsigned mask = (unsigned)e.y;
     if (!mask) continue;
-    int side2 = e < n2;
-    int sv = side2 ? s.S2[e] : s.S1[e - n2];
-    int d = __ffs(mask) - 1;
-    int r = sv + s.g.off[d];
-    int* ms = side2 ? s.minsad2 : s.minsad1;
-    if (ms[r] == sv) {
-      if (side2) {
+    int sv = e.x >> 1, side = e.x & 1;
+    int r = sv + s.g.off[__ffs(mask) - 1];
+    if ((side ? s.minsad1 : s.minsad2)[r] == sv) {
+      if (!side) {
         s.wn2[r] = 1;
-        int p = atomicAdd(&s.ctr[C_NFC], 1);
-        s.wnl2[p] = r;
+        l2.push(r);
       } else {
         s.wn1[r] = 1;
-        int p = atomicAdd(&s.ctr[C_NFC1], 1);
-        s.wnl1[p] = r;
+        l1.push(r);
       }
-      s.cmask[e] = (uint16_t)(mask | 0x8000u);  // done
+      active[a].y = (int)(mask | 0x8000u);  // done
     }
   }
+  l2.flush();
+  l1.flush();
 }
 
 // Releases this round's reservations and keeps unfinished saddles.
-__global__ void __launch_bounds__(256) k_wn_next(St s, int n2, const int* active, int actr,
-                                                 int* next, int nctr) {
+__global__ void __launch_bounds__(256) k_wn_next(St s, const int2* active, int actr, int2* next,
+                                                 int nctr) {
+  STAGED_LIST(nl, int2, kStage, next, &s.ctr[nctr]);
   int n = s.ctr[actr];
   GRID_STRIDE(a, n) {
-    int e = active[a];
-    unsigned mask = s.cmask[e];
-    int side2 = e < n2;
-    int sv = side2 ? s.S2[e] : s.S1[e - n2];
-    int* ms = side2 ? s.minsad2 : s.minsad1;
+    int2 e = active[a];
+    unsigned mask = (unsigned)e.y;
+    int sv = e.x >> 1, side = e.x & 1;
+    int* ms = side ? s.minsad1 : s.minsad2;
     for (int d = 0; d < kNbr; d++)
       if ((mask >> d) & 1) ms[sv + s.g.off[d]] = 0x7fffffff;
-    if (mask && !(mask & 0x8000u)) {
-      int p = atomicAdd(&s.ctr[nctr], 1);
-      next[p] = e;
-    }
+    if (mask && !(mask & 0x8000u)) nl.push(e);
   }
+  nl.flush();
 }
 
 __global__ void k_wn_reset(St s) {
@@ -652,12 +1066,13 @@ __global__ void k_wn_reset(St s) {
 // fixpath, split (src/ExtreM.cpp:2452-2575): walk the original steepest path
 // from a wrong neighbour until decp's steepest choice differs.
 __global__ void __launch_bounds__(256) k_fixpath(St s) {
+  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[C_TOUCHED]);
   int n2 = s.ctr[C_NFC], n1 = s.ctr[C_NFC1];
   GRID_STRIDE(e, n2 + n1) {
     int dir = e < n2 ? 0 : 1;
     int cur = dir == 0 ? s.wnl2[e] : s.wnl1[e - n2];
     int tru = kNone, fal = kNone;
-    for (long long step = 0; step <= (long long)s.g.N; step++) {
+    for (int step = 0; step <= s.g.N; step++) {
       int x, y, z;
       coords(s.g, cur, x, y, z);
       int key = c_cls_key[class_of(s.g, x, y, z)];
@@ -684,9 +1099,10 @@ __global__ void __launch_bounds__(256) k_fixpath(St s) {
       cur = nx;
     }
     if (tru == fal) continue;
-    if (dir == 0) propose(s, fal, lower_mid(s, fal));
-    else propose(s, tru, lower_mid(s, tru));
+    if (dir == 0) propose(s, fal, lower_mid(s, fal), tl);
+    else propose(s, tru, lower_mid(s, tru), tl);
   }
+  tl.flush();
 }
 
 // ------------------------------------------------- wrong index (ranks) ----
@@ -701,57 +1117,73 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
   const int* ol = side == 2 ? s.ol2 : s.ol1;
   const int* df = side == 2 ? s.df2 : s.df1;
   const int* dl = side == 2 ? s.dl2 : s.dl1;
+  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[C_TOUCHED]);
+  STAGED_COUNT(cf, &s.ctr[side == 2 ? C_WMC : C_WMINC2]);
+  STAGED_COUNT(cl, &s.ctr[side == 2 ? C_WMC2 : C_WMINC]);
   GRID_STRIDE(i, n) {
     int sv = S[i];
-    bool mf = of[i] != df[i], ml = ol[i] != dl[i];
+    int a = of[sv], b = ol[sv], c = df[sv], d = dl[sv];
+    bool mf = a != c, ml = b != d;
     if (phase == 0) {
-      if (mf) atomicMin(&s.win_a[of[i]], sv);
-      if (ml) atomicMin(&s.win_b[ol[i]], sv);
+      if (mf) atomicMin(&s.win_a[a], sv);
+      if (ml) atomicMin(&s.win_b[b], sv);
     } else if (phase == 1) {
-      if (mf && s.win_a[of[i]] == sv) {
-        if (side == 2) {  // [0] -> wrong_max_counter, fix dir 0 lowers orig
-          atomicAdd(&s.ctr[C_WMC], 1);
-          if (do_fix) propose(s, of[i], lower_mid(s, of[i]));
-        } else {          // [0] -> wrong_min_counter_2, fix dir 1 lowers dec
-          atomicAdd(&s.ctr[C_WMINC2], 1);
-          if (do_fix && df[i] != kNone) propose(s, df[i], lower_mid(s, df[i]));
+      if (mf && s.win_a[a] == sv) {
+        cf.inc();
+        if (side == 2) {  // triplet[0] -> wrong_max_counter, fix dir 0 lowers orig
+          if (do_fix) propose(s, a, lower_mid(s, a), tl);
+        } else {          // triplet[0] -> wrong_min_counter_2, fix dir 1 lowers decp's
+          if (do_fix && c != kNone) propose(s, c, lower_mid(s, c), tl);
         }
       }
-      if (ml && s.win_b[ol[i]] == sv) {
-        if (side == 2) {  // [last] -> wrong_max_counter_2, fix dir 1 lowers dec
-          atomicAdd(&s.ctr[C_WMC2], 1);
-          if (do_fix && dl[i] != kNone) propose(s, dl[i], lower_mid(s, dl[i]));
-        } else {          // [last] -> wrong_min_counter, fix dir 0 lowers orig
-          atomicAdd(&s.ctr[C_WMINC], 1);
-          if (do_fix) propose(s, ol[i], lower_mid(s, ol[i]));
+      if (ml && s.win_b[b] == sv) {
+        cl.inc();
+        if (side == 2) {  // triplet[last] -> wrong_max_counter_2, fix dir 1 lowers decp's
+          if (do_fix && d != kNone) propose(s, d, lower_mid(s, d), tl);
+        } else {          // triplet[last] -> wrong_min_counter, fix dir 0 lowers orig
+          if (do_fix) propose(s, b, lower_mid(s, b), tl);
         }
       }
     } else {
-      if (mf) s.win_a[of[i]] = 0x7fffffff;
-      if (ml) s.win_b[ol[i]] = 0x7fffffff;
+      if (mf) s.win_a[a] = 0x7fffffff;
+      if (ml) s.win_b[b] = 0x7fffffff;
     }
   }
+  tl.flush();
+  cf.flush();
+  cl.flush();
 }
 
 // get_wrong_index_saddles(_join) + fix_wrong_index_saddle
-// (src/ExtreM.cpp:2672-2715, 2811-2836).  A side without an adjacent saddle
-// (the reference reads one past its saddle vector there) is no mismatch.
+// (src/ExtreM.cpp:2672-2715, 2811-2836).  The global extremum has no saddle
+// (its triplet entries are skipped, src/ExtreM.cpp:2330); a side without an
+// adjacent saddle (the reference reads one past its saddle vector there) is
+// no mismatch.
 __global__ void __launch_bounds__(256) k_wrong_saddle(St s, int side, int do_fix) {
   int n = side == 2 ? s.nmax : s.nmin;
   const int* E = side == 2 ? s.MX : s.MN;
   const int* os = side == 2 ? s.olarg : s.osmall;
   const int* ds = side == 2 ? s.dlarg : s.dsmall;
+  int og = s.ctr[side == 2 ? C_GMAX : C_GMIN];
+  int dg = s.ctr[side == 2 ? C_DGMAX : C_DGMIN];
+  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[C_TOUCHED]);
+  STAGED_COUNT(cnt, &s.ctr[side == 2 ? C_WSC : C_WSCJ]);
   GRID_STRIDE(i, n) {
     int m = E[i];
-    int ov = os[m], dv = ds[m];
-    if (ov == dv || ov == kNone || dv == kNone) continue;  // D12
-    atomicAdd(&s.ctr[side == 2 ? C_WSC : C_WSCJ], 1);
+    int ov = m == og ? kNone : os[i];
+    int dv = m == dg ? kNone : ds[i];
+    if (ov == 0x7fffffff) ov = kNone;  // empty minimum slot
+    if (dv == 0x7fffffff) dv = kNone;
+    if (ov == dv || ov == kNone || dv == kNone) continue;
+    cnt.inc();
     if (!do_fix) continue;
     double ft = s.decp[ov], ff = s.decp[dv];
     if (ft > ff || (ft == ff && ov > dv)) continue;
-    if (side == 2) propose(s, dv, lower_mid(s, dv));
-    else propose(s, ov, raise_mid(s, ov));
+    if (side == 2) propose(s, dv, lower_mid(s, dv), tl);
+    else propose(s, ov, raise_mid(s, ov), tl);
   }
+  tl.flush();
+  cnt.flush();
 }
 
 // --------------------------------------------------------------- setup ----
@@ -760,10 +1192,10 @@ __global__ void __launch_bounds__(256) k_lists(St s) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   bool in = v < s.g.N;
   int t = in ? s.otype[v] : 5;
-  wave_append(s.S2, &s.ctr[C_N2], in && (t == 2 || t == 3), v);
-  wave_append(s.S1, &s.ctr[C_N1], in && (t == 1 || t == 3), v);
-  wave_append(s.MX, &s.ctr[C_NMAX], in && t == 4, v);
-  wave_append(s.MN, &s.ctr[C_NMIN], in && t == 0, v);
+  block_append(s.S2, &s.ctr[C_N2], in && (t == 2 || t == 3), v);
+  block_append(s.S1, &s.ctr[C_N1], in && (t == 1 || t == 3), v);
+  block_append(s.MX, &s.ctr[C_NMAX], in && t == 4, v);
+  block_append(s.MN, &s.ctr[C_NMIN], in && t == 0, v);
 }
 
 // main.cpp:74-75 and 122-131: decp := orig - xi at the global min / max and

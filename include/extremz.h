@@ -55,9 +55,9 @@ enum {
   EXTREMZ_K_CLASSIFY = 0, /* full-grid classification of the original */
   EXTREMZ_K_CHECK_FULL,   /* full-grid false-critical-point scan */
   EXTREMZ_K_CHECK_INCR,   /* frontier / neighbourhood re-check */
-  EXTREMZ_K_STEEPEST,     /* full-grid steepest ascent+descent pointers */
-  EXTREMZ_K_JUMP,         /* one pointer-jumping round, both manifolds */
-  EXTREMZ_K_SADDLE,       /* per-saddle triplet / extremum-map pass */
+  EXTREMZ_K_STEEPEST,     /* label tiles: steepest pointers resolved in LDS */
+  EXTREMZ_K_JUMP,         /* one full-grid pointer-jumping round, both manifolds */
+  EXTREMZ_K_SADDLE,       /* full-grid saddle pass (triplets, maps, candidates) */
   EXTREMZ_K_PROPOSE,      /* correction proposals (all loops) */
   EXTREMZ_K_APPLY,        /* apply_delta over touched vertices */
   EXTREMZ_K_FAMILIES_ = 8
