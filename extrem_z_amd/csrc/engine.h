@@ -45,6 +45,16 @@ enum Ctr {
   C_COUNT_
 };
 
+// Per-vertex record of a listed extremum, refreshed at every slot pass: its
+// value in the field of that pass and its list position (2*i in MX, 2*i+1 in
+// MN; -1 for every other vertex).  One 16-byte gather gives a saddle pass
+// both the ordering value and the slot of a root.
+struct __align__(16) ExtRec {
+  double f;
+  int ext;
+  int pad;
+};
+
 struct Geo {
   int W, H, D, WH, N;
   int off[kNbr];  // index offset of each stencil direction
@@ -71,10 +81,10 @@ struct St {
   // CP lists (original types; equal to the decp types wherever used)
   int *S2, *S1, *MX, *MN;
   int n2, n1, nmax, nmin;
-  int *of2, *ol2, *of1, *ol1;     // original triplet first / last (per vertex)
-  int *df2, *dl2, *df1, *dl1;     // decp triplet first / last (per vertex)
+  int *of2, *ol2, *of1, *ol1;     // original triplet first / last (per list entry)
+  int *df2, *dl2, *df1, *dl1;     // decp triplet first / last (per list entry)
   int *olarg, *osmall, *dlarg, *dsmall;  // per extremum, at its list position
-  int* ext_id;                    // 2*position (maxima) / 2*position+1 (minima), else -1
+  ExtRec* rec;                    // per vertex; valid at listed extrema
   int2* act0;                     // {2*saddle + side, candidate mask}
   uint8_t *wn2, *wn1;             // wrong_neighbors(_ds) flags
   int *wnl2, *wnl1;               // wrong_neighbors(_ds)_index
@@ -112,17 +122,17 @@ __global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t
 __global__ void k_jump(int n, int* ds, int* as);
 __global__ void k_chase(int n, int* ds, int* as);
 __global__ void k_list_fill(const int* list, int n, int* arr, int val);
-__global__ void k_saddle_grid(St s, int dec, const double* f, const int* L2, const int* L1,
+__global__ void k_saddle_list(St s, int dec, const double* f, const int* L2, const int* L1,
                               const uint16_t* flow, int* F2, int* La2, int* F1, int* La1,
-                              int2* pairs, long long* kmax, long long* kmin, int* bmax,
-                              int* bmin);
+                              int2* pairs, long long* kmax, long long* kmin, int nb2);
 __global__ void k_slot_ids(St s, const int2* pairs, const double* f, const long long* kmax,
                            const long long* kmin, int* slot_max, int* slot_min);
-__global__ void k_ext_index(St s);
+__global__ void k_rec_fill(St s, const double* f);
+__global__ void k_rec_clear(ExtRec* rec, long long n);
 __global__ void k_fill_pair(long long* a, long long va, int* b, int vb, int n);
 __global__ void k_list_fill64(const int* list, int n, long long* arr, long long val);
-__global__ void k_ext_reduce(const int* inmax, const int* inmin, int n, const double* f,
-                             int* outmax, int* outmin);
+__global__ void k_ext_reduce(const int* inmax, int nmax, const int* inmin, int nmin,
+                             const double* f, int* outmax, int* outmin);
 __global__ void k_act_copy(St s, int2* dst, int dctr);
 __global__ void k_wn_register(St s, int2* active, int actr);
 __global__ void k_wn_commit(St s, int2* active, int actr);

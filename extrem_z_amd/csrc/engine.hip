@@ -146,9 +146,9 @@ struct extremz_ctx {
   ez::DevBuf<uint8_t> cnt, otype, dflag, wn2, wn1;
   ez::DevBuf<uint16_t> olow, dlow;
   ez::DevBuf<int> oDS, oAS, dDS, dAS, stamp, touched, dirty, frontA, frontB;
-  ez::DevBuf<int> S2, S1, MX, MN, olarg, osmall, dlarg, dsmall, minsad2, minsad1, win_a, win_b,
-      ext_id;
-  ez::DevBuf<int> of2, ol2, of1, ol1, df2, dl2, df1, dl1, wnl2, wnl1, bmax, bmin, bmax2, bmin2;
+  ez::DevBuf<int> S2, S1, MX, MN, olarg, osmall, dlarg, dsmall, minsad2, minsad1, win_a, win_b;
+  ez::DevBuf<ez::ExtRec> rec;
+  ez::DevBuf<int> of2, ol2, of1, ol1, df2, dl2, df1, dl1, wnl2, wnl1, bmax2, bmin2;
   ez::DevBuf<int2> act0, actA, actB, pairs;
   ez::DevBuf<int> ctr;
   ez::DevBuf<double> orig_tmp, decp_tmp;
@@ -200,7 +200,7 @@ class Run {
     s_.g.N = W * H * D;
     for (int d = 0; d < kNbr; d++) s_.g.off[d] = kDx[d] + W * (kDy[d] + H * kDz[d]);
   }
-  // 64x4 row blocks (k_classify, k_check_full, k_saddle_grid)
+  // 64x4 row blocks (k_classify, k_check_full)
   int row_blocks() const { return ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4) * s_.g.D; }
   // 64x4x4 label tiles
   int tile_blocks() const {
@@ -261,17 +261,16 @@ void Run::allocate(long long N) {
   c->delta.ensure(N);
   c->kmax.ensure(N);
   c->kmin.ensure(N);
+  c->rec.ensure(N);
   for (DevBuf<uint8_t>* b : {&c->cnt, &c->otype, &c->dflag, &c->wn2, &c->wn1}) b->ensure(N);
   c->olow.ensure(N);
   c->dlow.ensure(N);
   for (DevBuf<int>* b : {&c->oDS, &c->oAS, &c->dDS, &c->dAS, &c->stamp, &c->touched, &c->dirty,
                          &c->frontA, &c->frontB, &c->S2, &c->S1, &c->MX, &c->MN, &c->olarg,
-                         &c->osmall, &c->dlarg, &c->dsmall, &c->minsad2, &c->minsad1, &c->ext_id,
+                         &c->osmall, &c->dlarg, &c->dsmall, &c->minsad2, &c->minsad1,
                          &c->win_a, &c->win_b, &c->of2, &c->ol2, &c->of1, &c->ol1, &c->df2,
                          &c->dl2, &c->df1, &c->dl1, &c->wnl2, &c->wnl1})
     b->ensure(N);
-  c->bmax.ensure(row_blocks());
-  c->bmin.ensure(row_blocks());
   c->bmax2.ensure(256);
   c->bmin2.ensure(256);
   c->act0.ensure(2 * N);
@@ -293,15 +292,18 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
   k_fill_pair<<<blocks_for(s_.nmin), 256, 0, st()>>>(kmin, 0x7fffffffffffffffLL, smin, 0x7fffffff,
                                                      s_.nmin);
   set_ctr2(C_ACT0, C_PAIRS, 0);
-  const int rb = row_blocks();
-  k_saddle_grid<<<rb, dim3(64, 4, 1), 0, st()>>>(s_, dec, f, L2, L1, flow, F2, La2, F1, La1,
-                                                  c_->pairs.p, kmax, kmin, c_->bmax.p,
-                                                  c_->bmin.p);
+  k_rec_fill<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(s_, f);
+  const int nb2 = (s_.n2 + 255) / 256, nb1 = (s_.n1 + 255) / 256;
+  if (nb2 + nb1)
+    k_saddle_list<<<nb2 + nb1, 256, 0, st()>>>(s_, dec, f, L2, L1, flow, F2, La2, F1, La1,
+                                               c_->pairs.p, kmax, kmin, nb2);
   k_slot_ids<<<blocks_for(14LL * (s_.n2 + s_.n1)), 256, 0, st()>>>(s_, c_->pairs.p, f, kmax, kmin,
                                                                    smax, smin);
   if (dec) {
-    k_ext_reduce<<<256, 256, 0, st()>>>(c_->bmax.p, c_->bmin.p, rb, f, c_->bmax2.p, c_->bmin2.p);
-    k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, c_->bmin2.p, 256, f, s_.ctr + C_DGMAX,
+    // decp global maximum / minimum among the listed extrema
+    k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, f, c_->bmax2.p,
+                                        c_->bmin2.p);
+    k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, 256, c_->bmin2.p, 256, f, s_.ctr + C_DGMAX,
                                       s_.ctr + C_DGMIN);
   }
   tend();
@@ -531,7 +533,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   s_.dlarg = c->dlarg.p;
   s_.dsmall = c->dsmall.p;
   s_.act0 = c->act0.p;
-  s_.ext_id = c->ext_id.p;
+  s_.rec = c->rec.p;
   s_.wn2 = c->wn2.p;
   s_.wn1 = c->wn1.p;
   s_.wnl2 = c->wnl2.p;
@@ -554,7 +556,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   ez::hip_check(hipMemsetAsync(s_.wn1, 0, N, q), "memset");
   for (int* p : {s_.minsad2, s_.minsad1, s_.win_a, s_.win_b})
     k_fill_i32<<<bN, 256, 0, q>>>(p, 0x7fffffff, N);
-  k_fill_i32<<<bN, 256, 0, q>>>(s_.ext_id, -1, N);
+  k_rec_clear<<<bN, 256, 0, q>>>(s_.rec, N);
   set_ctr2(C_GMIN, C_GMAX, kNone);
   pull();
   if (h(C_FAIL)) return EXTREMZ_EINVAL;
@@ -579,7 +581,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   counts_[EXTREMZ_STAT_N_MIN] = s_.nmin;
   // every saddle records at most one pair per distinct root of its star
   c->pairs.ensure(14LL * (s_.n2 + s_.n1) + 1);
-  k_ext_index<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, q>>>(s_);
+
   slot_pass(0, d_orig, s_.oDS, s_.oAS, s_.olow, s_.of2, s_.ol2, s_.of1, s_.ol1, s_.olarg,
             s_.osmall);
   // main.cpp:74-75, 122-131
@@ -681,9 +683,9 @@ static void free_ctx(extremz_ctx* c) {
   for (ez::DevBuf<int>* b :
        {&c->oDS, &c->oAS, &c->dDS, &c->dAS, &c->stamp, &c->touched, &c->dirty, &c->frontA,
         &c->frontB, &c->S2, &c->S1, &c->MX, &c->MN, &c->olarg, &c->osmall, &c->dlarg,
-        &c->dsmall, &c->minsad2, &c->minsad1, &c->win_a, &c->win_b, &c->ext_id, &c->of2, &c->ol2,
+        &c->dsmall, &c->minsad2, &c->minsad1, &c->win_a, &c->win_b, &c->of2, &c->ol2,
         &c->of1,
-        &c->ol1, &c->df2, &c->dl2, &c->df1, &c->dl1, &c->wnl2, &c->wnl1, &c->bmax, &c->bmin,
+        &c->ol1, &c->df2, &c->dl2, &c->df1, &c->dl1, &c->wnl2, &c->wnl1,
         &c->bmax2, &c->bmin2, &c->ctr})
     b->release();
   for (ez::DevBuf<uint8_t>* b : {&c->lut, &c->cnt, &c->otype, &c->dflag, &c->wn2, &c->wn1,
@@ -693,6 +695,7 @@ static void free_ctx(extremz_ctx* c) {
   c->delta.release();
   c->kmax.release();
   c->kmin.release();
+  c->rec.release();
   c->olow.release();
   c->dlow.release();
   c->orig_tmp.release();

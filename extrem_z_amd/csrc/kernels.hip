@@ -59,8 +59,10 @@ __device__ __forceinline__ int class_of(const Geo& g, int x, int y, int z) {
   return axis_state(x, g.W) + 4 * axis_state(y, g.H) + 16 * axis_state(z, g.D);
 }
 
-// Appends v to list when pred; all lanes of the wave must call it.
-__device__ __forceinline__ void wave_append(int* list, int* counter, bool pred, int v) {
+// Appends v to list when pred; all lanes of the wave must call it.  A wave
+// with no appending lane does nothing, so rare appends cost one ballot.
+template <typename T>
+__device__ __forceinline__ void wave_append(T* list, int* counter, bool pred, T v) {
   unsigned long long m = __ballot(pred);
   if (m == 0) return;
   int lane = __lane_id();
@@ -735,8 +737,8 @@ __device__ __forceinline__ int block_best(int cand, double fc, const double* f, 
   return best;
 }
 
-// One full-grid pass over saddle voxels (rows of 64 per wavefront, XCD-aware
-// block order), replacing computeCP + findAscPaths / findDesPaths +
+// One pass over the saddle lists (one lane per saddle and side), replacing
+// computeCP + findAscPaths / findDesPaths +
 // computelargestSaddlesForMax / computesmallestSaddlesForMin
 // (src/ExtreM.cpp:1597-1679, 1920-2040, 2309-2371) without sorting.  The
 // upper / lower stars come from flow, the lower-star masks of field f.
@@ -745,7 +747,7 @@ __device__ __forceinline__ int block_best(int cand, double fc, const double* f, 
 //   ends with the highest (value, id) saddle reaching it.
 //  1-saddles: lower neighbours, descent roots L1; F1 = highest root, La1 =
 //   lowest; every listed minimum ends with the lowest saddle.
-// Per-extremum results live at the extremum's list position (ext_id): slot
+// Per-extremum results live at the extremum's list position (ExtRec): slot
 // pass A raises the position's ordered value key (atomics without return)
 // and records a (position, saddle) pair; k_slot_ids settles the id among the
 // saddles holding the best key.  A root outside the lists (stale types, D10)
@@ -754,23 +756,22 @@ __device__ __forceinline__ int block_best(int cand, double fc, const double* f, 
 // get_wrong_split_neighbors / get_wrong_join_neighbors
 // (src/ExtreM.cpp:2405-2450): bit d set when the d-th neighbour is in the
 // decp upper (lower) star and its root differs from the original root of the
-// same-rank original upper (lower) neighbour; and the per-block best maximum
-// / minimum under decp.  Stencil loops are unrolled and all loads of a stage
-// are issued before any is used.
+// same-rank original upper (lower) neighbour.  Stencil loops are unrolled and
+// all loads of a stage are issued before any is used.
 template <int SIDE>
 __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* f, const int* L,
                                             const int* Lo, int v, unsigned star, unsigned ostar,
-                                            const int* s_off, int* F, int* La, unsigned& cand,
-                                            int (&R)[kNbr], unsigned& uniq) {
+                                            const int* s_off, int* F, int* La, int o,
+                                            unsigned& cand, int (&E)[kNbr]) {
   const Geo& g = s.g;
-  int O[kNbr];
+  int R[kNbr], O[kNbr];
 #pragma unroll
   for (int d = 0; d < kNbr; d++) {
     const bool in = (star >> d) & 1;
     R[d] = in ? L[v + g.off[d]] : kNone;
     O[d] = (dec && in) ? Lo[v + g.off[d]] : kNone;
   }
-  uniq = 0;
+  unsigned uniq = 0;
 #pragma unroll
   for (int d = 0; d < kNbr; d++) {
     bool dup = R[d] == kNone;
@@ -778,9 +779,19 @@ __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* 
     for (int e = 0; e < d; e++) dup |= R[e] == R[d];
     if (!dup) uniq |= 1u << d;
   }
+  // distinct roots: value and slot from the extremum record (a root outside
+  // the lists has no record and takes its value from the field)
   double V[kNbr];
 #pragma unroll
-  for (int d = 0; d < kNbr; d++) V[d] = ((uniq >> d) & 1) ? f[R[d]] : 0.0;
+  for (int d = 0; d < kNbr; d++) {
+    ExtRec r = {0.0, -1, 0};
+    if ((uniq >> d) & 1) r = s.rec[R[d]];
+    E[d] = r.ext;
+    V[d] = r.f;
+  }
+#pragma unroll
+  for (int d = 0; d < kNbr; d++)
+    if (((uniq >> d) & 1) && E[d] < 0) V[d] = f[R[d]];
   int lo = kNone, hi = kNone;
   double flo = 0.0, fhi = 0.0;
 #pragma unroll
@@ -789,12 +800,9 @@ __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* 
     if (lo == kNone || smaller(V[d], R[d], flo, lo)) { lo = R[d]; flo = V[d]; }
     if (hi == kNone || larger(V[d], R[d], fhi, hi)) { hi = R[d]; fhi = V[d]; }
   }
-  if (SIDE == 2) {
-    F[v] = lo;
-    La[v] = hi;
-  } else {
-    F[v] = hi;
-    La[v] = lo;
+  if (o >= 0) {  // o < 0: a lane past the end of its list
+    F[o] = SIDE == 2 ? lo : hi;
+    La[o] = SIDE == 2 ? hi : lo;
   }
   cand = 0;
   if (!dec) return;
@@ -817,60 +825,25 @@ __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* 
   }
 }
 
-__global__ void __launch_bounds__(256) k_saddle_grid(St s, int dec, const double* f,
-                                                     const int* L2, const int* L1,
-                                                     const uint16_t* flow, int* F2, int* La2,
-                                                     int* F1, int* La1, int2* pairs,
-                                                     long long* kmax, long long* kmin,
-                                                     int* bmax, int* bmin) {
-  __shared__ int2 s_pairs[kPairBuf];
-  __shared__ int s_off[kNbr];
-  __shared__ int s_np, s_pbase;
-  const Geo& g = s.g;
-  const int tid = threadIdx.x + 64 * threadIdx.y;
-  if (tid < kNbr) s_off[tid] = c_dx[tid] + g.W * (c_dy[tid] + g.H * c_dz[tid]);
-  if (tid == 0) s_np = 0;
-  __syncthreads();
-  RowPos p = row_pos(g);
-  const int v = p.in ? p.x + g.W * (p.y + g.H * p.z) : 0;
-  const int t = p.in ? s.otype[v] : 5;
-  const bool s2 = t == 2 || t == 3, s1 = t == 1 || t == 3;
-  const double fs = (p.in && (s2 || s1 || (dec && (t == 4 || t == 0)))) ? f[v] : 0.0;
-  int R2[kNbr], R1[kNbr];
-  unsigned u2 = 0, u1 = 0, m2 = 0, m1 = 0;
-  if (s2 || s1) {
-    const unsigned key = c_cls_key[class_of(g, p.x, p.y, p.z)];
-    const unsigned fl = flow[v];
-    const unsigned ol = s.olow[v];
-    if (s2)
-      saddle_side<2>(s, dec, f, L2, s.oDS, v, key & ~fl, key & ~ol, s_off, F2, La2, m2, R2, u2);
-    if (s1) saddle_side<1>(s, dec, f, L1, s.oAS, v, key & fl, ol, s_off, F1, La1, m1, R1, u1);
-  }
-  // slot pass A on list positions: maxima have even ext_id, minima odd
-  int E2[kNbr], E1[kNbr];
-#pragma unroll
-  for (int d = 0; d < kNbr; d++) {
-    E2[d] = ((u2 >> d) & 1) ? s.ext_id[R2[d]] : -1;
-    E1[d] = ((u1 >> d) & 1) ? s.ext_id[R1[d]] : -1;
-  }
-  const long long kv = okey_c(fs);
+// Slot pass A for one side: raises (maxima, even positions) or lowers (minima,
+// odd positions) each distinct root's best saddle key and stages the
+// (position, saddle) pair in LDS, one LDS atomic per wavefront.  Called by
+// every lane of the block.
+template <int SIDE>
+__device__ __forceinline__ void emit_side(const St& s, int v, long long kv, int (&E)[kNbr],
+                                          long long* key_slot, int2* s_pairs, int* s_np,
+                                          int2* pairs) {
   int cnt = 0;
 #pragma unroll
   for (int d = 0; d < kNbr; d++) {
-    if (E2[d] >= 0 && !(E2[d] & 1)) {
-      atomicMax(&kmax[E2[d] >> 1], kv);
+    if (E[d] >= 0 && (E[d] & 1) == (SIDE == 1)) {
+      if (SIDE == 2) atomicMax(&key_slot[E[d] >> 1], kv);
+      else atomicMin(&key_slot[E[d] >> 1], kv);
       cnt++;
     } else {
-      E2[d] = -1;
-    }
-    if (E1[d] >= 0 && (E1[d] & 1)) {
-      atomicMin(&kmin[E1[d] >> 1], kv);
-      cnt++;
-    } else {
-      E1[d] = -1;
+      E[d] = -1;
     }
   }
-  // (position, saddle) pairs: one LDS atomic per wavefront, then plain stores
   int pre = cnt;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -880,51 +853,97 @@ __global__ void __launch_bounds__(256) k_saddle_grid(St s, int dec, const double
   const int total = __shfl(pre, 63);
   pre -= cnt;
   int base = 0;
-  if (__lane_id() == 0 && total) base = atomicAdd(&s_np, total);
+  if (__lane_id() == 0 && total) base = atomicAdd(s_np, total);
   base = __shfl(base, 0) + pre;
 #pragma unroll
   for (int d = 0; d < kNbr; d++) {
-    if (E2[d] >= 0) {
-      int2 pr = make_int2(E2[d], v);  // even: maximum
+    if (E[d] >= 0) {
+      int2 pr = make_int2(E[d], v);
       if (base < kPairBuf) s_pairs[base] = pr;
       else pairs[atomicAdd(&s.ctr[C_PAIRS], 1)] = pr;
       base++;
     }
-    if (E1[d] >= 0) {
-      int2 pr = make_int2(E1[d], v);  // odd: minimum
-      if (base < kPairBuf) s_pairs[base] = pr;
-      else pairs[atomicAdd(&s.ctr[C_PAIRS], 1)] = pr;
-      base++;
-    }
+  }
+}
+
+// One side for one saddle: the list entry i of S2 (SIDE 2) or S1 (SIDE 1).
+template <int SIDE>
+__device__ __forceinline__ void saddle_entry(const St& s, int dec, const double* f, const int* L,
+                                             const uint16_t* flow, int* F, int* La, int i,
+                                             bool act, const int* s_off, int2* s_pairs,
+                                             int* s_np, int2* pairs, long long* key_slot) {
+  const Geo& g = s.g;
+  int v = 0;
+  unsigned star = 0, ostar = 0;
+  double fs = 0.0;
+  if (act) {
+    v = (SIDE == 2 ? s.S2 : s.S1)[i];
+    int x, y, z;
+    coords(g, v, x, y, z);
+    const unsigned key = c_cls_key[class_of(g, x, y, z)];
+    const unsigned fl = flow[v], ol = s.olow[v];
+    star = SIDE == 2 ? key & ~fl : key & fl;
+    ostar = SIDE == 2 ? key & ~ol : ol;
+    fs = f[v];
+  }
+  int E[kNbr];
+  unsigned cand = 0;
+  saddle_side<SIDE>(s, dec, f, L, SIDE == 2 ? s.oDS : s.oAS, v, star, ostar, s_off, F, La,
+                    act ? i : -1, cand, E);
+  emit_side<SIDE>(s, v, okey_c(fs), E, key_slot, s_pairs, s_np, pairs);
+  if (dec) wave_append(s.act0, &s.ctr[C_ACT0], cand != 0, make_int2(2 * v + (SIDE == 1), (int)cand));
+}
+
+// Blocks [0, nb2) take S2 entries, the rest S1 entries (block order remapped
+// so each XCD walks one contiguous range: list order is spatial order, so the
+// neighbour rows of a range stay in that XCD's L2).  Triplet ends F2 / La2 /
+// F1 / La1 are written at the list position.
+__global__ void __launch_bounds__(256) k_saddle_list(St s, int dec, const double* f,
+                                                     const int* L2, const int* L1,
+                                                     const uint16_t* flow, int* F2, int* La2,
+                                                     int* F1, int* La1, int2* pairs,
+                                                     long long* kmax, long long* kmin, int nb2) {
+  __shared__ int2 s_pairs[kPairBuf];
+  __shared__ int s_off[kNbr];
+  __shared__ int s_np, s_pbase;
+  const Geo& g = s.g;
+  const int tid = threadIdx.x;
+  if (tid < kNbr) s_off[tid] = c_dx[tid] + g.W * (c_dy[tid] + g.H * c_dz[tid]);
+  if (tid == 0) s_np = 0;
+  __syncthreads();
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  if (b < nb2) {
+    const int i = b * 256 + tid;
+    saddle_entry<2>(s, dec, f, L2, flow, F2, La2, i, i < s.n2, s_off, s_pairs, &s_np, pairs,
+                    kmax);
+  } else {
+    const int i = (b - nb2) * 256 + tid;
+    saddle_entry<1>(s, dec, f, L1, flow, F1, La1, i, i < s.n1, s_off, s_pairs, &s_np, pairs,
+                    kmin);
   }
   __syncthreads();
   const int np = s_np < kPairBuf ? s_np : kPairBuf;
   if (tid == 0) s_pbase = np ? atomicAdd(&s.ctr[C_PAIRS], np) : 0;
   __syncthreads();
-  for (int i = tid; i < np; i += 256) pairs[s_pbase + i] = s_pairs[i];
-  if (dec) {
-    block_append(s.act0, &s.ctr[C_ACT0], m2 != 0, make_int2(2 * v, (int)m2));
-    block_append(s.act0, &s.ctr[C_ACT0], m1 != 0, make_int2(2 * v + 1, (int)m1));
-    int bm = block_best(t == 4 ? v : kNone, fs, f, true);
-    int bn = block_best(t == 0 ? v : kNone, fs, f, false);
-    if (tid == 0) {
-      bmax[blockIdx.x] = bm;
-      bmin[blockIdx.x] = bn;
-    }
-  }
+  for (int k = tid; k < np; k += 256) pairs[s_pbase + k] = s_pairs[k];
 }
 
-// Reduction of per-block extrema: out[block] = best of its grid-stride share.
-__global__ void __launch_bounds__(256) k_ext_reduce(const int* inmax, const int* inmin, int n,
-                                                    const double* f, int* outmax, int* outmin) {
+// Best (value, id) of two vertex lists under f: out[block] = the highest of
+// inmax's grid-stride share and the lowest of inmin's (kNone entries skipped).
+// Run over MX / MN, then over the per-block results with one block.
+__global__ void __launch_bounds__(256) k_ext_reduce(const int* inmax, int nmax, const int* inmin,
+                                                    int nmin, const double* f, int* outmax,
+                                                    int* outmin) {
   int cmax = kNone, cmin = kNone;
   double fmax = 0, fmin = 0;
-  GRID_STRIDE(i, n) {
+  GRID_STRIDE(i, nmax) {
     int a = inmax[i];
     if (a != kNone) {
       double fa = f[a];
       if (cmax == kNone || larger(fa, a, fmax, cmax)) { cmax = a; fmax = fa; }
     }
+  }
+  GRID_STRIDE(i, nmin) {
     int b = inmin[i];
     if (b != kNone) {
       double fb = f[b];
@@ -959,11 +978,21 @@ __global__ void __launch_bounds__(256) k_slot_ids(St s, const int2* pairs, const
   }
 }
 
-// ext_id[v] = 2 * (position in MX) for maxima, 2 * (position in MN) + 1 for
-// minima (set once per run; -1 elsewhere).
-__global__ void k_ext_index(St s) {
-  GRID_STRIDE(i, s.nmax) s.ext_id[s.MX[i]] = 2 * i;
-  GRID_STRIDE(i, s.nmin) s.ext_id[s.MN[i]] = 2 * i + 1;
+// Records of the listed extrema for a pass over field f: value and position
+// (2 * position in MX for maxima, 2 * position in MN + 1 for minima).
+__global__ void k_rec_fill(St s, const double* f) {
+  GRID_STRIDE(i, s.nmax) {
+    int m = s.MX[i];
+    s.rec[m] = ExtRec{f[m], 2 * i, 0};
+  }
+  GRID_STRIDE(i, s.nmin) {
+    int m = s.MN[i];
+    s.rec[m] = ExtRec{f[m], 2 * i + 1, 0};
+  }
+}
+
+__global__ void k_rec_clear(ExtRec* rec, long long n) {
+  GRID_STRIDE(i, n) rec[i] = ExtRec{0.0, -1, 0};
 }
 
 __global__ void k_fill_pair(long long* a, long long va, int* b, int vb, int n) {
@@ -1122,7 +1151,7 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
   STAGED_COUNT(cl, &s.ctr[side == 2 ? C_WMC2 : C_WMINC]);
   GRID_STRIDE(i, n) {
     int sv = S[i];
-    int a = of[sv], b = ol[sv], c = df[sv], d = dl[sv];
+    int a = of[i], b = ol[i], c = df[i], d = dl[i];
     bool mf = a != c, ml = b != d;
     if (phase == 0) {
       if (mf) atomicMin(&s.win_a[a], sv);
