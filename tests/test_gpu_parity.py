@@ -125,3 +125,25 @@ def test_nonfinite_input_rejected():
     with pytest.raises(ez().ExtremzError) as e:
         ez().correct(o, d, 0.1)
     assert e.value.code == -22
+
+
+def test_full_size_256_digest():
+    """The 256^3 benchmark workload, corrected on the GPU, against the digest
+    of the oracle's result (tests/golden/make_full_digest.py): the same fixed
+    field and edit counts byte for byte after 72 outer iterations."""
+    import hashlib
+    import json
+    import os
+
+    from extrem_z_amd import synth
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "gauss32_256_digest.json")))
+    r = ref["recipe"]
+    orig, decp, xi = synth.make_case(r["kind"], tuple(ref["shape"]), seed=r["seed"], er=r["er"],
+                                     n_gauss=r["n_gauss"])
+    assert xi == ref["xi"]
+    fixed, counts, stats = ez().correct(orig, decp, xi)
+    for k, v in ref["stats"].items():
+        assert stats[k] == v, k
+    assert hashlib.sha256(counts.tobytes()).hexdigest() == ref["sha256_counts"]
+    assert hashlib.sha256(fixed.tobytes()).hexdigest() == ref["sha256_fixed"]
