@@ -351,24 +351,22 @@ __global__ void __launch_bounds__(256) k_check_list(St s, const int* list, int l
                                                     int with_nbrs, int* front, int fctr,
                                                     int epoch) {
   STAGED_LIST(fl, int, kStage, front, &s.ctr[fctr]);
-  int n = s.ctr[lctr];
-  GRID_STRIDE(i, n) {
-    int v = list[i];
-    int cand[kNbr + 1];
-    int m = 0;
-    cand[m++] = v;
-    if (with_nbrs) {
+  // one lane per (entry, candidate): candidate 0 is the entry itself,
+  // candidate 1 + d its neighbour in direction d
+  const int per = with_nbrs ? kNbr + 1 : 1;
+  const long long n = (long long)s.ctr[lctr] * per;
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(k / per), j = (int)(k - (long long)i * per);
+    int u = list[i];
+    if (j > 0) {
       int x, y, z;
-      coords(s.g, v, x, y, z);
-      int key = c_cls_key[class_of(s.g, x, y, z)];
-      for (int d = 0; d < kNbr; d++)
-        if ((key >> d) & 1) cand[m++] = v + s.g.off[d];
+      coords(s.g, u, x, y, z);
+      if (!((c_cls_key[class_of(s.g, x, y, z)] >> (j - 1)) & 1)) continue;
+      u += s.g.off[j - 1];
     }
-    for (int j = 0; j < m; j++) {
-      int u = cand[j];
-      if (atomicExch(&s.stamp[u], epoch) == epoch) continue;
-      if (mismatch_at(s, u)) fl.push(u);
-    }
+    if (atomicExch(&s.stamp[u], epoch) == epoch) continue;
+    if (mismatch_at(s, u)) fl.push(u);
   }
   fl.flush();
 }
@@ -498,7 +496,7 @@ __global__ void __launch_bounds__(256) k_apply(St s) {
 constexpr int kLTX = 64, kLTY = 4, kLTZ = 4, kLTN = kLTX * kLTY * kLTZ;
 constexpr int kPX = kLTX + 2, kPY = kLTY + 2, kPN = kPX * kPY;
 constexpr int kPPer = (kPN + 255) / 256;
-constexpr int kPairBuf = 2048;  // (root, saddle) pairs staged in LDS per block
+constexpr int kPairBuf = 2048;  // (position, saddle) pairs staged in LDS per block
 
 __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int* ds, int* as,
                                                      uint16_t* lower) {
