@@ -141,7 +141,9 @@ __global__ void k_wn_reset(St s);
 __global__ void k_fixpath(St s);
 __global__ void k_wrong_index(St s, int side, int phase, int do_fix);
 __global__ void k_wrong_saddle(St s, int side, int do_fix);
-__global__ void k_lists(St s);
+__global__ void k_lists_count(St s, int* cnt, int nb);
+__global__ void k_lists_scan(St s, int* cnt, int nb);
+__global__ void k_lists_write(St s, const int* cnt, int nb);
 __global__ void k_presnap(St s);
 __global__ void k_argext(const double* f, int n, int* slot_min, int* slot_max);
 __global__ void k_count_edited(const uint8_t* cnt, long long n, int* ctr);

@@ -146,7 +146,8 @@ struct extremz_ctx {
   ez::DevBuf<uint8_t> cnt, otype, dflag, wn2, wn1;
   ez::DevBuf<uint16_t> olow, dlow;
   ez::DevBuf<int> oDS, oAS, dDS, dAS, stamp, touched, dirty, frontA, frontB;
-  ez::DevBuf<int> S2, S1, MX, MN, olarg, osmall, dlarg, dsmall, minsad2, minsad1, win_a, win_b;
+  ez::DevBuf<int> S2, S1, MX, MN, olarg, osmall, dlarg, dsmall, minsad2, minsad1, win_a, win_b,
+      lcnt;
   ez::DevBuf<ez::ExtRec> rec;
   ez::DevBuf<int> of2, ol2, of1, ol1, df2, dl2, df1, dl1, wnl2, wnl1, bmax2, bmin2;
   ez::DevBuf<int2> act0, actA, actB, pairs;
@@ -569,7 +570,13 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   tend();
   label_pass(d_orig, s_.oDS, s_.oAS, nullptr);
   // main.cpp:100-119: CP lists, triplets, largest/smallest saddle maps of orig
-  k_lists<<<(N + 255) / 256, 256, 0, q>>>(s_);
+  {
+    const int nb = (int)((N + 255) / 256);
+    c->lcnt.ensure(4LL * nb);
+    k_lists_count<<<nb, 256, 0, q>>>(s_, c->lcnt.p, nb);
+    k_lists_scan<<<1, 1024, 0, q>>>(s_, c->lcnt.p, nb);
+    k_lists_write<<<nb, 256, 0, q>>>(s_, c->lcnt.p, nb);
+  }
   pull();
   s_.n2 = h(C_N2);
   s_.n1 = h(C_N1);
@@ -696,6 +703,7 @@ static void free_ctx(extremz_ctx* c) {
   c->kmax.release();
   c->kmin.release();
   c->rec.release();
+  c->lcnt.release();
   c->olow.release();
   c->dlow.release();
   c->orig_tmp.release();

@@ -1220,15 +1220,87 @@ __global__ void __launch_bounds__(256) k_wrong_saddle(St s, int side, int do_fix
 }
 
 // --------------------------------------------------------------- setup ----
-// Appends every vertex to the saddle / extremum lists by original type.
-__global__ void __launch_bounds__(256) k_lists(St s) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  bool in = v < s.g.N;
-  int t = in ? s.otype[v] : 5;
-  block_append(s.S2, &s.ctr[C_N2], in && (t == 2 || t == 3), v);
-  block_append(s.S1, &s.ctr[C_N1], in && (t == 1 || t == 3), v);
-  block_append(s.MX, &s.ctr[C_NMAX], in && t == 4, v);
-  block_append(s.MN, &s.ctr[C_NMIN], in && t == 0, v);
+// The saddle / extremum lists by original type, each in increasing vertex
+// order (an ordered compaction: per-block counts, one scan, ordered writes),
+// so list neighbours are grid neighbours and the per-extremum slot arrays
+// indexed by list position are touched in spatial order.
+__device__ __forceinline__ unsigned list_flags(const St& s, int v) {
+  if (v >= s.g.N) return 0u;
+  int t = s.otype[v];
+  return (unsigned)(t == 2 || t == 3) | (unsigned)(t == 1 || t == 3) << 1 |
+         (unsigned)(t == 4) << 2 | (unsigned)(t == 0) << 3;
+}
+
+// cnt[c * nb + block] = number of category-c vertices in the block (256
+// vertices per block; categories 2-saddle, 1-saddle, maximum, minimum).
+__global__ void __launch_bounds__(256) k_lists_count(St s, int* cnt, int nb) {
+  __shared__ int w[4][4];
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  const unsigned fl = list_flags(s, v);
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    unsigned long long m = __ballot((fl >> c) & 1u);
+    if (__lane_id() == 0) w[c][wave] = __popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int c = threadIdx.x;
+    cnt[c * nb + blockIdx.x] = w[c][0] + w[c][1] + w[c][2] + w[c][3];
+  }
+}
+
+// In-place exclusive scan of each category's nb block counts (one block of
+// 1024 threads, a contiguous chunk per thread); totals to the list-length
+// counters C_N2, C_N1, C_NMAX, C_NMIN.
+__global__ void __launch_bounds__(1024) k_lists_scan(St s, int* cnt, int nb) {
+  __shared__ int sh[1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+  for (int c = 0; c < 4; c++) {
+    int* a = cnt + (long long)c * nb;
+    int sum = 0;
+    for (int b = b0; b < b1; b++) sum += a[b];
+    sh[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int x = t >= o ? sh[t - o] : 0;
+      __syncthreads();
+      sh[t] += x;
+      __syncthreads();
+    }
+    int run = sh[t] - sum;
+    for (int b = b0; b < b1; b++) {
+      int x = a[b];
+      a[b] = run;
+      run += x;
+    }
+    if (t == 1023) s.ctr[C_N2 + c] = sh[1023];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb) {
+  __shared__ int w[4][4];
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  const unsigned fl = list_flags(s, v);
+  const int wave = threadIdx.x >> 6, lane = __lane_id();
+  unsigned long long m[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    m[c] = __ballot((fl >> c) & 1u);
+    if (lane == 0) w[c][wave] = __popcll(m[c]);
+  }
+  __syncthreads();
+  int* lists[4] = {s.S2, s.S1, s.MX, s.MN};
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    if (!((fl >> c) & 1u)) continue;
+    int base = cnt[c * nb + blockIdx.x];
+    for (int k = 0; k < wave; k++) base += w[c][k];
+    lists[c][base + __popcll(m[c] & ((1ull << lane) - 1ull))] = v;
+  }
 }
 
 // main.cpp:74-75 and 122-131: decp := orig - xi at the global min / max and
