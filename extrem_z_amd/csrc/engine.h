@@ -42,6 +42,8 @@ enum Ctr {
   C_EDITED,
   C_ACT0,         // wrong-neighbour candidates of the last CP pass
   C_PAIRS,        // (extremum, saddle) pairs of the slot pass
+  C_NLONG,        // slots with long adjacency ranges
+  C_NCHUNK,       // chunks of those ranges
   C_COUNT_
 };
 
@@ -86,11 +88,22 @@ struct St {
   int *olarg, *osmall, *dlarg, *dsmall;  // per extremum, at its list position
   ExtRec* rec;                    // per vertex; valid at listed extrema
   int2* act0;                     // {2*saddle + side, candidate mask}
+  long long *pv2, *pv1;           // per list position: key for the slot gather
+  int *csr, *csr_off, *csr_cur;   // original adjacent saddles (list positions) per slot index
+  int2* longs;                    // {slot index, first chunk} of long adjacency ranges
+  int4* chunks;                   // {slot index, begin, end, -}
   uint8_t *wn2, *wn1;             // wrong_neighbors(_ds) flags
   int *wnl2, *wnl1;               // wrong_neighbors(_ds)_index
   int *minsad2, *minsad1;         // greedy reservations
   int *win_a, *win_b;             // first-saddle-wins reservations
 };
+
+// Modes of the saddle pass.  kModeOrig: original field, every saddle pushes
+// its roots to the slots and counts its (extremum, saddle) adjacencies;
+// kModeFill: original field, the adjacencies are written to the per-extremum
+// index; kModeDec: decp, only saddles whose root set differs from the
+// original one push (the others are gathered per extremum by k_slot_pull).
+constexpr int kModeOrig = 0, kModeFill = 1, kModeDec = 2;
 
 struct HipError : std::runtime_error {
   explicit HipError(const std::string& m) : std::runtime_error(m) {}
@@ -122,13 +135,22 @@ __global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t
 __global__ void k_jump(int n, int* ds, int* as);
 __global__ void k_chase(int n, int* ds, int* as);
 __global__ void k_list_fill(const int* list, int n, int* arr, int val);
-__global__ void k_saddle_list(St s, int dec, const double* f, const int* L2, const int* L1,
+__global__ void k_saddle_list(St s, int mode, const double* f, const int* L2, const int* L1,
                               const uint16_t* flow, int* F2, int* La2, int* F1, int* La1,
                               int2* pairs, long long* kmax, long long* kmin, int nb2);
 __global__ void k_slot_ids(St s, const int2* pairs, const double* f, const long long* kmax,
                            const long long* kmin, int* slot_max, int* slot_min);
 __global__ void k_rec_fill(St s, const double* f);
 __global__ void k_rec_clear(ExtRec* rec, long long n);
+__global__ void k_slot_pull(St s, long long* pkey, int* pid);
+__global__ void k_slot_pull_long(St s, long long* ckey, int* cpos);
+__global__ void k_slot_pull_join(St s, const long long* ckey, const int* cpos, long long* pkey,
+                                 int* pid);
+__global__ void k_long_slots(St s);
+__global__ void k_slot_merge(St s, long long* kmax, long long* kmin, int* slot_max,
+                             int* slot_min, const long long* pkey, const int* pid,
+                             int use_pull);
+__global__ void k_scan_single(int* a, int n);
 __global__ void k_fill_pair(long long* a, long long va, int* b, int vb, int n);
 __global__ void k_list_fill64(const int* list, int n, long long* arr, long long val);
 __global__ void k_ext_reduce(const int* inmax, int nmax, const int* inmin, int nmin,

@@ -147,7 +147,10 @@ struct extremz_ctx {
   ez::DevBuf<uint16_t> olow, dlow;
   ez::DevBuf<int> oDS, oAS, dDS, dAS, stamp, touched, dirty, frontA, frontB;
   ez::DevBuf<int> S2, S1, MX, MN, olarg, osmall, dlarg, dsmall, minsad2, minsad1, win_a, win_b,
-      lcnt;
+      lcnt, csr, csr_off, csr_cur, pid, cpos;
+  ez::DevBuf<long long> pkey, ckey, pv2, pv1;
+  ez::DevBuf<int2> longs;
+  ez::DevBuf<int4> chunks;
   ez::DevBuf<ez::ExtRec> rec;
   ez::DevBuf<int> of2, ol2, of1, ol1, df2, dl2, df1, dl1, wnl2, wnl1, bmax2, bmin2;
   ez::DevBuf<int2> act0, actA, actB, pairs;
@@ -295,9 +298,55 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
   set_ctr2(C_ACT0, C_PAIRS, 0);
   k_rec_fill<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(s_, f);
   const int nb2 = (s_.n2 + 255) / 256, nb1 = (s_.n1 + 255) / 256;
-  if (nb2 + nb1)
-    k_saddle_list<<<nb2 + nb1, 256, 0, st()>>>(s_, dec, f, L2, L1, flow, F2, La2, F1, La1,
-                                               c_->pairs.p, kmax, kmin, nb2);
+  const int ns = s_.nmax + s_.nmin;
+  if (!dec) {
+    // original field: every saddle pushes, and the (extremum, saddle)
+    // adjacencies are indexed per extremum for the decp passes
+    k_fill_i32<<<blocks_for(ns + 1), 256, 0, st()>>>(s_.csr_off, 0, ns + 1);
+    if (nb2 + nb1)
+      k_saddle_list<<<nb2 + nb1, 256, 0, st()>>>(s_, kModeOrig, f, L2, L1, flow, F2, La2, F1,
+                                                 La1, c_->pairs.p, kmax, kmin, nb2);
+    {  // exclusive scan of the ns counts (entry ns, zero, receives the total)
+      const long long n1 = (long long)ns + 1;
+      const int nblk = (int)((n1 + 1023) / 1024);
+      c_->lcnt.ensure((size_t)nblk + 1);
+      k_scan_block<<<nblk, 1024, 0, st()>>>(s_.csr_off, s_.csr_off, n1, c_->lcnt.p);
+      k_scan_single<<<1, 1024, 0, st()>>>(c_->lcnt.p, nblk);
+      k_scan_add<<<nblk, 1024, 0, st()>>>(s_.csr_off, n1, c_->lcnt.p);
+    }
+    int total = 0;
+    ez::hip_check(hipMemcpyAsync(&total, s_.csr_off + ns, sizeof(int), hipMemcpyDeviceToHost,
+                                 st()),
+                  "hipMemcpyAsync");
+    ez::hip_check(hipStreamSynchronize(st()), "hipStreamSynchronize");
+    c_->host_syncs++;
+    c_->csr.ensure((size_t)total + 1);
+    s_.csr = c_->csr.p;
+    // every long range has more than kPullShort (256) entries
+    const size_t nch = (size_t)total / 256 + (size_t)total / 1024 + 2;
+    c_->chunks.ensure(nch);
+    c_->ckey.ensure(nch);
+    c_->cpos.ensure(nch);
+    s_.chunks = c_->chunks.p;
+    // after the buffers above exist: the launch captures their pointers
+    set_ctr2(C_NLONG, C_NCHUNK, 0);
+    k_long_slots<<<blocks_for(ns), 256, 0, st()>>>(s_);
+    ez::hip_check(hipMemcpyAsync(s_.csr_cur, s_.csr_off, sizeof(int) * (ns + 1),
+                                 hipMemcpyDeviceToDevice, st()),
+                  "hipMemcpyAsync");
+    if (nb2 + nb1)
+      k_saddle_list<<<nb2 + nb1, 256, 0, st()>>>(s_, kModeFill, f, L2, L1, flow, F2, La2, F1,
+                                                 La1, c_->pairs.p, kmax, kmin, nb2);
+  } else {
+    if (nb2 + nb1)
+      k_saddle_list<<<nb2 + nb1, 256, 0, st()>>>(s_, kModeDec, f, L2, L1, flow, F2, La2, F1,
+                                                 La1, c_->pairs.p, kmax, kmin, nb2);
+    k_slot_pull<<<std::max(1, (ns + 31) / 32), 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
+    k_slot_pull_long<<<512, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p);
+    k_slot_pull_join<<<64, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p, c_->pkey.p, c_->pid.p);
+  }
+  k_slot_merge<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(
+      s_, kmax, kmin, smax, smin, c_->pkey.p, c_->pid.p, dec);
   k_slot_ids<<<blocks_for(14LL * (s_.n2 + s_.n1)), 256, 0, st()>>>(s_, c_->pairs.p, f, kmax, kmin,
                                                                    smax, smin);
   if (dec) {
@@ -588,6 +637,21 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   counts_[EXTREMZ_STAT_N_MIN] = s_.nmin;
   // every saddle records at most one pair per distinct root of its star
   c->pairs.ensure(14LL * (s_.n2 + s_.n1) + 1);
+  {
+    const long long ns = (long long)s_.nmax + s_.nmin;
+    c->csr_off.ensure(ns + 1);
+    c->csr_cur.ensure(ns + 1);
+    c->pid.ensure(ns + 1);
+    c->pkey.ensure(ns + 1);
+    c->longs.ensure(ns + 1);
+    s_.longs = c->longs.p;
+    c->pv2.ensure((size_t)s_.n2 + 1);
+    c->pv1.ensure((size_t)s_.n1 + 1);
+    s_.pv2 = c->pv2.p;
+    s_.pv1 = c->pv1.p;
+    s_.csr_off = c->csr_off.p;
+    s_.csr_cur = c->csr_cur.p;
+  }
 
   slot_pass(0, d_orig, s_.oDS, s_.oAS, s_.olow, s_.of2, s_.ol2, s_.of1, s_.ol1, s_.olarg,
             s_.osmall);
@@ -704,6 +768,10 @@ static void free_ctx(extremz_ctx* c) {
   c->kmin.release();
   c->rec.release();
   c->lcnt.release();
+  for (DevBuf<int>* b : {&c->csr, &c->csr_off, &c->csr_cur, &c->pid, &c->cpos}) b->release();
+  for (DevBuf<long long>* b : {&c->pkey, &c->ckey, &c->pv2, &c->pv1}) b->release();
+  c->longs.release();
+  c->chunks.release();
   c->olow.release();
   c->dlow.release();
   c->orig_tmp.release();

@@ -870,13 +870,19 @@ __device__ __forceinline__ void emit_side(const St& s, int v, long long kv, int 
   }
 }
 
+// Slot index of a list position: maxima first, then minima.
+__device__ __forceinline__ int slot_index(const St& s, int e) {
+  return (e & 1) ? s.nmax + (e >> 1) : (e >> 1);
+}
+
 // One side for one saddle: the list entry i of S2 (SIDE 2) or S1 (SIDE 1).
 template <int SIDE>
-__device__ __forceinline__ void saddle_entry(const St& s, int dec, const double* f, const int* L,
+__device__ __forceinline__ void saddle_entry(const St& s, int mode, const double* f, const int* L,
                                              const uint16_t* flow, int* F, int* La, int i,
                                              bool act, const int* s_off, int2* s_pairs,
                                              int* s_np, int2* pairs, long long* key_slot) {
   const Geo& g = s.g;
+  const int dec = mode == kModeDec;
   int v = 0;
   unsigned star = 0, ostar = 0;
   double fs = 0.0;
@@ -893,7 +899,34 @@ __device__ __forceinline__ void saddle_entry(const St& s, int dec, const double*
   int E[kNbr];
   unsigned cand = 0;
   saddle_side<SIDE>(s, dec, f, L, SIDE == 2 ? s.oDS : s.oAS, v, star, ostar, s_off, F, La,
-                    act ? i : -1, cand, E);
+                    (act && mode != kModeFill) ? i : -1, cand, E);
+#pragma unroll
+  for (int d = 0; d < kNbr; d++)
+    if (E[d] >= 0 && (E[d] & 1) != (SIDE == 1)) E[d] = -1;
+  if (mode == kModeOrig) {
+#pragma unroll
+    for (int d = 0; d < kNbr; d++)
+      if (E[d] >= 0) atomicAdd(&s.csr_off[slot_index(s, E[d])], 1);
+  } else if (mode == kModeFill) {
+#pragma unroll
+    for (int d = 0; d < kNbr; d++) {
+      if (E[d] >= 0) s.csr[atomicAdd(&s.csr_cur[slot_index(s, E[d])], 1)] = i;
+      E[d] = -1;
+    }
+  } else {
+    // same star and the same root at every star neighbour: the root set is
+    // the original one, gathered per extremum instead of pushed (its key at
+    // its list position; a changed saddle leaves the empty key there)
+    const bool changed = star != ostar || cand != 0;
+    if (act)
+      (SIDE == 2 ? s.pv2 : s.pv1)[i] =
+          changed ? (SIDE == 2 ? (long long)0x8000000000000000ULL : 0x7fffffffffffffffLL)
+                  : okey_c(fs);
+    if (!changed) {
+#pragma unroll
+      for (int d = 0; d < kNbr; d++) E[d] = -1;
+    }
+  }
   emit_side<SIDE>(s, v, okey_c(fs), E, key_slot, s_pairs, s_np, pairs);
   if (dec) wave_append(s.act0, &s.ctr[C_ACT0], cand != 0, make_int2(2 * v + (SIDE == 1), (int)cand));
 }
@@ -902,7 +935,7 @@ __device__ __forceinline__ void saddle_entry(const St& s, int dec, const double*
 // so each XCD walks one contiguous range: list order is spatial order, so the
 // neighbour rows of a range stay in that XCD's L2).  Triplet ends F2 / La2 /
 // F1 / La1 are written at the list position.
-__global__ void __launch_bounds__(256) k_saddle_list(St s, int dec, const double* f,
+__global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const double* f,
                                                      const int* L2, const int* L1,
                                                      const uint16_t* flow, int* F2, int* La2,
                                                      int* F1, int* La1, int2* pairs,
@@ -918,11 +951,11 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int dec, const double
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   if (b < nb2) {
     const int i = b * 256 + tid;
-    saddle_entry<2>(s, dec, f, L2, flow, F2, La2, i, i < s.n2, s_off, s_pairs, &s_np, pairs,
+    saddle_entry<2>(s, mode, f, L2, flow, F2, La2, i, i < s.n2, s_off, s_pairs, &s_np, pairs,
                     kmax);
   } else {
     const int i = (b - nb2) * 256 + tid;
-    saddle_entry<1>(s, dec, f, L1, flow, F1, La1, i, i < s.n1, s_off, s_pairs, &s_np, pairs,
+    saddle_entry<1>(s, mode, f, L1, flow, F1, La1, i, i < s.n1, s_off, s_pairs, &s_np, pairs,
                     kmin);
   }
   __syncthreads();
@@ -930,6 +963,196 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int dec, const double
   if (tid == 0) s_pbase = np ? atomicAdd(&s.ctr[C_PAIRS], np) : 0;
   __syncthreads();
   for (int k = tid; k < np; k += 256) pairs[s_pbase + k] = s_pairs[k];
+}
+
+// Slot gather for the unchanged saddles of a decp pass: for slot index j
+// (maxima, then minima) the best (key, list position) among the extremum's
+// original adjacent saddles, over the keys pv2 / pv1 that the saddle pass
+// left at the list positions (lists are in vertex order, so position order
+// is id order and (key, position) is the (value, id) order).  pkey = the
+// best key (the empty key if none), pid = its saddle vertex.  Slots with at
+// most kPullShort adjacencies take a group of kPullGroup lanes; longer ones
+// are cut into chunks of kPullChunk (k_long_slots), one block per chunk
+// (k_slot_pull_long), joined by k_slot_pull_join.
+constexpr int kPullShort = 256;
+constexpr int kPullGroup = 8;
+constexpr int kPullChunk = 1024;
+
+__device__ __forceinline__ bool pull_better(bool mx, long long ka, int pa, long long kb, int pb) {
+  return mx ? (ka > kb || (ka == kb && pa > pb)) : (ka < kb || (ka == kb && pa < pb));
+}
+
+__device__ __forceinline__ void pull_range(const St& s, bool mx, int k0, int k1, int step,
+                                           long long& bk, int& bp) {
+  const long long* pv = mx ? s.pv2 : s.pv1;
+  int k = k0;
+  for (; k + 3 * step < k1; k += 4 * step) {
+    int p[4];
+    long long q[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) p[u] = s.csr[k + u * step];
+#pragma unroll
+    for (int u = 0; u < 4; u++) q[u] = pv[p[u]];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (bp < 0 || pull_better(mx, q[u], p[u], bk, bp)) { bk = q[u]; bp = p[u]; }
+  }
+  for (; k < k1; k += step) {
+    const int p = s.csr[k];
+    const long long q = pv[p];
+    if (bp < 0 || pull_better(mx, q, p, bk, bp)) { bk = q; bp = p; }
+  }
+}
+
+__device__ __forceinline__ void pull_group(bool mx, int width, long long& bk, int& bp) {
+  for (int o = 1; o < width; o <<= 1) {
+    const long long ok = __shfl_xor(bk, o);
+    const int op = __shfl_xor(bp, o);
+    if (op >= 0 && (bp < 0 || pull_better(mx, ok, op, bk, bp))) { bk = ok; bp = op; }
+  }
+}
+
+__device__ __forceinline__ void pull_store(const St& s, int j, bool mx, long long bk, int bp,
+                                           long long* pkey, int* pid) {
+  const long long empty = mx ? (long long)0x8000000000000000ULL : 0x7fffffffffffffffLL;
+  const bool none = bp < 0 || bk == empty;
+  pkey[j] = none ? empty : bk;
+  pid[j] = none ? kNone : (mx ? s.S2 : s.S1)[bp];
+}
+
+__global__ void __launch_bounds__(256) k_slot_pull(St s, long long* pkey, int* pid) {
+  const int n = s.nmax + s.nmin;
+  const int gl = threadIdx.x & (kPullGroup - 1);
+  const int groups = gridDim.x * (blockDim.x / kPullGroup);
+  for (int j0 = blockIdx.x * (blockDim.x / kPullGroup); j0 < n; j0 += groups) {
+    const int j = j0 + threadIdx.x / kPullGroup;
+    int k0 = 0, k1 = 0;
+    if (j < n) {
+      k0 = s.csr_off[j];
+      k1 = s.csr_off[j + 1];
+    }
+    const bool mine = k1 - k0 <= kPullShort;  // else taken by the chunked path
+    const bool mx = j < s.nmax;
+    long long bk = 0;
+    int bp = -1;
+    if (mine) pull_range(s, mx, k0 + gl, k1, kPullGroup, bk, bp);
+    pull_group(mx, kPullGroup, bk, bp);
+    if (j < n && mine && gl == 0) pull_store(s, j, mx, bk, bp, pkey, pid);
+  }
+}
+
+// One block per chunk of a long slot: the chunk's best (key, position).
+__global__ void __launch_bounds__(256) k_slot_pull_long(St s, long long* ckey, int* cpos) {
+  __shared__ long long wk[4];
+  __shared__ int wp[4];
+  const int n = s.ctr[C_NCHUNK];
+  for (int c = blockIdx.x; c < n; c += gridDim.x) {
+    const int4 ch = s.chunks[c];  // {slot, k0, k1, -}
+    const bool mx = ch.x < s.nmax;
+    long long bk = 0;
+    int bp = -1;
+    pull_range(s, mx, ch.y + threadIdx.x, ch.z, 256, bk, bp);
+    pull_group(mx, 64, bk, bp);
+    if (__lane_id() == 0) {
+      wk[threadIdx.x >> 6] = bk;
+      wp[threadIdx.x >> 6] = bp;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < 4; w++)
+        if (wp[w] >= 0 && (bp < 0 || pull_better(mx, wk[w], wp[w], bk, bp))) {
+          bk = wk[w];
+          bp = wp[w];
+        }
+      ckey[c] = bk;
+      cpos[c] = bp;
+    }
+    __syncthreads();
+  }
+}
+
+// Joins the chunks of every long slot.
+__global__ void k_slot_pull_join(St s, const long long* ckey, const int* cpos, long long* pkey,
+                                 int* pid) {
+  const int n = s.ctr[C_NLONG];
+  GRID_STRIDE(q, n) {
+    const int2 l = s.longs[q];  // {slot, first chunk}
+    const bool mx = l.x < s.nmax;
+    const int nch = (s.csr_off[l.x + 1] - s.csr_off[l.x] + kPullChunk - 1) / kPullChunk;
+    long long bk = 0;
+    int bp = -1;
+    for (int c = l.y; c < l.y + nch; c++)
+      if (cpos[c] >= 0 && (bp < 0 || pull_better(mx, ckey[c], cpos[c], bk, bp))) {
+        bk = ckey[c];
+        bp = cpos[c];
+      }
+    pull_store(s, l.x, mx, bk, bp, pkey, pid);
+  }
+}
+
+// Lists the slots with more than kPullShort adjacencies and cuts them into
+// chunks of kPullChunk.
+__global__ void k_long_slots(St s) {
+  const int n = s.nmax + s.nmin;
+  GRID_STRIDE(j, n) {
+    const int k0 = s.csr_off[j], k1 = s.csr_off[j + 1];
+    if (k1 - k0 <= kPullShort) continue;
+    const int nch = (k1 - k0 + kPullChunk - 1) / kPullChunk;
+    const int c0 = atomicAdd(&s.ctr[C_NCHUNK], nch);
+    s.longs[atomicAdd(&s.ctr[C_NLONG], 1)] = make_int2(j, c0);
+    for (int c = 0; c < nch; c++)
+      s.chunks[c0 + c] = make_int4(j, k0 + c * kPullChunk, min(k1, k0 + (c + 1) * kPullChunk), 0);
+  }
+}
+
+// Final slot keys: the better of the gathered and the pushed key; the slot
+// starts with the gathered saddle when it holds the final key (k_slot_ids
+// then adds the pushed saddles holding it).
+__global__ void k_slot_merge(St s, long long* kmax, long long* kmin, int* slot_max,
+                             int* slot_min, const long long* pkey, const int* pid,
+                             int use_pull) {
+  if (!use_pull) {  // every saddle pushed
+    GRID_STRIDE(j, s.nmax) slot_max[j] = kNone;
+    GRID_STRIDE(j, s.nmin) slot_min[j] = 0x7fffffff;
+    return;
+  }
+  GRID_STRIDE(j, s.nmax) {
+    const long long k = kmax[j] > pkey[j] ? kmax[j] : pkey[j];
+    kmax[j] = k;
+    slot_max[j] = (pid[j] != kNone && pkey[j] == k) ? pid[j] : kNone;
+  }
+  GRID_STRIDE(j, s.nmin) {
+    const int q = s.nmax + j;
+    const long long k = kmin[j] < pkey[q] ? kmin[j] : pkey[q];
+    kmin[j] = k;
+    slot_min[j] = (pid[q] != kNone && pkey[q] == k) ? pid[q] : 0x7fffffff;
+  }
+}
+
+// In-place exclusive scan of a[0..n) by one block of 1024 threads; a[n]
+// receives the total.
+__global__ void __launch_bounds__(1024) k_scan_single(int* a, int n) {
+  __shared__ int sh[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b0 = min(n, t * per), b1 = min(n, b0 + per);
+  int sum = 0;
+  for (int b = b0; b < b1; b++) sum += a[b];
+  sh[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int x = t >= o ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  int run = sh[t] - sum;
+  for (int b = b0; b < b1; b++) {
+    int x = a[b];
+    a[b] = run;
+    run += x;
+  }
+  if (t == 1023) a[n] = sh[1023];
 }
 
 // Best (value, id) of two vertex lists under f: out[block] = the highest of
