@@ -47,16 +47,6 @@ enum Ctr {
   C_COUNT_
 };
 
-// Per-vertex record of a listed extremum, refreshed at every slot pass: its
-// value in the field of that pass and its list position (2*i in MX, 2*i+1 in
-// MN; -1 for every other vertex).  One 16-byte gather gives a saddle pass
-// both the ordering value and the slot of a root.
-struct __align__(16) ExtRec {
-  double f;
-  int ext;
-  int pad;
-};
-
 struct Geo {
   int W, H, D, WH, N;
   int off[kNbr];  // index offset of each stencil direction
@@ -86,7 +76,7 @@ struct St {
   int *of2, *ol2, *of1, *ol1;     // original triplet first / last (per list entry)
   int *df2, *dl2, *df1, *dl1;     // decp triplet first / last (per list entry)
   int *olarg, *osmall, *dlarg, *dsmall;  // per extremum, at its list position
-  ExtRec* rec;                    // per vertex; valid at listed extrema
+  int* ext_id;                    // 2*position in MX (maxima) / 2*position+1 in MN, else -1
   int2* act0;                     // {2*saddle + side, candidate mask}
   long long *pv2, *pv1;           // per list position: key for the slot gather
   int *csr, *csr_off, *csr_cur;   // original adjacent saddles (list positions) per slot index
@@ -140,8 +130,7 @@ __global__ void k_saddle_list(St s, int mode, const double* f, const int* L2, co
                               int2* pairs, long long* kmax, long long* kmin, int nb2);
 __global__ void k_slot_ids(St s, const int2* pairs, const double* f, const long long* kmax,
                            const long long* kmin, int* slot_max, int* slot_min);
-__global__ void k_rec_fill(St s, const double* f);
-__global__ void k_rec_clear(ExtRec* rec, long long n);
+__global__ void k_ext_index(St s);
 __global__ void k_slot_pull(St s, long long* pkey, int* pid);
 __global__ void k_slot_pull_long(St s, long long* ckey, int* cpos);
 __global__ void k_slot_pull_join(St s, const long long* ckey, const int* cpos, long long* pkey,

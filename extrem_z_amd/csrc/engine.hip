@@ -151,7 +151,7 @@ struct extremz_ctx {
   ez::DevBuf<long long> pkey, ckey, pv2, pv1;
   ez::DevBuf<int2> longs;
   ez::DevBuf<int4> chunks;
-  ez::DevBuf<ez::ExtRec> rec;
+  ez::DevBuf<int> ext_id;
   ez::DevBuf<int> of2, ol2, of1, ol1, df2, dl2, df1, dl1, wnl2, wnl1, bmax2, bmin2;
   ez::DevBuf<int2> act0, actA, actB, pairs;
   ez::DevBuf<int> ctr;
@@ -265,7 +265,7 @@ void Run::allocate(long long N) {
   c->delta.ensure(N);
   c->kmax.ensure(N);
   c->kmin.ensure(N);
-  c->rec.ensure(N);
+  c->ext_id.ensure(N);
   for (DevBuf<uint8_t>* b : {&c->cnt, &c->otype, &c->dflag, &c->wn2, &c->wn1}) b->ensure(N);
   c->olow.ensure(N);
   c->dlow.ensure(N);
@@ -296,7 +296,6 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
   k_fill_pair<<<blocks_for(s_.nmin), 256, 0, st()>>>(kmin, 0x7fffffffffffffffLL, smin, 0x7fffffff,
                                                      s_.nmin);
   set_ctr2(C_ACT0, C_PAIRS, 0);
-  k_rec_fill<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(s_, f);
   const int nb2 = (s_.n2 + 255) / 256, nb1 = (s_.n1 + 255) / 256;
   const int ns = s_.nmax + s_.nmin;
   if (!dec) {
@@ -583,7 +582,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   s_.dlarg = c->dlarg.p;
   s_.dsmall = c->dsmall.p;
   s_.act0 = c->act0.p;
-  s_.rec = c->rec.p;
+  s_.ext_id = c->ext_id.p;
   s_.wn2 = c->wn2.p;
   s_.wn1 = c->wn1.p;
   s_.wnl2 = c->wnl2.p;
@@ -606,7 +605,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   ez::hip_check(hipMemsetAsync(s_.wn1, 0, N, q), "memset");
   for (int* p : {s_.minsad2, s_.minsad1, s_.win_a, s_.win_b})
     k_fill_i32<<<bN, 256, 0, q>>>(p, 0x7fffffff, N);
-  k_rec_clear<<<bN, 256, 0, q>>>(s_.rec, N);
+  k_fill_i32<<<bN, 256, 0, q>>>(s_.ext_id, -1, N);
   set_ctr2(C_GMIN, C_GMAX, kNone);
   pull();
   if (h(C_FAIL)) return EXTREMZ_EINVAL;
@@ -637,6 +636,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   counts_[EXTREMZ_STAT_N_MIN] = s_.nmin;
   // every saddle records at most one pair per distinct root of its star
   c->pairs.ensure(14LL * (s_.n2 + s_.n1) + 1);
+  k_ext_index<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, q>>>(s_);
   {
     const long long ns = (long long)s_.nmax + s_.nmin;
     c->csr_off.ensure(ns + 1);
@@ -766,7 +766,7 @@ static void free_ctx(extremz_ctx* c) {
   c->delta.release();
   c->kmax.release();
   c->kmin.release();
-  c->rec.release();
+  c->ext_id.release();
   c->lcnt.release();
   for (DevBuf<int>* b : {&c->csr, &c->csr_off, &c->csr_cur, &c->pid, &c->cpos}) b->release();
   for (DevBuf<long long>* b : {&c->pkey, &c->ckey, &c->pv2, &c->pv1}) b->release();

@@ -745,7 +745,7 @@ __device__ __forceinline__ int block_best(int cand, double fc, const double* f, 
 //   ends with the highest (value, id) saddle reaching it.
 //  1-saddles: lower neighbours, descent roots L1; F1 = highest root, La1 =
 //   lowest; every listed minimum ends with the lowest saddle.
-// Per-extremum results live at the extremum's list position (ExtRec): slot
+// Per-extremum results live at the extremum's list position (ext_id): slot
 // pass A raises the position's ordered value key (atomics without return)
 // and records a (position, saddle) pair; k_slot_ids settles the id among the
 // saddles holding the best key.  A root outside the lists (stale types, D10)
@@ -760,7 +760,7 @@ template <int SIDE>
 __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* f, const int* L,
                                             const int* Lo, int v, unsigned star, unsigned ostar,
                                             const int* s_off, int* F, int* La, int o,
-                                            unsigned& cand, int (&E)[kNbr]) {
+                                            unsigned& cand, int (&U)[kNbr]) {
   const Geo& g = s.g;
   int R[kNbr], O[kNbr];
 #pragma unroll
@@ -777,19 +777,12 @@ __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* 
     for (int e = 0; e < d; e++) dup |= R[e] == R[d];
     if (!dup) uniq |= 1u << d;
   }
-  // distinct roots: value and slot from the extremum record (a root outside
-  // the lists has no record and takes its value from the field)
   double V[kNbr];
 #pragma unroll
   for (int d = 0; d < kNbr; d++) {
-    ExtRec r = {0.0, -1, 0};
-    if ((uniq >> d) & 1) r = s.rec[R[d]];
-    E[d] = r.ext;
-    V[d] = r.f;
+    V[d] = ((uniq >> d) & 1) ? f[R[d]] : 0.0;
+    U[d] = ((uniq >> d) & 1) ? R[d] : kNone;  // the distinct roots
   }
-#pragma unroll
-  for (int d = 0; d < kNbr; d++)
-    if (((uniq >> d) & 1) && E[d] < 0) V[d] = f[R[d]];
   int lo = kNone, hi = kNone;
   double flo = 0.0, fhi = 0.0;
 #pragma unroll
@@ -900,9 +893,19 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
   unsigned cand = 0;
   saddle_side<SIDE>(s, dec, f, L, SIDE == 2 ? s.oDS : s.oAS, v, star, ostar, s_off, F, La,
                     (act && mode != kModeFill) ? i : -1, cand, E);
+  // list positions of the distinct roots (maxima even, minima odd; roots
+  // outside the lists have none) where this saddle pushes or is indexed
+  const bool changed = dec && (star != ostar || cand != 0);
+  if (!dec || changed) {
 #pragma unroll
-  for (int d = 0; d < kNbr; d++)
-    if (E[d] >= 0 && (E[d] & 1) != (SIDE == 1)) E[d] = -1;
+    for (int d = 0; d < kNbr; d++) {
+      const int e = E[d] >= 0 ? s.ext_id[E[d]] : -1;
+      E[d] = (e >= 0 && (e & 1) == (SIDE == 1)) ? e : -1;
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < kNbr; d++) E[d] = -1;
+  }
   if (mode == kModeOrig) {
 #pragma unroll
     for (int d = 0; d < kNbr; d++)
@@ -917,15 +920,10 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
     // same star and the same root at every star neighbour: the root set is
     // the original one, gathered per extremum instead of pushed (its key at
     // its list position; a changed saddle leaves the empty key there)
-    const bool changed = star != ostar || cand != 0;
     if (act)
       (SIDE == 2 ? s.pv2 : s.pv1)[i] =
           changed ? (SIDE == 2 ? (long long)0x8000000000000000ULL : 0x7fffffffffffffffLL)
                   : okey_c(fs);
-    if (!changed) {
-#pragma unroll
-      for (int d = 0; d < kNbr; d++) E[d] = -1;
-    }
   }
   emit_side<SIDE>(s, v, okey_c(fs), E, key_slot, s_pairs, s_np, pairs);
   if (dec) wave_append(s.act0, &s.ctr[C_ACT0], cand != 0, make_int2(2 * v + (SIDE == 1), (int)cand));
@@ -1205,21 +1203,11 @@ __global__ void __launch_bounds__(256) k_slot_ids(St s, const int2* pairs, const
   }
 }
 
-// Records of the listed extrema for a pass over field f: value and position
-// (2 * position in MX for maxima, 2 * position in MN + 1 for minima).
-__global__ void k_rec_fill(St s, const double* f) {
-  GRID_STRIDE(i, s.nmax) {
-    int m = s.MX[i];
-    s.rec[m] = ExtRec{f[m], 2 * i, 0};
-  }
-  GRID_STRIDE(i, s.nmin) {
-    int m = s.MN[i];
-    s.rec[m] = ExtRec{f[m], 2 * i + 1, 0};
-  }
-}
-
-__global__ void k_rec_clear(ExtRec* rec, long long n) {
-  GRID_STRIDE(i, n) rec[i] = ExtRec{0.0, -1, 0};
+// ext_id[v] = 2 * (position in MX) for maxima, 2 * (position in MN) + 1 for
+// minima (set once per run; -1 elsewhere).
+__global__ void k_ext_index(St s) {
+  GRID_STRIDE(i, s.nmax) s.ext_id[s.MX[i]] = 2 * i;
+  GRID_STRIDE(i, s.nmin) s.ext_id[s.MN[i]] = 2 * i + 1;
 }
 
 __global__ void k_fill_pair(long long* a, long long va, int* b, int vb, int n) {
