@@ -44,6 +44,7 @@ enum Ctr {
   C_PAIRS,        // (extremum, saddle) pairs of the slot pass
   C_NLONG,        // slots with long adjacency ranges
   C_NCHUNK,       // chunks of those ranges
+  C_WN_BAIL,      // the one-block reservation solver left the list to the rounds
   C_COUNT_
 };
 
@@ -149,6 +150,7 @@ __global__ void k_wn_register(St s, int2* active, int actr);
 __global__ void k_wn_commit(St s, int2* active, int actr);
 __global__ void k_wn_next(St s, const int2* active, int actr, int2* next, int nctr);
 __global__ void k_wn_reset(St s);
+__global__ void k_wn_solve(St s, int2* bufA, int2* bufB, int limit);
 __global__ void k_fixpath(St s);
 __global__ void k_wrong_index(St s, int side, int phase, int do_fix);
 __global__ void k_wrong_saddle(St s, int side, int do_fix);

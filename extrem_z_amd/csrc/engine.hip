@@ -451,6 +451,11 @@ void Run::cp_dec(bool force) {
 // saddle-id order (main.cpp:309-323), as rounds of deterministic reservations
 // on the candidates of the last CP pass.
 void Run::wrong_neighbors() {
+  // short candidate lists: every round in one block, one host read
+  k_wn_solve<<<1, 1024, 0, st()>>>(s_, c_->actA.p, c_->actB.p, 32768);
+  pull();
+  if (h(C_WN_BAIL) == 2) throw HipError("wrong-neighbour reservations did not finish");
+  if (!h(C_WN_BAIL)) return;
   k_wn_reset<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
   set_ctr2(C_NFC, C_NFC1, 0);
   int2* act = c_->actA.p;

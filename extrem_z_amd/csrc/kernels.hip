@@ -1299,6 +1299,109 @@ __global__ void __launch_bounds__(256) k_wn_next(St s, const int2* active, int a
   nl.flush();
 }
 
+// All rounds of the reservations above in one block of 1024 threads, for
+// candidate lists of at most `limit` entries (else C_WN_BAIL is set and
+// nothing is done): the flag reset of k_wn_reset, the copy of k_act_copy and
+// the register / commit / next phases, separated by block barriers (which
+// order the block's plain accesses).  The reservations are atomics, which the
+// CU's L1 does not see: they are read and cleared with atomic loads and
+// stores.
+__global__ void __launch_bounds__(1024) k_wn_solve(St s, int2* bufA, int2* bufB, int limit) {
+  __shared__ int n_cur, n_next, n_fc, n_fc1, bail;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    const int n0 = s.ctr[C_ACT0];
+    bail = n0 > limit;
+    s.ctr[C_WN_BAIL] = bail;
+    n_cur = n0;
+    n_next = 0;
+    n_fc = 0;
+    n_fc1 = 0;
+  }
+  __syncthreads();
+  if (bail) return;
+  uint8_t* wn2 = s.wn2;
+  uint8_t* wn1 = s.wn1;
+  {
+    const int n = s.ctr[C_NFC], n1 = s.ctr[C_NFC1];
+    for (int i = t; i < n; i += 1024) wn2[s.wnl2[i]] = 0;
+    for (int i = t; i < n1; i += 1024) wn1[s.wnl1[i]] = 0;
+  }
+  for (int i = t; i < n_cur; i += 1024) bufA[i] = s.act0[i];
+  __syncthreads();
+  int2* act = bufA;
+  int2* nxt = bufB;
+  for (int round = 0; n_cur > 0 && round <= limit; round++) {
+    const int n = n_cur;
+    int2* va = act;
+    // register (k_wn_register)
+    for (int a = t; a < n; a += 1024) {
+      const int ex = va[a].x;
+      unsigned mask = (unsigned)va[a].y;
+      const int sv = ex >> 1, side = ex & 1;
+      const uint8_t* fl = side ? wn1 : wn2;
+      int* ms = side ? s.minsad1 : s.minsad2;
+      for (int d = 0; d < kNbr; d++) {
+        if (!((mask >> d) & 1)) continue;
+        const int r = sv + s.g.off[d];
+        if (fl[r]) {
+          mask &= ~(1u << d);
+          continue;
+        }
+        atomicMin(&ms[r], sv);
+      }
+      va[a].y = (int)mask;
+    }
+    __syncthreads();
+    // commit (k_wn_commit)
+    for (int a = t; a < n; a += 1024) {
+      const int ex = va[a].x;
+      const unsigned mask = (unsigned)va[a].y;
+      if (!mask) continue;
+      const int sv = ex >> 1, side = ex & 1;
+      const int r = sv + s.g.off[__ffs(mask) - 1];
+      int* ms = side ? s.minsad1 : s.minsad2;
+      if (__hip_atomic_load(&ms[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == sv) {
+        if (!side) {
+          wn2[r] = 1;
+          s.wnl2[atomicAdd(&n_fc, 1)] = r;
+        } else {
+          wn1[r] = 1;
+          s.wnl1[atomicAdd(&n_fc1, 1)] = r;
+        }
+        va[a].y = (int)(mask | 0x8000u);  // done
+      }
+    }
+    __syncthreads();
+    // release and compaction (k_wn_next)
+    for (int a = t; a < n; a += 1024) {
+      const int ex = va[a].x;
+      const unsigned mask = (unsigned)va[a].y;
+      const int sv = ex >> 1, side = ex & 1;
+      int* ms = side ? s.minsad1 : s.minsad2;
+      for (int d = 0; d < kNbr; d++)
+        if ((mask >> d) & 1)
+          __hip_atomic_store(&ms[sv + s.g.off[d]], 0x7fffffff, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+      if (mask && !(mask & 0x8000u)) nxt[atomicAdd(&n_next, 1)] = make_int2(ex, (int)mask);
+    }
+    __syncthreads();
+    if (t == 0) {
+      n_cur = n_next;
+      n_next = 0;
+    }
+    int2* tmp = act;
+    act = nxt;
+    nxt = tmp;
+    __syncthreads();
+  }
+  if (t == 0) {
+    s.ctr[C_NFC] = n_fc;
+    s.ctr[C_NFC1] = n_fc1;
+    if (n_cur > 0) s.ctr[C_WN_BAIL] = 2;  // every round commits one saddle: unreachable
+  }
+}
+
 __global__ void k_wn_reset(St s) {
   int n = s.ctr[C_NFC];
   GRID_STRIDE(i, n) s.wn2[s.wnl2[i]] = 0;
