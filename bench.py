@@ -1,20 +1,31 @@
 """Benchmark: corrected voxels/s of the ExtreM correction on one MI355X.
 
-Workload (default): one float32 field 256x256x256, sum of 32 anisotropic
+Workload (default "256"): one float32 field 256x256x256, sum of 32 anisotropic
 Gaussians + 1e-3*range uniform noise (seed 2), xi = 1e-4 * range, decp the
 uniform quantiser 2*xi*rint(orig/(2*xi)).  A step is one full correction call
 (main.cpp:65-446 semantics) on device-resident inputs; each step gets its own
-pristine copy of decp, prepared before the timed region.
+pristine copy of decp, prepared before the timed region.  "512" is the
+512^3 Gaussian random field (k^-3, xi = 1e-3); "64" the 64^3 CPU-sized case.
 
-Prints one JSON line.  roofline is for the kernel family with the largest
-summed time (HIP events on the engine's stream); cpu_baseline is the CPU
-oracle on a bounded sample of the same workload recipe.
+Prints one JSON line.  The roofline object is for the dominant single kernel:
+the kernel family with the largest summed time in the warm-up call (every
+family timed there); the timed steps record HIP events around that family's
+launches only.  achieved = algorithmic bytes per launch / average launch time,
+with the per-unit bytes of ALG_BYTES.  traffic is the measured HBM bytes per
+launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a
+plain copy) when a summary for this kernel and workload exists under
+profiles/.  cpu_baseline is the CPU oracle on a bounded sample of the same
+workload recipe.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 256|512|64]
+    python bench.py --calibrate-copy      # plain 1 GiB copies, for the counter scale
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -24,14 +35,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "corrected voxels/s + % HBM roofline (byte-exact vs CPU); p50/p99 job latency"
 
-# Algorithmic HBM bytes per voxel for one launch of each full-grid family.
-BYTES_PER_VOXEL = {
-    "classify": 8 + 1 + 2,        # read f64 field, write u8 type + u16 mask
-    "check_full": 8 + 2,          # read f64 decp + u16 original mask
-    "steepest": 8 + 4 + 4,        # label tiles: read f64 field, write two i32 pointers
-    "jump": 4 * 4,                # read + write both i32 label arrays
-    "saddle": 1 + 8 + 8 + 2 + 8,  # saddle pass: type, decp, two decp roots, mask, two orig roots
+# Algorithmic bytes per unit for the single-kernel families (unit, bytes):
+#   label_tile  per voxel: read decp f64 (8) + write both pointer labels (4 + 4)
+#               + write the lower-star mask (2)
+#   saddle_pass per saddle side: one frontier item, 15 x (8 B value + 4 B label)
+#   classify / check_full per voxel: read f64 (8) + type (1) + mask (2)
+#   label_jump / label_chase: extra pointer-jumping rounds carry no algorithmic bytes
+ALG_BYTES = {
+    "label_tile": ("voxel", 18),
+    "saddle_pass": ("saddle side", 180),
+    "classify": ("voxel", 11),
+    "check_full": ("voxel", 11),
+    "label_jump": ("voxel", 0),
+    "label_chase": ("voxel", 0),
 }
 
 
@@ -51,19 +69,48 @@ def workload(name):
     return o, d, xi, desc
 
 
-def cpu_baseline(seconds_budget=20.0, threads=16):
-    """Oracle on a bounded sample: a 96^3 field built by the same recipe."""
+def units(fam, st, n):
+    if ALG_BYTES[fam][0] == "saddle side":
+        return st["n_saddle1"] + st["n_saddle2"]
+    return n
+
+
+def traffic_from_profiles(fam, workload_name):
+    """HBM bytes per launch from profiles/<fam>_<workload>_traffic.md, if present."""
+    path = os.path.join(ROOT, "profiles", f"{fam}_{workload_name}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(threads):
+    """Oracle on a bounded sample: a 80^3 field built by the workload's recipe."""
     from extrem_z_amd import synth
     from oracle import oracle
-    o, d, xi = synth.make_case("gauss", (96, 96, 96), seed=2, er=1e-4, n_gauss=2)
+    o, d, xi = synth.make_case("gauss", (80, 80, 80), seed=2, er=1e-4, n_gauss=2)
     t = time.perf_counter()
     f, c, st = oracle.correct(o, d, xi, threads=threads)
     dt = time.perf_counter() - t
-    n = o.size
-    return {"value": n / dt, "unit": "voxels/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ restatement, OpenMP {threads} threads) on one 96^3 field, "
+    return {"value": o.size / dt, "unit": "voxels/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ restatement, OpenMP {threads} threads) on one 80^3 field, "
                       f"2 Gaussians + 1e-3 noise, er=1e-4: {dt:.2f} s, "
                       f"{st['outer_iters']} outer iterations"}
+
+
+def calibrate_copy():
+    import torch
+    n = 1 << 30
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / 5
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copies": 5,
+                      "GB_per_s": 2 * n / dt / 1e9}), flush=True)
 
 
 def main():
@@ -74,11 +121,15 @@ def main():
     ap.add_argument("--workload", default="256")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--calibrate-copy", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: the multi-GPU path is not part of this build "
-              "(fields and jobs shard whole per GPU; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: fields and jobs shard whole per GPU and this build "
+              "runs on a single MI355X; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     import torch
     import extrem_z_amd as ez
@@ -90,20 +141,25 @@ def main():
           file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")
     d_orig = torch.from_numpy(orig.ravel()).to(dev)
-    copies = [torch.from_numpy(decp.ravel()).to(dev) for _ in range(args.steps + args.warmup)]
+    copies = [torch.from_numpy(decp.ravel()).to(dev) for _ in range(args.steps + max(1, args.warmup))]
     d_counts = torch.zeros(N, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
     ctx = ez.Context(0)
-    for i in range(args.warmup):
-        st = ctx.correct_device(d_orig.data_ptr(), copies[i].data_ptr(), (W, H, D), xi,
-                                d_counts.data_ptr())
-        print(f"bench: warmup {i} done, {st['wall_ms']:.1f} ms", file=sys.stderr, flush=True)
+    warm = None
+    for i in range(max(1, args.warmup)):
+        warm = ctx.correct_device(d_orig.data_ptr(), copies[i].data_ptr(), (W, H, D), xi,
+                                  d_counts.data_ptr(), timing=True)
+        print(f"bench: warmup {i} done, {warm['wall_ms']:.1f} ms", file=sys.stderr, flush=True)
+    # dominant single kernel of the warm-up call
+    dom = max(ALG_BYTES, key=lambda k: warm["kernel_ms"][k])
+    mask = 1 << ez.KERNEL_FAMILIES.index(dom)
     torch.cuda.synchronize()
     stats = []
+    base = max(1, args.warmup)
     t0 = time.perf_counter()
     for i in range(args.steps):
-        st = ctx.correct_device(d_orig.data_ptr(), copies[args.warmup + i].data_ptr(), (W, H, D),
-                                xi, d_counts.data_ptr(), timing=True)
+        st = ctx.correct_device(d_orig.data_ptr(), copies[base + i].data_ptr(), (W, H, D), xi,
+                                d_counts.data_ptr(), timing=mask)
         stats.append(st)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -112,20 +168,18 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     value = N * args.steps / elapsed
 
-    # roofline of the dominant full-grid kernel family
-    fam_ms = {k: sum(s["kernel_ms"][k] for s in stats) for k in stats[0]["kernel_ms"]}
-    fam_n = {k: sum(s["kernel_launches"][k] for s in stats) for k in stats[0]["kernel_launches"]}
-    ranked = sorted((k for k in BYTES_PER_VOXEL if fam_n[k]), key=lambda k: -fam_ms[k])
-    dom = ranked[0]
-    avg_ms = fam_ms[dom] / fam_n[dom]
-    bytes_launch = BYTES_PER_VOXEL[dom] * N
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    launches = sum(s["kernel_launches"][dom] for s in stats)
+    avg_ms = sum(s["kernel_ms"][dom] for s in stats) / max(1, launches)
+    unit, per_unit = ALG_BYTES[dom]
+    bytes_launch = per_unit * units(dom, stats[-1], N)
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": dom, "avg_launch_ms": round(avg_ms, 4),
-                "bytes_per_launch": bytes_launch}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(dom, args.workload),
+                "kernel": dom, "launches": launches, "avg_launch_ms": round(avg_ms, 4),
+                "alg_bytes_per_launch": bytes_launch, "alg_bytes_unit": f"{per_unit} B/{unit}"}
     out = {
-        "metric": "corrected voxels/s (byte-exact vs CPU oracle)",
+        "metric": METRIC,
         "value": value,
         "unit": "voxels/s",
         "n_gpus": 1,
@@ -140,15 +194,16 @@ def main():
         "config": {"workload": desc, "shape": [W, H, D], "xi": xi,
                    "outer_iters": stats[-1]["outer_iters"],
                    "c_loop_iters": stats[-1]["c_loop_iters"],
+                   "label_passes": stats[-1]["label_passes"],
                    "edited": stats[-1]["edited"],
                    "host_syncs": stats[-1]["host_syncs"],
-                   "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in fam_ms.items()},
-                   "launches_per_step": {k: v // args.steps for k, v in fam_n.items()}},
+                   "kernel_ms_warmup_call": {k: round(v, 3) for k, v in warm["kernel_ms"].items()},
+                   "launches_per_call": warm["kernel_launches"]},
         "roofline": roofline,
     }
     if not args.no_cpu:
         print("bench: cpu baseline (oracle on a bounded sample)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(threads=args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_threads)
     print(json.dumps(out), flush=True)
     ctx.close()
 

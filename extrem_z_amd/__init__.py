@@ -25,8 +25,9 @@ LIB_PATH = os.path.join(_HERE, "_build", "libextremz.so")
 STAT_KEYS = ("status", "c_loop_iters", "outer_iters", "r_loop_calls", "s_loop_calls",
              "saddle_loop_calls", "c_loops_calls", "edited", "n_max", "n_min", "n_saddle1",
              "n_saddle2", "label_passes", "jump_rounds", "host_syncs", "frontier_items")
-KERNEL_FAMILIES = ("classify", "check_full", "check_incr", "steepest", "jump", "saddle",
-                   "propose", "apply")
+KERNEL_FAMILIES = ("classify", "check_full", "check_incr", "label_tile", "label_jump",
+                   "label_chase", "saddle_pass", "slot_gather", "propose", "apply", "reserve")
+TIME_ALL = (1 << len(KERNEL_FAMILIES)) - 1
 
 ERRORS = {-3: "iteration cap reached", -5: "HIP runtime error", -12: "device allocation failed",
           -19: "no gfx950 device", -22: "invalid argument"}
@@ -44,8 +45,8 @@ class Opts(ctypes.Structure):
 
 
 class Stats(ctypes.Structure):
-    _fields_ = [("counts", ctypes.c_int64 * 16), ("kernel_ms", ctypes.c_double * 8),
-                ("kernel_launches", ctypes.c_int64 * 8), ("wall_ms", ctypes.c_double)]
+    _fields_ = [("counts", ctypes.c_int64 * 16), ("kernel_ms", ctypes.c_double * 11),
+                ("kernel_launches", ctypes.c_int64 * 11), ("wall_ms", ctypes.c_double)]
 
     def as_dict(self):
         d = {k: int(self.counts[i]) for i, k in enumerate(STAT_KEYS)}
@@ -119,10 +120,23 @@ def labels(field):
     return ds, asc
 
 
+def _timing_mask(timing):
+    """True = every kernel family, False = none, or a bit mask over KERNEL_FAMILIES
+    (or an iterable of family names)."""
+    if timing is True:
+        return TIME_ALL
+    if not timing:
+        return 0
+    if isinstance(timing, int):
+        return timing & TIME_ALL
+    return sum(1 << KERNEL_FAMILIES.index(n) for n in timing)
+
+
 def correct(orig, decp, xi, max_c_iters=0, max_outer=0, timing=False):
     """Corrects decp against orig within xi.  Returns (fixed, counts, stats).
 
     orig may be float32 or float64 (promoted exactly); inputs are not modified.
+    timing selects the kernel families timed with HIP events (see _timing_mask).
     """
     o = np.ascontiguousarray(orig)
     if o.dtype == np.float32:
@@ -135,7 +149,7 @@ def correct(orig, decp, xi, max_c_iters=0, max_outer=0, timing=False):
         raise ValueError("orig and decp shapes differ")
     counts = np.zeros(o.shape, np.uint8)
     st = Stats()
-    opts = Opts(int(max_c_iters), int(max_outer), int(bool(timing)), 0)
+    opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), 0)
     rc = lib().extremz_correct(_p(o), dt, _p(d), *_dims(o), float(xi), ctypes.byref(opts),
                                ctypes.byref(st), _p(counts))
     _check(rc)
@@ -155,7 +169,7 @@ class Context:
         """Device pointers (ints): d_orig float64, d_decp float64 edited in place."""
         W, H, D = shape
         st = Stats()
-        opts = Opts(int(max_c_iters), int(max_outer), int(bool(timing)), 0)
+        opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), 0)
         rc = lib().extremz_correct_device(self.handle, ctypes.c_void_p(d_orig),
                                           ctypes.c_void_p(d_decp), W, H, D, float(xi),
                                           ctypes.byref(opts), ctypes.byref(st),

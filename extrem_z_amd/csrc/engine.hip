@@ -160,7 +160,8 @@ struct extremz_ctx {
   int* h_ctr = nullptr;
   std::vector<ez::EventPair> ev_pool;
   size_t ev_used = 0;
-  bool timing = false;
+  int timing = 0;       // bit mask of timed kernel families
+  bool ev_open = false;  // the current tbegin recorded an event
   int64_t host_syncs = 0;
 };
 
@@ -173,7 +174,7 @@ class Run {
   Run(extremz_ctx* c, const extremz_opts* o) : c_(c) {
     max_c_ = (o && o->max_c_iters > 0) ? o->max_c_iters : 100000;
     max_outer_ = (o && o->max_outer > 0) ? o->max_outer : 10000;
-    c_->timing = o && o->timing;
+    c_->timing = o ? o->timing : 0;
     c_->ev_used = 0;
   }
 
@@ -217,7 +218,8 @@ class Run {
 
   // ---- timing
   void tbegin(int fam) {
-    if (!c_->timing) return;
+    c_->ev_open = (c_->timing >> fam) & 1;
+    if (!c_->ev_open) return;
     if (c_->ev_used == c_->ev_pool.size()) {
       EventPair p;
       ez::hip_check(hipEventCreate(&p.a), "hipEventCreate");
@@ -229,7 +231,8 @@ class Run {
     ez::hip_check(hipEventRecord(p.a, st()), "hipEventRecord");
   }
   void tend() {
-    if (!c_->timing) return;
+    if (!c_->ev_open) return;
+    c_->ev_open = false;
     ez::hip_check(hipEventRecord(c_->ev_pool[c_->ev_used].b, st()), "hipEventRecord");
     c_->ev_used++;
   }
@@ -288,23 +291,29 @@ void Run::allocate(long long N) {
 // wrong-neighbour candidates and decp's global maximum / minimum.
 void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, const uint16_t* flow,
                     int* F2, int* La2, int* F1, int* La1, int* smax, int* smin) {
-  tbegin(EXTREMZ_K_SADDLE);
   long long* kmax = c_->kmax.p;
   long long* kmin = c_->kmin.p;
+  tbegin(EXTREMZ_K_SLOT_GATHER);
   k_fill_pair<<<blocks_for(s_.nmax), 256, 0, st()>>>(kmax, (long long)0x8000000000000000ULL, smax,
                                                      kNone, s_.nmax);
   k_fill_pair<<<blocks_for(s_.nmin), 256, 0, st()>>>(kmin, 0x7fffffffffffffffLL, smin, 0x7fffffff,
                                                      s_.nmin);
   set_ctr2(C_ACT0, C_PAIRS, 0);
+  tend();
   const int nb2 = (s_.n2 + 255) / 256, nb1 = (s_.n1 + 255) / 256;
   const int ns = s_.nmax + s_.nmin;
+  auto saddle_pass = [&](int mode) {
+    if (!(nb2 + nb1)) return;
+    tbegin(EXTREMZ_K_SADDLE_PASS);
+    k_saddle_list<<<nb2 + nb1, 256, 0, st()>>>(s_, mode, f, L2, L1, flow, F2, La2, F1, La1,
+                                               c_->pairs.p, kmax, kmin, nb2);
+    tend();
+  };
   if (!dec) {
     // original field: every saddle pushes, and the (extremum, saddle)
     // adjacencies are indexed per extremum for the decp passes
     k_fill_i32<<<blocks_for(ns + 1), 256, 0, st()>>>(s_.csr_off, 0, ns + 1);
-    if (nb2 + nb1)
-      k_saddle_list<<<nb2 + nb1, 256, 0, st()>>>(s_, kModeOrig, f, L2, L1, flow, F2, La2, F1,
-                                                 La1, c_->pairs.p, kmax, kmin, nb2);
+    saddle_pass(kModeOrig);
     {  // exclusive scan of the ns counts (entry ns, zero, receives the total)
       const long long n1 = (long long)ns + 1;
       const int nblk = (int)((n1 + 1023) / 1024);
@@ -333,13 +342,11 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
     ez::hip_check(hipMemcpyAsync(s_.csr_cur, s_.csr_off, sizeof(int) * (ns + 1),
                                  hipMemcpyDeviceToDevice, st()),
                   "hipMemcpyAsync");
-    if (nb2 + nb1)
-      k_saddle_list<<<nb2 + nb1, 256, 0, st()>>>(s_, kModeFill, f, L2, L1, flow, F2, La2, F1,
-                                                 La1, c_->pairs.p, kmax, kmin, nb2);
+    saddle_pass(kModeFill);
+    tbegin(EXTREMZ_K_SLOT_GATHER);
   } else {
-    if (nb2 + nb1)
-      k_saddle_list<<<nb2 + nb1, 256, 0, st()>>>(s_, kModeDec, f, L2, L1, flow, F2, La2, F1,
-                                                 La1, c_->pairs.p, kmax, kmin, nb2);
+    saddle_pass(kModeDec);
+    tbegin(EXTREMZ_K_SLOT_GATHER);
     k_slot_pull<<<std::max(1, (ns + 31) / 32), 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
     k_slot_pull_long<<<512, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p);
     k_slot_pull_join<<<64, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p, c_->pkey.p, c_->pid.p);
@@ -371,14 +378,14 @@ void Run::apply() {
 // round trip.
 void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower) {
   counts_[EXTREMZ_STAT_LABEL_PASSES]++;
-  tbegin(EXTREMZ_K_STEEPEST);
+  tbegin(EXTREMZ_K_LABEL_TILE);
   k_labels_tile<<<tile_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as, lower);
   tend();
   int nb = (s_.g.N + 255) / 256;
-  tbegin(EXTREMZ_K_JUMP);
+  tbegin(EXTREMZ_K_LABEL_JUMP);
   k_jump<<<nb, 256, 0, st()>>>(s_.g.N, ds, as);
   tend();
-  tbegin(EXTREMZ_K_JUMP);
+  tbegin(EXTREMZ_K_LABEL_CHASE);
   k_chase<<<nb, 256, 0, st()>>>(s_.g.N, ds, as);
   tend();
   counts_[EXTREMZ_STAT_JUMP_ROUNDS] += 2;
@@ -421,6 +428,8 @@ void Run::c_loops() {
     tbegin(EXTREMZ_K_CHECK_INCR);
     k_check_list<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_, 0, back_buf_, back,
                                                     epoch_);
+    tend();
+    tbegin(EXTREMZ_K_CHECK_INCR);
     k_check_list<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, s_.dirty, C_DIRTY, 1, back_buf_,
                                                         back, epoch_);
     tend();
@@ -452,7 +461,9 @@ void Run::cp_dec(bool force) {
 // on the candidates of the last CP pass.
 void Run::wrong_neighbors() {
   // short candidate lists: every round in one block, one host read
+  tbegin(EXTREMZ_K_RESERVE);
   k_wn_solve<<<1, 1024, 0, st()>>>(s_, c_->actA.p, c_->actB.p, 32768);
+  tend();
   pull();
   if (h(C_WN_BAIL) == 2) throw HipError("wrong-neighbour reservations did not finish");
   if (!h(C_WN_BAIL)) return;
@@ -471,9 +482,11 @@ void Run::wrong_neighbors() {
     if (round > guard) throw HipError("wrong-neighbour reservations did not finish");
     set_ctr(nctr, 0);
     int b = blocks_for(na);
+    tbegin(EXTREMZ_K_RESERVE);
     k_wn_register<<<b, 256, 0, st()>>>(s_, act, actr);
     k_wn_commit<<<b, 256, 0, st()>>>(s_, act, actr);
     k_wn_next<<<b, 256, 0, st()>>>(s_, act, actr, nxt, nctr);
+    tend();
     std::swap(act, nxt);
     std::swap(actr, nctr);
   }

@@ -77,6 +77,11 @@ def quantise(orig, xi):
     over = np.abs(decp - o) > xi
     if over.any():
         decp[over] -= np.sign(decp[over] - o[over]) * q
+    # a value half-way between two grid points can miss the bound by one
+    # rounding on both sides: such voxels keep their original value
+    still = np.abs(decp - o) > xi
+    if still.any():
+        decp[still] = o[still]
     assert np.all(np.abs(decp - o) <= xi)
     return decp
 

@@ -50,23 +50,28 @@ enum {
   EXTREMZ_STAT_COUNT_ = 16
 };
 
-/* Kernel families timed with HIP events on the context stream. */
+/* Kernel families timed with HIP events on the context stream.  Each event
+ * pair brackets one kernel launch, except EXTREMZ_K_SLOT_GATHER, EXTREMZ_K_PROPOSE
+ * and EXTREMZ_K_RESERVE, which bracket the short kernel sequences named below. */
 enum {
   EXTREMZ_K_CLASSIFY = 0, /* full-grid classification of the original */
   EXTREMZ_K_CHECK_FULL,   /* full-grid false-critical-point scan */
-  EXTREMZ_K_CHECK_INCR,   /* frontier / neighbourhood re-check */
-  EXTREMZ_K_STEEPEST,     /* label tiles: steepest pointers resolved in LDS */
-  EXTREMZ_K_JUMP,         /* one full-grid pointer-jumping round, both manifolds */
-  EXTREMZ_K_SADDLE,       /* full-grid saddle pass (triplets, maps, candidates) */
+  EXTREMZ_K_CHECK_INCR,   /* frontier / edited-neighbourhood re-check */
+  EXTREMZ_K_LABEL_TILE,   /* steepest pointers + lower masks, resolved inside LDS tiles */
+  EXTREMZ_K_LABEL_JUMP,   /* one full-grid pointer-jumping round, both manifolds */
+  EXTREMZ_K_LABEL_CHASE,  /* full-grid walk of the remaining pointers to their roots */
+  EXTREMZ_K_SADDLE_PASS,  /* saddle-list pass: triplet ends, candidates, changed saddles */
+  EXTREMZ_K_SLOT_GATHER,  /* largest / smallest saddle per extremum (gather + merge + ties) */
   EXTREMZ_K_PROPOSE,      /* correction proposals (all loops) */
   EXTREMZ_K_APPLY,        /* apply_delta over touched vertices */
-  EXTREMZ_K_FAMILIES_ = 8
+  EXTREMZ_K_RESERVE,      /* wrong-neighbour reservation rounds */
+  EXTREMZ_K_FAMILIES_ = 11
 };
 
 typedef struct extremz_opts {
   int max_c_iters; /* cap on c_loops inner iterations per run; 0 = 100000 */
   int max_outer;   /* cap on outer iterations; 0 = 10000 */
-  int timing;      /* 1 = record HIP events per kernel family */
+  int timing;      /* bit f set: time family f with HIP events (0 = none) */
   int reserved;
 } extremz_opts;
 
