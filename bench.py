@@ -17,7 +17,15 @@ plain copy) when a summary for this kernel and workload exists under
 profiles/.  cpu_baseline is the CPU oracle on a bounded sample of the same
 workload recipe.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 256|512|64]
+"batch" is the mixed-size batch: 64 float32 fields, shapes log-uniform
+per axis from 32^3 to 384x384x256, Gaussian sums, er from {1e-2, 1e-3, 1e-4};
+a step is one batch over an extremz_pool, timed beside the plain path (the
+same fields one after another on one context).  "serve" is a Poisson stream
+of jobs (cubic edges log-uniform 16..512, Gaussian sums and k^-3 random
+fields, er from the same set) at offered loads below saturation; it reports
+p50 / p99 job latency and the sustained voxels/s per load.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 256|512|64|batch|serve]
     python bench.py --calibrate-copy      # plain 1 GiB copies, for the counter scale
 """
 from __future__ import annotations
@@ -99,18 +107,253 @@ def cpu_baseline(threads):
 
 
 def calibrate_copy():
-    import torch
+    from extrem_z_amd.device import DeviceBuffer, synchronize
     n = 1 << 30
-    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
+    src = DeviceBuffer.from_numpy(np.ones(n, np.uint8))
+    dst = DeviceBuffer(n)
+    synchronize()
     t = time.perf_counter()
     for _ in range(5):
-        dst.copy_(src)
-    torch.cuda.synchronize()
+        dst.copy_from(src)
+    synchronize()
     dt = (time.perf_counter() - t) / 5
     print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copies": 5,
                       "GB_per_s": 2 * n / dt / 1e9}), flush=True)
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = min(len(xs) - 1, max(0, int(np.ceil(q / 100.0 * len(xs))) - 1))
+    return xs[k]
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def upload_fields(fields, copies):
+    """Device copies of orig (float64) and `copies` working copies of decp."""
+    from extrem_z_amd.device import DeviceBuffer, synchronize
+    out = []
+    for f in fields:
+        orig, decp, xi = f[0], f[1], f[2]
+        pristine = DeviceBuffer.from_numpy(decp.ravel())
+        out.append({"shape": orig.shape[::-1], "xi": xi,
+                    "orig": DeviceBuffer.from_numpy(orig.ravel()), "pristine": pristine,
+                    "work": [pristine.empty_like() for _ in range(copies)], "N": orig.size})
+    synchronize()
+    return out
+
+
+def reset(fields, k):
+    from extrem_z_amd.device import synchronize
+    for f in fields:
+        f["work"][k].copy_from(f["pristine"])
+    synchronize()
+
+
+def fam_roofline(stats_list, fam):
+    """Achieved GB/s of kernel family fam over a set of job stats."""
+    unit, per_unit = ALG_BYTES[fam]
+    launches = sum(s["kernel_launches"][fam] for s in stats_list)
+    ms = sum(s["kernel_ms"][fam] for s in stats_list)
+    alg = 0
+    for s in stats_list:
+        u = s["n_saddle1"] + s["n_saddle2"] if unit == "saddle side" else s["_N"]
+        alg += per_unit * u * s["kernel_launches"][fam]
+    achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": fam,
+            "launches": launches, "avg_launch_ms": round(ms / max(1, launches), 4),
+            "alg_bytes_per_launch": round(alg / max(1, launches)),
+            "alg_bytes_unit": f"{per_unit} B/{unit}"}
+
+
+def job_stats(job, N):
+    import extrem_z_amd as ez
+    d = job.stats.as_dict()
+    d["_N"] = N
+    return d
+
+
+def run_batch(args):
+    """Config 4: 64 mixed fields, pool vs the plain one-context path."""
+    import extrem_z_amd as ez
+    from extrem_z_amd import synth
+    from extrem_z_amd.device import DeviceBuffer, synchronize
+
+    t = time.perf_counter()
+    fields = synth.mixed_batch(64)
+    log(f"generated 64 fields in {time.perf_counter() - t:.1f} s")
+    total = sum(f[0].size for f in fields)
+    F = upload_fields(fields, 2)
+    del fields
+    counts = [DeviceBuffer.zeros(f["N"], np.uint8) for f in F]
+    # plain path: one context, fields one after another
+    ctx = ez.Context(0)
+    big = max(range(len(F)), key=lambda i: F[i]["N"])
+    reset(F, 0)
+    ctx.correct_device(F[big]["orig"].ptr, F[big]["work"][0].ptr, F[big]["shape"],
+                       F[big]["xi"], counts[big].ptr)
+    plain_t = []
+    for step in range(args.steps):
+        reset(F, 0)
+        t0 = time.perf_counter()
+        for i, f in enumerate(F):
+            ctx.correct_device(f["orig"].ptr, f["work"][0].ptr, f["shape"], f["xi"],
+                               counts[i].ptr)
+        synchronize()
+        plain_t.append(time.perf_counter() - t0)
+        log(f"plain pass {step}: {plain_t[-1]:.3f} s")
+    ctx.close()
+    # batched: a pool of lanes
+    pool = ez.Pool(0, lanes=args.lanes)
+
+    def jobs(timing):
+        pool_opts = []
+        for i, f in enumerate(F):
+            pool_opts.append(pool.device_job(f["orig"].ptr, f["work"][1].ptr,
+                                             f["shape"], f["xi"], counts[i].ptr))
+        return pool_opts
+
+    for w in range(max(1, args.warmup)):
+        reset(F, 1)
+        rc, arr = pool.batch(jobs(False))
+        assert rc == 0, rc
+        log(f"batch warm-up {w}: done")
+    batch_t, lat, stats = [], [], []
+    for step in range(args.steps):
+        reset(F, 1)
+        t0 = time.perf_counter()
+        rc, arr = pool.batch(jobs(False))
+        synchronize()
+        batch_t.append(time.perf_counter() - t0)
+        assert rc == 0, rc
+        lat += [j.wait_ms + j.run_ms for j in arr]
+        log(f"batch pass {step}: {batch_t[-1]:.3f} s")
+    same = all(np.array_equal(f["work"][0].numpy(), f["work"][1].numpy()) for f in F)
+    info = pool.info()
+    pool.close()
+    # kernel timing of one more batch pass (every family, events per lane)
+    pool = ez.Pool(0, lanes=args.lanes, timing=True)
+    reset(F, 1)
+    rc, arr = pool.batch(jobs(True))
+    stats = [job_stats(j, F[i]["N"]) for i, j in enumerate(arr)]
+    pool.close()
+    fam_ms = {k: sum(s["kernel_ms"][k] for s in stats) for k in ALG_BYTES}
+    dom = max(fam_ms, key=fam_ms.get)
+    plain = total / min(plain_t)
+    value = total / min(batch_t)
+    return {
+        "metric": METRIC, "value": value, "unit": "voxels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": min(batch_t) * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batch64-mixed-32^3..384x384x256-er{1e-2,1e-3,1e-4}",
+                   "fields": len(F), "voxels": total, "lanes": args.lanes,
+                   "plain_voxels_per_s": plain, "speedup_vs_plain": value / plain,
+                   "plain_ms": min(plain_t) * 1e3, "identical_to_plain": same,
+                   "job_latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)},
+                   "peak_footprint_GiB": info["peak_footprint_bytes"] / 2 ** 30,
+                   "kernel_ms_timed_pass": {k: round(v, 2) for k, v in fam_ms.items()}},
+        "roofline": fam_roofline(stats, dom),
+    }
+
+
+def run_serve(args):
+    """Config 5: Poisson arrivals at offered loads below saturation."""
+    import threading
+    import extrem_z_amd as ez
+    from extrem_z_amd import synth
+    from extrem_z_amd.device import synchronize
+
+    t = time.perf_counter()
+    S = synth.serve_set(args.serve_fields)
+    log(f"generated {len(S)} jobs in {time.perf_counter() - t:.1f} s: edges "
+        f"{[s[0].shape[0] for s in S]}")
+    F = upload_fields(S, 0)
+    del S
+    M = args.serve_jobs
+    order = [i % len(F) for i in range(M)]
+    work = [F[i]["pristine"].empty_like() for i in order]
+    rng = np.random.default_rng(5)
+    pool = ez.Pool(0, lanes=args.lanes, hbm_budget=int(args.budget_gib * 2 ** 30))
+
+    def make_jobs():
+        for k, i in enumerate(order):
+            work[k].copy_from(F[i]["pristine"])
+        synchronize()
+        return [pool.device_job(F[i]["orig"].ptr, work[k].ptr, F[i]["shape"],
+                                F[i]["xi"]) for k, i in enumerate(order)]
+
+    def run(rate):
+        """Submits every job at Poisson times (rate jobs/s; inf = all at once)."""
+        jobs = make_jobs()
+        gaps = rng.exponential(1.0 / rate, size=M) if np.isfinite(rate) else np.zeros(M)
+        arrive, done = {}, {}
+        t0 = time.perf_counter()
+
+        def feeder():
+            due = t0
+            for k, j in enumerate(jobs):
+                due += gaps[k]
+                d = due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tk = pool.submit(j)
+                arrive[tk] = time.perf_counter()
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        while len(done) < M:
+            r = pool.wait_any(timeout_ms=100000)
+            assert r is not None, "no job finished within 100 s"
+            tk, j = r
+            assert j.status == 0, ez.job_error(j)
+            done[tk] = time.perf_counter()
+        th.join()
+        lat = [(done[k] - arrive[k]) * 1e3 for k in done]
+        span = max(done.values()) - t0
+        vox = sum(F[i]["N"] for i in order)
+        return {"offered_jobs_per_s": rate if np.isfinite(rate) else None,
+                "voxels_per_s": vox / span, "jobs_per_s": M / span,
+                "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99), "span_s": span}
+
+    run(float("inf"))  # warm-up: lanes grow their buffers
+    sat = run(float("inf"))
+    log(f"saturation: {sat['jobs_per_s']:.2f} jobs/s, {sat['voxels_per_s']:.3e} voxels/s")
+    loads = []
+    for frac in args.loads:
+        r = run(frac * sat["jobs_per_s"])
+        r["load_fraction"] = frac
+        loads.append(r)
+        log(f"load {frac:.2f}: p50 {r['p50_ms']:.1f} ms, p99 {r['p99_ms']:.1f} ms, "
+            f"{r['voxels_per_s']:.3e} voxels/s")
+    info = pool.info()
+    pool.close()
+    # plain path over the same stream, one job after another
+    ctx = ez.Context(0)
+    jobs = make_jobs()
+    t0 = time.perf_counter()
+    for k, i in enumerate(order):
+        ctx.correct_device(F[i]["orig"].ptr, work[k].ptr, F[i]["shape"], F[i]["xi"])
+    synchronize()
+    plain = sum(F[i]["N"] for i in order) / (time.perf_counter() - t0)
+    ctx.close()
+    top = max(loads, key=lambda r: r["load_fraction"]) if loads else sat
+    return {
+        "metric": METRIC, "value": top["voxels_per_s"], "unit": "voxels/s", "n_gpus": 1,
+        "steps": M, "warmup": 1, "ms_per_step": top["span_s"] * 1e3 / M, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"serve-poisson-{M}jobs-edges16..512-gauss+grf-er{{1e-2,1e-3,1e-4}}",
+                   "distinct_fields": len(F), "lanes": args.lanes,
+                   "budget_GiB": args.budget_gib, "saturation": sat, "loads": loads,
+                   "latency_ms": {"p50": top["p50_ms"], "p99": top["p99_ms"],
+                                  "load_fraction": top["load_fraction"] if loads else None},
+                   "plain_voxels_per_s": plain,
+                   "peak_footprint_GiB": info["peak_footprint_bytes"] / 2 ** 30},
+    }
 
 
 def main():
@@ -122,6 +365,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--budget-gib", type=float, default=200.0)
+    ap.add_argument("--serve-fields", type=int, default=16)
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--loads", type=float, nargs="*", default=[0.3, 0.6, 0.9])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: fields and jobs shard whole per GPU and this build "
@@ -131,37 +379,41 @@ def main():
         calibrate_copy()
         return
 
-    import torch
+    if args.workload in ("batch", "serve"):
+        out = run_batch(args) if args.workload == "batch" else run_serve(args)
+        print(json.dumps(out), flush=True)
+        return
+
     import extrem_z_amd as ez
+    from extrem_z_amd.device import DeviceBuffer, synchronize
 
     orig, decp, xi, desc = workload(args.workload)
     D, H, W = orig.shape
     N = orig.size
     print(f"bench: workload {desc}, N={N}, steps={args.steps}, warmup={args.warmup}",
           file=sys.stderr, flush=True)
-    dev = torch.device("cuda:0")
-    d_orig = torch.from_numpy(orig.ravel()).to(dev)
-    copies = [torch.from_numpy(decp.ravel()).to(dev) for _ in range(args.steps + max(1, args.warmup))]
-    d_counts = torch.zeros(N, dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
+    d_orig = DeviceBuffer.from_numpy(orig.ravel())
+    copies = [DeviceBuffer.from_numpy(decp.ravel()) for _ in range(args.steps + max(1, args.warmup))]
+    d_counts = DeviceBuffer.zeros(N, np.uint8)
+    synchronize()
     ctx = ez.Context(0)
     warm = None
     for i in range(max(1, args.warmup)):
-        warm = ctx.correct_device(d_orig.data_ptr(), copies[i].data_ptr(), (W, H, D), xi,
-                                  d_counts.data_ptr(), timing=True)
+        warm = ctx.correct_device(d_orig.ptr, copies[i].ptr, (W, H, D), xi,
+                                  d_counts.ptr, timing=True)
         print(f"bench: warmup {i} done, {warm['wall_ms']:.1f} ms", file=sys.stderr, flush=True)
     # dominant single kernel of the warm-up call
     dom = max(ALG_BYTES, key=lambda k: warm["kernel_ms"][k])
     mask = 1 << ez.KERNEL_FAMILIES.index(dom)
-    torch.cuda.synchronize()
+    synchronize()
     stats = []
     base = max(1, args.warmup)
     t0 = time.perf_counter()
     for i in range(args.steps):
-        st = ctx.correct_device(d_orig.data_ptr(), copies[base + i].data_ptr(), (W, H, D), xi,
-                                d_counts.data_ptr(), timing=mask)
+        st = ctx.correct_device(d_orig.ptr, copies[base + i].ptr, (W, H, D), xi,
+                                d_counts.ptr, timing=mask)
         stats.append(st)
-    torch.cuda.synchronize()
+    synchronize()
     elapsed = time.perf_counter() - t0
     for st in stats:
         assert st["status"] == 0, st
@@ -197,6 +449,8 @@ def main():
                    "label_passes": stats[-1]["label_passes"],
                    "edited": stats[-1]["edited"],
                    "host_syncs": stats[-1]["host_syncs"],
+                   "call_latency_ms": {"p50": pct([s["wall_ms"] for s in stats], 50),
+                                       "p99": pct([s["wall_ms"] for s in stats], 99)},
                    "kernel_ms_warmup_call": {k: round(v, 3) for k, v in warm["kernel_ms"].items()},
                    "launches_per_call": warm["kernel_launches"]},
         "roofline": roofline,

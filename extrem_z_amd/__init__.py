@@ -9,6 +9,9 @@ Functions mirror the reference's correction path:
   classify()       classifyAllVertices (src/ExtreM.cpp:1088-1097)
   labels()         ComputeDescending/AscendingManifold (src/ExtreM.cpp:144-183, 502-541)
   encode()         cost() edit streams (src/ExtreM.cpp:272-389)
+  Pool             batched and served corrections over several HIP streams
+                   (no reference counterpart: the reference corrects one
+                   field per process)
   getdata()        the raw float64 reader + bound of getdata (src/ExtreM.cpp:95-138)
   saveArrayToBin() the raw float64 writer (src/ExtreM.cpp:2918-2928)
 """
@@ -29,8 +32,9 @@ KERNEL_FAMILIES = ("classify", "check_full", "check_incr", "label_tile", "label_
                    "label_chase", "saddle_pass", "slot_gather", "propose", "apply", "reserve")
 TIME_ALL = (1 << len(KERNEL_FAMILIES)) - 1
 
-ERRORS = {-3: "iteration cap reached", -5: "HIP runtime error", -12: "device allocation failed",
-          -19: "no gfx950 device", -22: "invalid argument"}
+ERRORS = {-3: "iteration cap reached", -5: "HIP runtime error", -11: "not finished",
+          -12: "device allocation failed", -19: "no gfx950 device", -22: "invalid argument"}
+EAGAIN = -11
 
 
 class ExtremzError(RuntimeError):
@@ -57,6 +61,22 @@ class Stats(ctypes.Structure):
         return d
 
 
+class Job(ctypes.Structure):
+    """extremz_job: one correction handed to a Pool."""
+    _fields_ = [("orig", ctypes.c_void_p), ("decp", ctypes.c_void_p), ("counts", ctypes.c_void_p),
+                ("W", ctypes.c_int), ("H", ctypes.c_int), ("D", ctypes.c_int),
+                ("orig_dtype", ctypes.c_int), ("on_host", ctypes.c_int), ("status", ctypes.c_int),
+                ("xi", ctypes.c_double), ("lane", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("wait_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("stats", Stats)]
+
+
+class PoolInfo(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_int64), ("footprint_bytes", ctypes.c_int64),
+                ("peak_footprint_bytes", ctypes.c_int64), ("submitted", ctypes.c_int64),
+                ("completed", ctypes.c_int64), ("running", ctypes.c_int64),
+                ("queued", ctypes.c_int64), ("lanes", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -77,6 +97,13 @@ def lib():
         L.extremz_lut.argtypes = [vp, i32]
         L.extremz_encode.argtypes = [vp, vp, vp, vp, i64, dbl, vp, vp, vp, vp, vp]
         L.extremz_last_error.restype = ctypes.c_char_p
+        L.extremz_pool_create.argtypes = [i32, i32, i64, vp, ctypes.POINTER(vp)]
+        L.extremz_pool_destroy.argtypes = [vp]
+        L.extremz_pool_info_get.argtypes = [vp, vp]
+        L.extremz_batch_correct.argtypes = [vp, vp, i32]
+        L.extremz_serve_submit.argtypes = [vp, vp, ctypes.POINTER(i64)]
+        L.extremz_serve_wait.argtypes = [vp, i64, i32]
+        L.extremz_serve_wait_any.argtypes = [vp, i32, ctypes.POINTER(i64)]
         _lib = L
     return _lib
 
@@ -187,6 +214,98 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Pool:
+    """Lanes (HIP streams + work buffers) on one device with HBM-budget admission
+    (extremz_pool).  batch() corrects many fields at once; submit() / wait()
+    serve jobs as they arrive.  Every job's result equals the single-field
+    correct() result byte for byte."""
+
+    def __init__(self, device=0, lanes=8, hbm_budget=0, max_c_iters=0, max_outer=0, timing=False):
+        h = ctypes.c_void_p()
+        opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), 0)
+        _check(lib().extremz_pool_create(int(device), int(lanes), int(hbm_budget),
+                                         ctypes.byref(opts), ctypes.byref(h)))
+        self.handle = h
+        self._held = {}  # ticket -> (Job, arrays kept alive while the job runs)
+
+    @staticmethod
+    def host_job(orig, decp, xi):
+        """A Job over host arrays: decp (float64) is corrected in place, counts
+        is filled.  Returns (job, keep) where keep holds the arrays."""
+        o = np.ascontiguousarray(orig)
+        if o.dtype != np.float32:
+            o = np.ascontiguousarray(o, dtype=np.float64)
+        if decp.dtype != np.float64 or not decp.flags.c_contiguous:
+            raise ValueError("decp must be a C-contiguous float64 array (corrected in place)")
+        if decp.shape != o.shape:
+            raise ValueError("orig and decp shapes differ")
+        counts = np.zeros(o.shape, np.uint8)
+        W, H, D = _dims(o)
+        j = Job(orig=o.ctypes.data, decp=decp.ctypes.data, counts=counts.ctypes.data, W=W, H=H,
+                D=D, orig_dtype=0 if o.dtype == np.float32 else 1, on_host=1, xi=float(xi))
+        return j, (o, decp, counts)
+
+    @staticmethod
+    def device_job(d_orig, d_decp, shape, xi, d_counts=0, orig_dtype=1):
+        """A Job over device pointers (ints); orig_dtype 0 = float32, 1 = float64."""
+        W, H, D = shape
+        return Job(orig=d_orig, decp=d_decp, counts=d_counts or None, W=W, H=H, D=D,
+                   orig_dtype=int(orig_dtype), on_host=0, xi=float(xi))
+
+    def batch(self, jobs):
+        """Runs a ctypes array / list of Job; returns the first failing status (0 if none)."""
+        arr = (Job * len(jobs))(*jobs) if not isinstance(jobs, ctypes.Array) else jobs
+        rc = lib().extremz_batch_correct(self.handle, arr, len(arr))
+        return rc, arr
+
+    def submit(self, job, keep=None):
+        t = ctypes.c_int64(0)
+        _check(lib().extremz_serve_submit(self.handle, ctypes.byref(job), ctypes.byref(t)))
+        self._held[t.value] = (job, keep)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        """Returns the Job once finished (status in job.status), or None on time-out."""
+        rc = lib().extremz_serve_wait(self.handle, int(ticket), int(timeout_ms))
+        if rc == EAGAIN:
+            return None
+        job, _ = self._held.pop(ticket)
+        return job
+
+    def wait_any(self, timeout_ms=-1):
+        """Returns (ticket, Job) of the next finished job, or None on time-out."""
+        t = ctypes.c_int64(0)
+        rc = lib().extremz_serve_wait_any(self.handle, int(timeout_ms), ctypes.byref(t))
+        if rc == EAGAIN:
+            return None
+        job, _ = self._held.pop(t.value)
+        return t.value, job
+
+    def info(self):
+        i = PoolInfo()
+        _check(lib().extremz_pool_info_get(self.handle, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in PoolInfo._fields_ if k != "reserved"}
+
+    def close(self):
+        if self.handle:
+            lib().extremz_pool_destroy(self.handle)
+            self.handle = None
+            self._held.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def job_error(job):
+    """ExtremzError for a finished Job with a non-zero status, else None."""
+    if job.status == 0:
+        return None
+    return ExtremzError(job.status, lib().extremz_last_error().decode(errors="replace"))
 
 
 def encode(orig, decp_before, decp_after, counts, xi):

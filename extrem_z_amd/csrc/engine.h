@@ -4,8 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <new>
 #include <stdexcept>
 #include <string>
+
+#include "../../include/extremz.h"
 
 namespace ez {
 
@@ -102,6 +105,42 @@ struct HipError : std::runtime_error {
 
 inline void hip_check(hipError_t e, const char* what = "hip") {
   if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Host engine (engine.hip), shared with the lane pool (pool.hip).
+extern thread_local std::string g_err;
+extremz_ctx* new_ctx(int device);            // nullptr: no gfx950 device with that ordinal
+void free_ctx(extremz_ctx* c);
+size_t ctx_bytes(extremz_ctx* c);            // device bytes held by the work buffers
+void ctx_release(extremz_ctx* c);            // frees the work buffers (kept: stream, LUT)
+bool valid_shape(int W, int H, int D);
+int run_correct(extremz_ctx* c, const double* d_orig, double* d_decp, int W, int H, int D,
+                double xi, const extremz_opts* opts, extremz_stats* stats, uint8_t* d_counts);
+// One correction with orig float32 / float64 and host or device buffers:
+// host buffers are staged through the context's own device buffers.
+int correct_staged(extremz_ctx* c, const void* orig, int orig_dtype, bool on_host, double* decp,
+                   uint8_t* counts, int W, int H, int D, double xi, const extremz_opts* opts,
+                   extremz_stats* stats);
+
+// Runs fn with C++ errors translated into EXTREMZ_E* codes (message in g_err).
+template <class F>
+int guarded(F&& fn) {
+  g_err.clear();
+  try {
+    return fn();
+  } catch (const std::bad_alloc&) {
+    g_err = "device allocation failed";
+    return EXTREMZ_ENOMEM;
+  } catch (const std::range_error& e) {
+    g_err = std::string("iteration cap reached in ") + e.what();
+    return EXTREMZ_ENOTCONV;
+  } catch (const HipError& e) {
+    g_err = e.what();
+    return EXTREMZ_EHIP;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return EXTREMZ_EHIP;
+  }
 }
 
 void upload_constants(const int* dx, const int* dy, const int* dz, const int* cls_key,

@@ -129,6 +129,7 @@ struct DevBuf {
     p = nullptr;
     cap = 0;
   }
+  size_t bytes() const { return cap * sizeof(T); }
 };
 
 struct EventPair {
@@ -163,11 +164,28 @@ struct extremz_ctx {
   int timing = 0;       // bit mask of timed kernel families
   bool ev_open = false;  // the current tbegin recorded an event
   int64_t host_syncs = 0;
+
+  // Calls f on every work buffer (not the LUT).
+  template <class F>
+  void each_buf(F&& f) {
+    for (ez::DevBuf<int>* b :
+         {&oDS, &oAS, &dDS, &dAS, &stamp, &touched, &dirty, &frontA, &frontB, &S2, &S1, &MX, &MN,
+          &olarg, &osmall, &dlarg, &dsmall, &minsad2, &minsad1, &win_a, &win_b, &lcnt, &csr,
+          &csr_off, &csr_cur, &pid, &cpos, &ext_id, &of2, &ol2, &of1, &ol1, &df2, &dl2, &df1,
+          &dl1, &wnl2, &wnl1, &bmax2, &bmin2, &ctr})
+      f(*b);
+    for (ez::DevBuf<long long>* b : {&delta, &kmax, &kmin, &pkey, &ckey, &pv2, &pv1}) f(*b);
+    for (ez::DevBuf<uint8_t>* b : {&cnt, &otype, &dflag, &wn2, &wn1, &counts_tmp}) f(*b);
+    for (ez::DevBuf<uint16_t>* b : {&olow, &dlow}) f(*b);
+    for (ez::DevBuf<int2>* b : {&longs, &act0, &actA, &actB, &pairs}) f(*b);
+    f(chunks);
+    for (ez::DevBuf<double>* b : {&orig_tmp, &decp_tmp}) f(*b);
+  }
 };
 
 namespace ez {
 
-static thread_local std::string g_err;
+thread_local std::string g_err;
 
 class Run {
  public:
@@ -739,7 +757,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
 static std::mutex g_cache_mu;
 static std::vector<extremz_ctx*> g_cache;
 
-static extremz_ctx* new_ctx(int device) {
+extremz_ctx* new_ctx(int device) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
     (void)hipGetLastError();
@@ -757,11 +775,15 @@ static extremz_ctx* new_ctx(int device) {
   const LutTable& L = lut_table();
   c->lut.ensure(L.table.size());
   ez::hip_check(hipMemcpy(c->lut.p, L.table.data(), L.table.size(), hipMemcpyHostToDevice), "lut");
-  upload_constants(kDx, kDy, kDz, L.cls_key, L.cls_off);
+  {
+    static std::mutex mu;  // contexts may be created from several threads
+    std::lock_guard<std::mutex> lk(mu);
+    upload_constants(kDx, kDy, kDz, L.cls_key, L.cls_off);
+  }
   return c;
 }
 
-static void free_ctx(extremz_ctx* c) {
+void free_ctx(extremz_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
@@ -769,34 +791,22 @@ static void free_ctx(extremz_ctx* c) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
   }
-  for (ez::DevBuf<int>* b :
-       {&c->oDS, &c->oAS, &c->dDS, &c->dAS, &c->stamp, &c->touched, &c->dirty, &c->frontA,
-        &c->frontB, &c->S2, &c->S1, &c->MX, &c->MN, &c->olarg, &c->osmall, &c->dlarg,
-        &c->dsmall, &c->minsad2, &c->minsad1, &c->win_a, &c->win_b, &c->of2, &c->ol2,
-        &c->of1,
-        &c->ol1, &c->df2, &c->dl2, &c->df1, &c->dl1, &c->wnl2, &c->wnl1,
-        &c->bmax2, &c->bmin2, &c->ctr})
-    b->release();
-  for (ez::DevBuf<uint8_t>* b : {&c->lut, &c->cnt, &c->otype, &c->dflag, &c->wn2, &c->wn1,
-                                 &c->counts_tmp})
-    b->release();
-  for (ez::DevBuf<int2>* b : {&c->act0, &c->actA, &c->actB, &c->pairs}) b->release();
-  c->delta.release();
-  c->kmax.release();
-  c->kmin.release();
-  c->ext_id.release();
-  c->lcnt.release();
-  for (DevBuf<int>* b : {&c->csr, &c->csr_off, &c->csr_cur, &c->pid, &c->cpos}) b->release();
-  for (DevBuf<long long>* b : {&c->pkey, &c->ckey, &c->pv2, &c->pv1}) b->release();
-  c->longs.release();
-  c->chunks.release();
-  c->olow.release();
-  c->dlow.release();
-  c->orig_tmp.release();
-  c->decp_tmp.release();
+  c->each_buf([](auto& b) { b.release(); });
+  c->lut.release();
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+size_t ctx_bytes(extremz_ctx* c) {
+  size_t n = 0;
+  c->each_buf([&](auto& b) { n += b.bytes(); });
+  return n;
+}
+
+void ctx_release(extremz_ctx* c) {
+  (void)hipStreamSynchronize(c->stream);
+  c->each_buf([](auto& b) { b.release(); });
 }
 
 static void fill_stats(extremz_ctx* c, const Run& r, int status, double wall_ms,
@@ -819,31 +829,10 @@ static void fill_stats(extremz_ctx* c, const Run& r, int status, double wall_ms,
   }
 }
 
-static bool valid_shape(int W, int H, int D) {
+bool valid_shape(int W, int H, int D) {
   if (W <= 0 || H <= 0 || D <= 0) return false;
   long long N = (long long)W * H * D;
   return N <= (1LL << 30);  // entries 2*v + side must fit an int
-}
-
-// Runs fn with error translation; fn returns a status.
-template <class F>
-static int guarded(F&& fn) {
-  g_err.clear();
-  try {
-    return fn();
-  } catch (const std::bad_alloc&) {
-    g_err = "device allocation failed";
-    return EXTREMZ_ENOMEM;
-  } catch (const std::range_error& e) {
-    g_err = std::string("iteration cap reached in ") + e.what();
-    return EXTREMZ_ENOTCONV;
-  } catch (const HipError& e) {
-    g_err = e.what();
-    return EXTREMZ_EHIP;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return EXTREMZ_EHIP;
-  }
 }
 
 static extremz_ctx* cached_ctx(int device) {
@@ -854,9 +843,8 @@ static extremz_ctx* cached_ctx(int device) {
   return c;
 }
 
-static int run_correct(extremz_ctx* c, const double* d_orig, double* d_decp, int W, int H, int D,
-                       double xi, const extremz_opts* opts, extremz_stats* stats,
-                       uint8_t* d_counts) {
+int run_correct(extremz_ctx* c, const double* d_orig, double* d_decp, int W, int H, int D,
+                double xi, const extremz_opts* opts, extremz_stats* stats, uint8_t* d_counts) {
   auto t0 = std::chrono::steady_clock::now();
   ez::hip_check(hipSetDevice(c->device), "hipSetDevice");
   c->host_syncs = 0;
@@ -865,6 +853,47 @@ static int run_correct(extremz_ctx* c, const double* d_orig, double* d_decp, int
   double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   fill_stats(c, r, status, ms, stats);
   return status;
+}
+
+
+int correct_staged(extremz_ctx* c, const void* orig, int orig_dtype, bool on_host, double* decp,
+                   uint8_t* counts, int W, int H, int D, double xi, const extremz_opts* opts,
+                   extremz_stats* stats) {
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  const size_t N = (size_t)W * H * D;
+  hipStream_t q = c->stream;
+  const double* d_orig = (const double*)orig;
+  double* d_decp = decp;
+  uint8_t* d_counts = counts;
+  if (on_host) {
+    c->decp_tmp.ensure(N);
+    c->counts_tmp.ensure(N);
+    d_decp = c->decp_tmp.p;
+    d_counts = c->counts_tmp.p;
+  }
+  if (on_host || orig_dtype == EXTREMZ_DTYPE_F32) {
+    c->orig_tmp.ensure(N);
+    if (!on_host) {
+      k_f32_to_f64<<<2048, 256, 0, q>>>((const float*)orig, c->orig_tmp.p, (long long)N);
+    } else if (orig_dtype == EXTREMZ_DTYPE_F64) {
+      hip_check(hipMemcpyAsync(c->orig_tmp.p, orig, N * 8, hipMemcpyHostToDevice, q), "h2d");
+    } else {
+      // stage float32 in the decp buffer, then promote exactly
+      hip_check(hipMemcpyAsync(c->decp_tmp.p, orig, N * 4, hipMemcpyHostToDevice, q), "h2d");
+      k_f32_to_f64<<<2048, 256, 0, q>>>((const float*)c->decp_tmp.p, c->orig_tmp.p,
+                                        (long long)N);
+    }
+    d_orig = c->orig_tmp.p;
+  }
+  if (on_host)
+    hip_check(hipMemcpyAsync(d_decp, decp, N * 8, hipMemcpyHostToDevice, q), "h2d");
+  int rc = run_correct(c, d_orig, d_decp, W, H, D, xi, opts, stats, d_counts);
+  if (on_host) {
+    hip_check(hipMemcpyAsync(decp, d_decp, N * 8, hipMemcpyDeviceToHost, q), "d2h");
+    if (counts) hip_check(hipMemcpyAsync(counts, d_counts, N, hipMemcpyDeviceToHost, q), "d2h");
+  }
+  hip_check(hipStreamSynchronize(q), "sync");
+  return rc;
 }
 
 }  // namespace ez
@@ -920,27 +949,7 @@ int extremz_correct(const void* orig, int orig_dtype, double* decp, int W, int H
       ez::g_err = "no gfx950 device";
       return EXTREMZ_ENODEV;
     }
-    ez::hip_check(hipSetDevice(c->device), "hipSetDevice");
-    const size_t N = (size_t)W * H * D;
-    c->orig_tmp.ensure(N);
-    c->decp_tmp.ensure(N);
-    c->counts_tmp.ensure(N);
-    if (orig_dtype == EXTREMZ_DTYPE_F64) {
-      ez::hip_check(hipMemcpyAsync(c->orig_tmp.p, orig, N * 8, hipMemcpyHostToDevice, c->stream), "h2d");
-    } else {
-      // stage float32 in the decp buffer, then promote exactly
-      ez::hip_check(hipMemcpyAsync(c->decp_tmp.p, orig, N * 4, hipMemcpyHostToDevice, c->stream), "h2d");
-      ez::k_f32_to_f64<<<2048, 256, 0, c->stream>>>((const float*)c->decp_tmp.p, c->orig_tmp.p,
-                                                    (long long)N);
-    }
-    ez::hip_check(hipMemcpyAsync(c->decp_tmp.p, decp, N * 8, hipMemcpyHostToDevice, c->stream), "h2d");
-    int rc = ez::run_correct(c, c->orig_tmp.p, c->decp_tmp.p, W, H, D, xi, opts, stats,
-                             c->counts_tmp.p);
-    ez::hip_check(hipMemcpyAsync(decp, c->decp_tmp.p, N * 8, hipMemcpyDeviceToHost, c->stream), "d2h");
-    if (counts)
-      ez::hip_check(hipMemcpyAsync(counts, c->counts_tmp.p, N, hipMemcpyDeviceToHost, c->stream), "d2h");
-    ez::hip_check(hipStreamSynchronize(c->stream), "sync");
-    return rc;
+    return ez::correct_staged(c, orig, orig_dtype, true, decp, counts, W, H, D, xi, opts, stats);
   });
 }
 

@@ -22,6 +22,7 @@ extern "C" {
 #define EXTREMZ_OK 0
 #define EXTREMZ_ENOTCONV (-3)  /* an iteration cap was reached */
 #define EXTREMZ_EHIP (-5)      /* a HIP runtime call failed */
+#define EXTREMZ_EAGAIN (-11)   /* a served job is not finished yet */
 #define EXTREMZ_ENOMEM (-12)   /* device allocation failed */
 #define EXTREMZ_ENODEV (-19)   /* no usable gfx950 device */
 #define EXTREMZ_EINVAL (-22)   /* bad shape, bound, dtype or non-finite input */
@@ -140,6 +141,72 @@ int extremz_encode(const double* orig, const double* decp_before,
                    const double* decp_after, const uint8_t* counts, int64_t n,
                    double xi, uint8_t* index_bitmap, double* edits,
                    uint64_t* mantissas, int32_t* exponents, int64_t* n_edits);
+
+/* ---------------------------------------------------------------------------
+ * Batching and serving.  No reference counterpart: the reference corrects one
+ * field per process (its state is global, include/api/ExtreM.h:17-41) and its
+ * GPU binary one field per run (src/ExtreM_cuda.cu:4954-6078).
+ *
+ * A pool owns `lanes` contexts (one HIP stream and one set of work buffers
+ * each) on one device and a host worker per lane.  Jobs are admitted as a
+ * lane and device memory free up: a job is admitted when the pool's device
+ * footprint with the job's working set (an estimate, reconciled with the
+ * buffers actually held after each job) stays within hbm_budget bytes.  Jobs
+ * that do not fit are passed over by later jobs that do, unless they have
+ * waited longer than `lanes` admissions; a job larger than the whole budget is
+ * rejected with EXTREMZ_ENOMEM.  Each job's result is the single-field
+ * result, byte for byte, whatever runs beside it.
+ * ------------------------------------------------------------------------- */
+typedef struct extremz_job {
+  const void* orig;   /* float32 or float64 (orig_dtype) */
+  double* decp;       /* corrected in place */
+  uint8_t* counts;    /* nullable: receives delta_counter as uint8 */
+  int W, H, D;
+  int orig_dtype;     /* EXTREMZ_DTYPE_F32 / _F64 */
+  int on_host;        /* 1: host buffers (staged through the lane), 0: device buffers */
+  int status;         /* out: 0 or a negative EXTREMZ_E* code */
+  double xi;          /* absolute bound */
+  int lane;           /* out: the lane that ran the job */
+  int reserved;
+  double wait_ms;     /* out: submission -> admission */
+  double run_ms;      /* out: admission -> completion */
+  extremz_stats stats; /* out */
+} extremz_job;
+
+typedef struct extremz_pool extremz_pool;
+
+typedef struct extremz_pool_info {
+  int64_t budget_bytes;
+  int64_t footprint_bytes;       /* device bytes charged to the lanes now */
+  int64_t peak_footprint_bytes;  /* largest footprint so far */
+  int64_t submitted, completed, running, queued;
+  int lanes;
+  int reserved;
+} extremz_pool_info;
+
+/* lanes in [1, 64]; hbm_budget 0 = 80% of the device's memory; opts
+ * (nullable) apply to every job. */
+int extremz_pool_create(int device, int lanes, int64_t hbm_budget, const extremz_opts* opts,
+                        extremz_pool** out);
+/* Waits for every submitted job, then frees the pool. */
+int extremz_pool_destroy(extremz_pool* pool);
+int extremz_pool_info_get(extremz_pool* pool, extremz_pool_info* info);
+
+/* Batch: corrects jobs[0..n) over the lanes (largest first) and returns when
+ * all are done: 0, or the first failing job's status (every job's own status
+ * is in jobs[i].status). */
+int extremz_batch_correct(extremz_pool* pool, extremz_job* jobs, int n);
+
+/* Serving: queues one job and returns at once with its ticket.  The job and
+ * its buffers must stay valid until the job is waited for. */
+int extremz_serve_submit(extremz_pool* pool, extremz_job* job, int64_t* ticket);
+/* Waits up to timeout_ms (< 0: no limit) for the job of ticket; returns its
+ * status, or EXTREMZ_EAGAIN if it is still queued or running.  A ticket can
+ * be waited for until it has returned a status once. */
+int extremz_serve_wait(extremz_pool* pool, int64_t ticket, int timeout_ms);
+/* Waits up to timeout_ms for any finished job that has not been waited for;
+ * stores its ticket.  Returns its status or EXTREMZ_EAGAIN. */
+int extremz_serve_wait_any(extremz_pool* pool, int timeout_ms, int64_t* ticket);
 
 /* Last error message of the calling thread ("" if none). */
 const char* extremz_last_error(void);
