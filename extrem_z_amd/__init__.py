@@ -9,6 +9,7 @@ Functions mirror the reference's correction path:
   classify()       classifyAllVertices (src/ExtreM.cpp:1088-1097)
   labels()         ComputeDescending/AscendingManifold (src/ExtreM.cpp:144-183, 502-541)
   encode()         cost() edit streams (src/ExtreM.cpp:272-389)
+  cost()           cost()'s zstd container and statistics (src/ExtreM.cpp:252-389)
   Pool             batched and served corrections over several HIP streams
                    (no reference counterpart: the reference corrects one
                    field per process)
@@ -33,7 +34,8 @@ KERNEL_FAMILIES = ("classify", "check_full", "check_incr", "label_tile", "label_
 TIME_ALL = (1 << len(KERNEL_FAMILIES)) - 1
 
 ERRORS = {-3: "iteration cap reached", -5: "HIP runtime error", -11: "not finished",
-          -12: "device allocation failed", -19: "no gfx950 device", -22: "invalid argument"}
+          -12: "device allocation failed", -19: "no gfx950 device", -22: "invalid argument",
+          -38: "libzstd.so.1 not available"}
 EAGAIN = -11
 
 
@@ -70,6 +72,15 @@ class Job(ctypes.Structure):
                 ("wait_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("stats", Stats)]
 
 
+class CostStats(ctypes.Structure):
+    _fields_ = [("changed", ctypes.c_int64), ("snapped", ctypes.c_int64),
+                ("edit_ratio", ctypes.c_double), ("psnr", ctypes.c_double),
+                ("fixed_psnr", ctypes.c_double), ("index_bytes", ctypes.c_int64),
+                ("index_zst_bytes", ctypes.c_int64), ("edits_bytes", ctypes.c_int64),
+                ("edits_zst_bytes", ctypes.c_int64), ("ocr", ctypes.c_double),
+                ("cr", ctypes.c_double), ("obr", ctypes.c_double), ("br", ctypes.c_double)]
+
+
 class PoolInfo(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_int64), ("footprint_bytes", ctypes.c_int64),
                 ("peak_footprint_bytes", ctypes.c_int64), ("submitted", ctypes.c_int64),
@@ -97,6 +108,13 @@ def lib():
         L.extremz_lut.argtypes = [vp, i32]
         L.extremz_encode.argtypes = [vp, vp, vp, vp, i64, dbl, vp, vp, vp, vp, vp]
         L.extremz_last_error.restype = ctypes.c_char_p
+        L.extremz_cost.argtypes = [vp, vp, vp, vp, i64, dbl, i64, i64, i32, vp, i64, vp, i64, vp]
+        L.extremz_zstd_bound.argtypes = [i64]
+        L.extremz_zstd_bound.restype = i64
+        L.extremz_zstd_compress.argtypes = [vp, i64, vp, i64, i32]
+        L.extremz_zstd_compress.restype = i64
+        L.extremz_zstd_decompress.argtypes = [vp, i64, vp, i64]
+        L.extremz_zstd_decompress.restype = i64
         L.extremz_pool_create.argtypes = [i32, i32, i64, vp, ctypes.POINTER(vp)]
         L.extremz_pool_destroy.argtypes = [vp]
         L.extremz_pool_info_get.argtypes = [vp, vp]
@@ -324,6 +342,55 @@ def encode(orig, decp_before, decp_after, counts, xi):
                                 _p(mant), _p(expo), ctypes.byref(ne)))
     k = ne.value
     return bm, edits[:k].copy(), mant[:k].copy(), expo[:k].copy()
+
+
+def zstd_compress(data, level=3):
+    """One zstd frame of the bytes of data (host only)."""
+    src = np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray) else \
+        np.ascontiguousarray(data).view(np.uint8).ravel()
+    cap = lib().extremz_zstd_bound(src.size)
+    if cap < 0:
+        raise ExtremzError(int(cap), "zstd")
+    dst = np.empty(max(1, cap), np.uint8)
+    n = lib().extremz_zstd_compress(_p(src), src.size, _p(dst), dst.size, int(level))
+    if n < 0:
+        raise ExtremzError(int(n), "zstd compress")
+    return dst[:n].tobytes()
+
+
+def zstd_decompress(frame):
+    """Content of one zstd frame (with its content size, as zstd -f writes)."""
+    src = np.frombuffer(bytes(frame), np.uint8)
+    need = lib().extremz_zstd_decompress(_p(src), src.size, None, 0)
+    if need < 0:
+        raise ExtremzError(int(need), "zstd frame")
+    dst = np.empty(max(1, need), np.uint8)
+    n = lib().extremz_zstd_decompress(_p(src), src.size, _p(dst), dst.size)
+    if n < 0:
+        raise ExtremzError(int(n), "zstd decompress")
+    return dst[:n].tobytes()
+
+
+def cost(orig, decp_before, decp_after, counts, xi, original_bytes=0, compressed_bytes=0,
+         level=3):
+    """cost(): (stats dict, index bitmap zstd frame, snapped-edits zstd frame)."""
+    o = np.ascontiguousarray(orig, dtype=np.float64).ravel()
+    b = np.ascontiguousarray(decp_before, dtype=np.float64).ravel()
+    a = np.ascontiguousarray(decp_after, dtype=np.float64).ravel()
+    c = np.ascontiguousarray(counts, dtype=np.uint8).ravel()
+    n = o.size
+    icap = lib().extremz_zstd_bound((3 * n + 7) // 8)
+    ecap = lib().extremz_zstd_bound(8 * n)
+    if icap < 0 or ecap < 0:
+        raise ExtremzError(-38, "libzstd.so.1")
+    iz = np.empty(icap, np.uint8)
+    ez_ = np.empty(ecap, np.uint8)
+    st = CostStats()
+    _check(lib().extremz_cost(_p(o), _p(b), _p(a), _p(c), n, float(xi), int(original_bytes),
+                              int(compressed_bytes), int(level), _p(iz), iz.size, _p(ez_),
+                              ez_.size, ctypes.byref(st)))
+    d = {k: getattr(st, k) for k, _ in CostStats._fields_}
+    return d, iz[:st.index_zst_bytes].tobytes(), ez_[:st.edits_zst_bytes].tobytes()
 
 
 def getdata(filename, shape, er):

@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libextremz.so")
-SOURCES = ["engine.hip", "kernels.hip", "pool.hip"]
+SOURCES = ["engine.hip", "kernels.hip", "pool.hip", "cost.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -17,7 +17,7 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     # packed fp32 instructions stay off
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-pthread",
+    "-Wall", "-Wno-unused-function", "-pthread", "-ldl",
 ]
 
 

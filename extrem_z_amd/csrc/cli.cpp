@@ -2,6 +2,7 @@
 //
 //   extremz_cli <path,W,H,D> <er> <compressor_id> [--decp decp.bin]
 //               [--orig-f32] [--out fixed.bin] [--counts counts.bin]
+//               [--cost prefix] [--cmp-bytes n]
 //
 // Same positional arguments as main.cpp:13-37: a raw float64 field (getdata,
 // src/ExtreM.cpp:95-118; --orig-f32 for float32), the relative bound er
@@ -9,12 +10,18 @@
 // only names outputs.  The reference produces decp with SZ3 inside getdata
 // (src/ExtreM.cpp:121-131); here decp is read from --decp, or, without it,
 // made by the uniform quantiser decp = 2*xi*rint(orig/(2*xi)).  The corrected
-// field is written as raw float64 to fixed.bin (main.cpp:507).
+// field is written as raw float64 to fixed.bin (main.cpp:507).  With --cost,
+// the edit container of cost() (src/ExtreM.cpp:272-389) is written as
+// <prefix>index.bin / <prefix>edits.bin and their zstd frames (.zst), and the
+// statistics block of the reference's result file is appended to
+// <prefix>stats.txt; --cmp-bytes is the lossy compressor's output size
+// (SZ3's cmpSize in the reference).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iomanip>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -65,7 +72,8 @@ int main(int argc, char** argv) {
   }
   double er = std::strtod(argv[2], nullptr);
   std::string compressor_id = argv[3];
-  std::string decp_path, out_path = "fixed.bin", counts_path;
+  std::string decp_path, out_path = "fixed.bin", counts_path, cost_prefix;
+  long long cmp_bytes = 0;
   bool f32 = false;
   for (int i = 4; i < argc; i++) {
     std::string a = argv[i];
@@ -73,6 +81,8 @@ int main(int argc, char** argv) {
     else if (a == "--out" && i + 1 < argc) out_path = argv[++i];
     else if (a == "--counts" && i + 1 < argc) counts_path = argv[++i];
     else if (a == "--orig-f32") f32 = true;
+    else if (a == "--cost" && i + 1 < argc) cost_prefix = argv[++i];
+    else if (a == "--cmp-bytes" && i + 1 < argc) cmp_bytes = std::atoll(argv[++i]);
     else {
       std::fprintf(stderr, "unknown argument %s\n", a.c_str());
       return 2;
@@ -124,5 +134,45 @@ int main(int argc, char** argv) {
   std::printf("edit ratio: %.16f\nwall ms: %.3f\n", (double)changed / (double)N, st.wall_ms);
   if (!write_raw(out_path, decp.data(), N * 8)) return 1;
   if (!counts_path.empty() && !write_raw(counts_path, counts.data(), N)) return 1;
+  if (!cost_prefix.empty()) {
+    // raw streams (the two files of cost()) and their zstd frames
+    std::vector<uint8_t> bitmap((3 * N + 7) / 8);
+    std::vector<double> edits(N);
+    int64_t n_edits = 0;
+    rc = extremz_encode(orig.data(), before.data(), decp.data(), counts.data(), (int64_t)N, xi,
+                        bitmap.data(), edits.data(), nullptr, nullptr, &n_edits);
+    const int64_t icap = extremz_zstd_bound((int64_t)bitmap.size());
+    const int64_t ecap = extremz_zstd_bound(8 * n_edits);
+    if (rc != 0 || icap < 0 || ecap < 0) {
+      std::fprintf(stderr, "cost failed (%d): %s\n", rc != 0 ? rc : (int)icap,
+                   extremz_last_error());
+      return 1;
+    }
+    std::vector<uint8_t> iz(icap), ez(ecap > 0 ? ecap : 1);
+    extremz_cost_stats cs;
+    const int64_t orig_bytes = (int64_t)N * (f32 ? 4 : 8);
+    rc = extremz_cost(orig.data(), before.data(), decp.data(), counts.data(), (int64_t)N, xi,
+                      orig_bytes, cmp_bytes, 3, iz.data(), icap, ez.data(), (int64_t)ez.size(),
+                      &cs);
+    if (rc != 0) {
+      std::fprintf(stderr, "cost failed (%d): %s\n", rc, extremz_last_error());
+      return 1;
+    }
+    if (!write_raw(cost_prefix + "index.bin", bitmap.data(), bitmap.size()) ||
+        !write_raw(cost_prefix + "index.bin.zst", iz.data(), (size_t)cs.index_zst_bytes) ||
+        !write_raw(cost_prefix + "edits.bin", edits.data(), (size_t)n_edits * 8) ||
+        !write_raw(cost_prefix + "edits.bin.zst", ez.data(), (size_t)cs.edits_zst_bytes))
+      return 1;
+    std::ofstream sf(cost_prefix + "stats.txt", std::ios::app);
+    sf << std::to_string(xi) << ":\n" << std::setprecision(17) << "related_error: " << er
+       << "\nOCR: " << cs.ocr << "\nCR: " << cs.cr << "\nOBR: " << cs.obr << "\nBR: " << cs.br
+       << "\npsnr: " << cs.psnr << "\nfixed_psnr: " << cs.fixed_psnr
+       << "\nedit_ratio: " << cs.edit_ratio << "\nadditional_time: " << st.wall_ms / 1000.0
+       << "\n\n";
+    std::printf("%lld,%.17g\npsnr: %.17g\nfixed_psnr: %.17g\nindex zst bytes: %lld\n"
+                "edits zst bytes: %lld\n",
+                (long long)cs.changed, cs.edit_ratio, cs.psnr, cs.fixed_psnr,
+                (long long)cs.index_zst_bytes, (long long)cs.edits_zst_bytes);
+  }
   return 0;
 }

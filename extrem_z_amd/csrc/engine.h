@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -114,6 +115,10 @@ void free_ctx(extremz_ctx* c);
 size_t ctx_bytes(extremz_ctx* c);            // device bytes held by the work buffers
 void ctx_release(extremz_ctx* c);            // frees the work buffers (kept: stream, LUT)
 bool valid_shape(int W, int H, int D);
+std::mutex& cache_mutex();                   // guards the per-device contexts of the host entry points
+extremz_ctx* cached_ctx(int device);         // the shared context of a device (caller holds the mutex)
+int ctx_device(extremz_ctx* c);
+hipStream_t ctx_stream(extremz_ctx* c);
 int run_correct(extremz_ctx* c, const double* d_orig, double* d_decp, int W, int H, int D,
                 double xi, const extremz_opts* opts, extremz_stats* stats, uint8_t* d_counts);
 // One correction with orig float32 / float64 and host or device buffers:

@@ -835,7 +835,11 @@ bool valid_shape(int W, int H, int D) {
   return N <= (1LL << 30);  // entries 2*v + side must fit an int
 }
 
-static extremz_ctx* cached_ctx(int device) {
+std::mutex& cache_mutex() { return g_cache_mu; }
+int ctx_device(extremz_ctx* c) { return c->device; }
+hipStream_t ctx_stream(extremz_ctx* c) { return c->stream; }
+
+extremz_ctx* cached_ctx(int device) {
   for (extremz_ctx* c : g_cache)
     if (c->device == device) return c;
   extremz_ctx* c = new_ctx(device);

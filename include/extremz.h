@@ -26,6 +26,7 @@ extern "C" {
 #define EXTREMZ_ENOMEM (-12)   /* device allocation failed */
 #define EXTREMZ_ENODEV (-19)   /* no usable gfx950 device */
 #define EXTREMZ_EINVAL (-22)   /* bad shape, bound, dtype or non-finite input */
+#define EXTREMZ_ENOSYS (-38)   /* libzstd.so.1 is not available */
 
 #define EXTREMZ_DTYPE_F32 0
 #define EXTREMZ_DTYPE_F64 1
@@ -141,6 +142,42 @@ int extremz_encode(const double* orig, const double* decp_before,
                    const double* decp_after, const uint8_t* counts, int64_t n,
                    double xi, uint8_t* index_bitmap, double* edits,
                    uint64_t* mantissas, int32_t* exponents, int64_t* n_edits);
+
+/* The edit container and statistics of cost() (src/ExtreM.cpp:272-389):
+ * the 3-bit index bitmap and the snapped edits of extremz_encode, each
+ * compressed to one zstd frame (the reference runs `zstd -f` on the two
+ * files, src/ExtreM.cpp:312-340: level 3), plus the reference's statistics.
+ * psnr / fixed_psnr are calculatePSNR (src/ExtreM.cpp:252-270) of
+ * decp_before / decp_after against orig over orig's max - min.  The ratios
+ * follow src/ExtreM.cpp:346-383 with original_bytes (the input file size) and
+ * compressed_bytes (the lossy compressor's output, SZ3's cmpSize in the
+ * reference); a ratio that needs a size the caller gave as 0 is NaN.
+ * index_zst / edits_zst are nullable; their capacities must be at least
+ * extremz_zstd_bound() of the raw stream sizes (ceil(3N/8) and 8 * snapped
+ * edits, N at most).  Host pointers; runs on the GPU. */
+typedef struct extremz_cost_stats {
+  int64_t changed;          /* vertices with decp_after != decp_before */
+  int64_t snapped;          /* of those, edit count 6: entries of the edit stream */
+  double edit_ratio;        /* changed / N */
+  double psnr, fixed_psnr;
+  int64_t index_bytes, index_zst_bytes;
+  int64_t edits_bytes, edits_zst_bytes;
+  double ocr, cr, obr, br;
+} extremz_cost_stats;
+
+int extremz_cost(const double* orig, const double* decp_before, const double* decp_after,
+                 const uint8_t* counts, int64_t n, double xi, int64_t original_bytes,
+                 int64_t compressed_bytes, int zstd_level, uint8_t* index_zst,
+                 int64_t index_zst_cap, uint8_t* edits_zst, int64_t edits_zst_cap,
+                 extremz_cost_stats* stats);
+
+/* zstd single frames through libzstd.so.1 (host only, no GPU).  bound: the
+ * capacity a frame of raw_bytes may need.  compress returns the frame size;
+ * decompress returns the content size, which exceeds cap when the buffer is
+ * too small (nothing is written then).  EXTREMZ_ENOSYS without libzstd. */
+int64_t extremz_zstd_bound(int64_t raw_bytes);
+int64_t extremz_zstd_compress(const void* src, int64_t n, void* dst, int64_t cap, int level);
+int64_t extremz_zstd_decompress(const void* src, int64_t n, void* dst, int64_t cap);
 
 /* ---------------------------------------------------------------------------
  * Batching and serving.  No reference counterpart: the reference corrects one

@@ -48,6 +48,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <unordered_map>
 #include <vector>
@@ -1006,6 +1007,45 @@ int64_t oracle_bitmap3(const uint8_t* vals, int64_t n, uint8_t* out) {
   }
   if (bit > 0) out[nb++] = (uint8_t)cur;
   return nb;
+}
+
+// cost() of src/ExtreM.cpp:272-389 without the files: indexs / edits /
+// cnt / ratio (:273-294), the 3-bit bitmap of indexs and calculatePSNR
+// (:252-270: serial sum of squared differences, -20 log10(sqrt(mse)/range))
+// of decp_before and decp_after against orig over max - min of orig.
+// out = {changed, snapped, edit_ratio, psnr, fixed_psnr}; bitmap receives
+// ceil(3N/8) bytes, edits the snapped edits in index order.
+int oracle_cost(const double* orig, const double* before, const double* after,
+                const uint8_t* counts, int64_t n, double xi, double* out, uint8_t* bitmap,
+                double* edits) {
+  std::vector<uint8_t> indexs((size_t)n, 0);
+  int64_t cnt = 0, ne = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (before[i] != after[i]) {
+      indexs[i] = counts[i];
+      if (indexs[i] == 6) edits[ne++] = -(before[i] - (orig[i] - xi));
+      cnt++;
+    }
+  }
+  if (oracle_bitmap3(indexs.data(), n, bitmap) < 0) return -22;
+  double mn = orig[0], mx = orig[0];
+  for (int64_t i = 0; i < n; i++) {
+    mn = std::min(mn, orig[i]);
+    mx = std::max(mx, orig[i]);
+  }
+  auto psnr = [&](const double* c) {
+    double mse = 0.0;
+    for (int64_t i = 0; i < n; i++) mse += std::pow(orig[i] - c[i], 2);
+    mse /= (double)n;
+    if (mse == 0) return std::numeric_limits<double>::infinity();
+    return -20.0 * std::log10(std::sqrt(mse) / (mx - mn));
+  };
+  out[0] = (double)cnt;
+  out[1] = (double)ne;
+  out[2] = (double)cnt / (double)n;
+  out[3] = psnr(before);
+  out[4] = psnr(after);
+  return 0;
 }
 
 }  // extern "C"

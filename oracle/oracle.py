@@ -40,6 +40,7 @@ def lib():
         L.oracle_correct.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, vp, vp, i32]
         L.oracle_bitmap3.argtypes = [vp, i64, vp]
         L.oracle_bitmap3.restype = i64
+        L.oracle_cost.argtypes = [vp, vp, vp, vp, i64, dbl, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -107,3 +108,20 @@ def bitmap3(vals):
     if n < 0:
         raise ValueError("value outside 0..6")
     return out[:n]
+
+
+def cost(orig, before, after, counts, xi):
+    """cost() restated: ({changed, snapped, edit_ratio, psnr, fixed_psnr}, bitmap, edits)."""
+    o = np.ascontiguousarray(orig, dtype=np.float64).ravel()
+    b = np.ascontiguousarray(before, dtype=np.float64).ravel()
+    a = np.ascontiguousarray(after, dtype=np.float64).ravel()
+    c = np.ascontiguousarray(counts, dtype=np.uint8).ravel()
+    out = np.zeros(5, np.float64)
+    bm = np.zeros((3 * o.size + 7) // 8 + 1, np.uint8)
+    ed = np.zeros(o.size, np.float64)
+    rc = lib().oracle_cost(_p(o), _p(b), _p(a), _p(c), o.size, float(xi), _p(out), _p(bm), _p(ed))
+    if rc:
+        raise ValueError("edit count outside 0..6")
+    st = {"changed": int(out[0]), "snapped": int(out[1]), "edit_ratio": float(out[2]),
+          "psnr": float(out[3]), "fixed_psnr": float(out[4])}
+    return st, bm[:(3 * o.size + 7) // 8], ed[:st["snapped"]]
