@@ -81,6 +81,12 @@ class CostStats(ctypes.Structure):
                 ("cr", ctypes.c_double), ("obr", ctypes.c_double), ("br", ctypes.c_double)]
 
 
+class TreeCmp(ctypes.Structure):
+    _fields_ = [("n_max_a", ctypes.c_int64), ("n_max_b", ctypes.c_int64),
+                ("rounds_a", ctypes.c_int64), ("rounds_b", ctypes.c_int64),
+                ("pair_mismatch", ctypes.c_int64), ("branch_mismatch", ctypes.c_int64)]
+
+
 class PoolInfo(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_int64), ("footprint_bytes", ctypes.c_int64),
                 ("peak_footprint_bytes", ctypes.c_int64), ("submitted", ctypes.c_int64),
@@ -115,6 +121,8 @@ def lib():
         L.extremz_zstd_compress.restype = i64
         L.extremz_zstd_decompress.argtypes = [vp, i64, vp, i64]
         L.extremz_zstd_decompress.restype = i64
+        L.extremz_split_tree.argtypes = [vp, i32, i32, i32, vp, vp, vp, vp, vp, i32, i32, vp]
+        L.extremz_compare_split_trees.argtypes = [vp, vp, i32, i32, i32, vp]
         L.extremz_pool_create.argtypes = [i32, i32, i64, vp, ctypes.POINTER(vp)]
         L.extremz_pool_destroy.argtypes = [vp]
         L.extremz_pool_info_get.argtypes = [vp, vp]
@@ -342,6 +350,35 @@ def encode(orig, decp_before, decp_after, counts, xi):
                                 _p(mant), _p(expo), ctypes.byref(ne)))
     k = ne.value
     return bm, edits[:k].copy(), mant[:k].copy(), expo[:k].copy()
+
+
+def split_tree(field):
+    """compute_MergeT of one field: dict with per-maximum (ascending (value, id))
+    'max', 'pair_saddle', 'parent' vertex ids, 'branch' vertex lists, 'rounds'."""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    cap = f.size + 1
+    mx, ps, pa = (np.zeros(cap, np.int32) for _ in range(3))
+    off = np.zeros(cap + 1, np.int32)
+    bv = np.zeros(3 * cap, np.int32)
+    rounds = ctypes.c_int(0)
+    n = lib().extremz_split_tree(_p(f), *_dims(f), _p(mx), _p(ps), _p(pa), _p(off), _p(bv), cap,
+                                 bv.size, ctypes.byref(rounds))
+    if n < 0:
+        _check(n if n > -(f.size + 2) * 4 else -22)
+    return {"max": mx[:n].copy(), "pair_saddle": ps[:n].copy(), "parent": pa[:n].copy(),
+            "branch": [bv[off[i]:off[i + 1]].tolist() for i in range(n)],
+            "rounds": rounds.value}
+
+
+def compare_split_trees(a, b):
+    """Branch / pair comparison of main.cpp:480-504 between two fields."""
+    fa = np.ascontiguousarray(a, dtype=np.float64)
+    fb = np.ascontiguousarray(b, dtype=np.float64)
+    if fa.shape != fb.shape:
+        raise ValueError("shapes differ")
+    out = TreeCmp()
+    _check(lib().extremz_compare_split_trees(_p(fa), _p(fb), *_dims(fa), ctypes.byref(out)))
+    return {k: getattr(out, k) for k, _ in TreeCmp._fields_}
 
 
 def zstd_compress(data, level=3):

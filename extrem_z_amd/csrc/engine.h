@@ -167,7 +167,6 @@ __global__ void k_clear_dirty(St s);
 __global__ void k_propose_c(St s, const int* front, int fctr);
 __global__ void k_apply(St s);
 __global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t* lower);
-__global__ void k_jump(int n, int* ds, int* as);
 __global__ void k_chase(int n, int* ds, int* as);
 __global__ void k_list_fill(const int* list, int n, int* arr, int val);
 __global__ void k_saddle_list(St s, int mode, const double* f, const int* L2, const int* L1,
@@ -213,5 +212,16 @@ __global__ void k_edit_write(const double* orig, const double* before, const int
                              unsigned long long* mant, int* expo);
 __global__ void k_scan_block(const int* in, int* out, long long n, int* block_sums);
 __global__ void k_scan_add(int* out, long long n, const int* offs);
+__global__ void k_tree_triplets(Geo g, const double* f, const int* S2, int n2, const int* ds,
+                                const int* ext_id, const long long* mkey, const int* MX, int* T,
+                                int* cnt);
+__global__ void k_tree_best(const int* T, const int* cnt, int n2, const long long* skey, int gmax,
+                            int pass, long long* bestkey, int* bestpos);
+__global__ void k_tree_pair(const int* T, const int* cnt, int nmax, const int* bestpos,
+                            int* pair_pos, int* parent, int* ptr, int* changed);
+__global__ void k_tree_compress(int n, int* ptr);
+__global__ void k_tree_relabel(int* T, int* cnt, int n2, const int* ptr, const long long* mkey,
+                               const int* MX);
+__global__ void k_list_keys(const int* list, int n, const double* f, long long* key);
 
 }  // namespace ez

@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <vector>
 
@@ -187,6 +188,14 @@ namespace ez {
 
 thread_local std::string g_err;
 
+// Split tree of one field (compute_MergeT): per maximum in ascending
+// (value, id) order.
+struct TreeOut {
+  std::vector<int> max_v, pair_v, parent_v;
+  std::vector<std::vector<int>> branch;
+  int rounds = 0;
+};
+
 class Run {
  public:
   Run(extremz_ctx* c, const extremz_opts* o) : c_(c) {
@@ -200,6 +209,7 @@ class Run {
               uint8_t* d_counts);
   void classify(const double* d_f, int W, int H, int D, uint8_t* d_t, uint16_t* d_l);
   int labels(const double* d_f, int W, int H, int D, int* d_ds, int* d_as);
+  void split_tree(const double* d_f, int W, int H, int D, TreeOut& out);
 
   int64_t counts_[EXTREMZ_STAT_COUNT_] = {};
 
@@ -392,21 +402,18 @@ void Run::apply() {
 }
 
 // ComputeDescendingManifold + ComputeAscendingManifold on field f (D7): tile
-// resolution, two jumping rounds, then a chase of whatever is left.  No host
-// round trip.
+// resolution, then a chase of whatever is left (a separate pointer-jumping
+// round before the chase costs more than the chase saves).  No host round
+// trip.
 void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower) {
   counts_[EXTREMZ_STAT_LABEL_PASSES]++;
   tbegin(EXTREMZ_K_LABEL_TILE);
   k_labels_tile<<<tile_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as, lower);
   tend();
   int nb = (s_.g.N + 255) / 256;
-  tbegin(EXTREMZ_K_LABEL_JUMP);
-  k_jump<<<nb, 256, 0, st()>>>(s_.g.N, ds, as);
-  tend();
   tbegin(EXTREMZ_K_LABEL_CHASE);
   k_chase<<<nb, 256, 0, st()>>>(s_.g.N, ds, as);
   tend();
-  counts_[EXTREMZ_STAT_JUMP_ROUNDS] += 2;
 }
 
 // c_loops, src/ExtreM.cpp:1505-1583.  The first call scans the full grid; later
@@ -569,6 +576,203 @@ int Run::labels(const double* d_f, int W, int H, int D, int* d_ds, int* d_as) {
   ez::hip_check(hipGetLastError(), "labels");
   ez::hip_check(hipStreamSynchronize(st()), "sync");
   return 0;
+}
+
+// compute_MergeT (src/ExtreM.cpp:2930-2952) as main.cpp:454-463 runs it
+// after the correction, on one field with its own types, labels and value
+// order: the triplets and the rounds of constructPersistencePairs
+// (src/ExtreM.cpp:2132-2249) on the device, constructMergeTree
+// (src/ExtreM.cpp:1863-1918) and the branch lists on the host (one entry per
+// maximum).  Ties in value are ranked by vertex id; maxima never paired
+// keep the reference's zero-initialised (saddle index 0, rank 0) record.
+void Run::split_tree(const double* d_f, int W, int H, int D, TreeOut& out) {
+  setup_geo(W, H, D);
+  const long long N = s_.g.N;
+  extremz_ctx* c = c_;
+  hipStream_t q = st();
+  c->otype.ensure(N);
+  c->olow.ensure(N);
+  c->oDS.ensure(N);
+  c->oAS.ensure(N);
+  for (DevBuf<int>* b : {&c->S2, &c->S1, &c->MX, &c->MN, &c->ext_id}) b->ensure(N);
+  c->ctr.ensure(C_COUNT_);
+  s_ = St{};
+  setup_geo(W, H, D);
+  s_.lut = c->lut.p;
+  s_.otype = c->otype.p;
+  s_.olow = c->olow.p;
+  s_.S2 = c->S2.p;
+  s_.S1 = c->S1.p;
+  s_.MX = c->MX.p;
+  s_.MN = c->MN.p;
+  s_.ext_id = c->ext_id.p;
+  s_.ctr = c->ctr.p;
+  hip_check(hipMemsetAsync(s_.ctr, 0, sizeof(int) * C_COUNT_, q), "memset");
+  k_classify<<<row_blocks(), dim3(64, 4, 1), 0, q>>>(s_.g, s_.lut, d_f, s_.otype, s_.olow);
+  label_pass(d_f, c->oDS.p, c->oAS.p, nullptr);
+  const int nb = (int)((N + 255) / 256);
+  c->lcnt.ensure(4LL * nb);
+  k_lists_count<<<nb, 256, 0, q>>>(s_, c->lcnt.p, nb);
+  k_lists_scan<<<1, 1024, 0, q>>>(s_, c->lcnt.p, nb);
+  k_lists_write<<<nb, 256, 0, q>>>(s_, c->lcnt.p, nb);
+  set_ctr2(C_GMIN, C_GMAX, kNone);
+  k_argext<<<blocks_for(N), 256, 0, q>>>(d_f, (int)N, s_.ctr + C_GMIN, s_.ctr + C_GMAX);
+  pull();
+  const int n2 = h(C_N2), nmax = h(C_NMAX), gmin = h(C_GMIN);
+  out = TreeOut{};
+  if (nmax == 0) return;
+  s_.nmax = nmax;
+  s_.nmin = 0;
+  k_fill_i32<<<blocks_for(N), 256, 0, q>>>(s_.ext_id, -1, N);
+  k_ext_index<<<blocks_for(nmax), 256, 0, q>>>(s_);
+  DevBuf<long long> mkey, skey, bestkey;
+  DevBuf<int> T, cnt, pair_pos, parent, ptr, bestpos;
+  mkey.ensure(nmax);
+  skey.ensure((size_t)n2 + 1);
+  bestkey.ensure(nmax);
+  T.ensure((size_t)n2 * kNbr + 1);
+  cnt.ensure((size_t)n2 + 1);
+  for (DevBuf<int>* b : {&pair_pos, &parent, &ptr, &bestpos}) b->ensure(nmax);
+  k_list_keys<<<blocks_for(nmax), 256, 0, q>>>(s_.MX, nmax, d_f, mkey.p);
+  if (n2) k_list_keys<<<blocks_for(n2), 256, 0, q>>>(s_.S2, n2, d_f, skey.p);
+  std::vector<int> hMX(nmax), hS2(n2);
+  std::vector<long long> hmk(nmax), hsk(n2);
+  hip_check(hipMemcpyAsync(hMX.data(), s_.MX, 4LL * nmax, hipMemcpyDeviceToHost, q), "d2h");
+  hip_check(hipMemcpyAsync(hmk.data(), mkey.p, 8LL * nmax, hipMemcpyDeviceToHost, q), "d2h");
+  if (n2) {
+    hip_check(hipMemcpyAsync(hS2.data(), s_.S2, 4LL * n2, hipMemcpyDeviceToHost, q), "d2h");
+    hip_check(hipMemcpyAsync(hsk.data(), skey.p, 8LL * n2, hipMemcpyDeviceToHost, q), "d2h");
+  }
+  hip_check(hipStreamSynchronize(q), "sync");
+  auto max_less = [&](int a, int b) {
+    return hmk[a] < hmk[b] || (hmk[a] == hmk[b] && hMX[a] < hMX[b]);
+  };
+  int gmax = 0;
+  for (int p = 1; p < nmax; p++)
+    if (max_less(gmax, p)) gmax = p;
+  std::vector<int> iota(nmax);
+  for (int p = 0; p < nmax; p++) iota[p] = p;
+  hip_check(hipMemcpyAsync(ptr.p, iota.data(), 4LL * nmax, hipMemcpyHostToDevice, q), "h2d");
+  k_fill_i32<<<blocks_for(nmax), 256, 0, q>>>(pair_pos.p, -1, nmax);
+  k_fill_i32<<<blocks_for(nmax), 256, 0, q>>>(parent.p, -1, nmax);
+  if (n2)
+    k_tree_triplets<<<blocks_for(n2), 256, 0, q>>>(s_.g, d_f, s_.S2, n2, c->oDS.p, s_.ext_id,
+                                                   mkey.p, s_.MX, T.p, cnt.p);
+  // constructPersistencePairs: rounds until no maximum is paired
+  for (int round = 0;; round++) {
+    if (round > nmax + 1) throw HipError("split tree pairing did not finish");
+    k_fill_pair<<<blocks_for(nmax), 256, 0, q>>>(bestkey.p, (long long)0x8000000000000000ULL,
+                                                 bestpos.p, -1, nmax);
+    if (n2) {
+      k_tree_best<<<blocks_for(n2), 256, 0, q>>>(T.p, cnt.p, n2, skey.p, gmax, 0, bestkey.p,
+                                                 bestpos.p);
+      k_tree_best<<<blocks_for(n2), 256, 0, q>>>(T.p, cnt.p, n2, skey.p, gmax, 1, bestkey.p,
+                                                 bestpos.p);
+    }
+    set_ctr(C_CHANGED, 0);
+    k_tree_pair<<<blocks_for(nmax), 256, 0, q>>>(T.p, cnt.p, nmax, bestpos.p, pair_pos.p,
+                                                 parent.p, ptr.p, s_.ctr + C_CHANGED);
+    k_tree_compress<<<blocks_for(nmax), 256, 0, q>>>(nmax, ptr.p);
+    if (n2)
+      k_tree_relabel<<<blocks_for(n2), 256, 0, q>>>(T.p, cnt.p, n2, ptr.p, mkey.p, s_.MX);
+    pull();
+    out.rounds++;
+    if (!h(C_CHANGED)) break;
+  }
+  std::vector<int> hpair(nmax), hpar(nmax);
+  hip_check(hipMemcpyAsync(hpair.data(), pair_pos.p, 4LL * nmax, hipMemcpyDeviceToHost, q), "d2h");
+  hip_check(hipMemcpyAsync(hpar.data(), parent.p, 4LL * nmax, hipMemcpyDeviceToHost, q), "d2h");
+  hip_check(hipStreamSynchronize(q), "sync");
+  // ranks: ascending (value, id)
+  std::vector<int> byrank(nmax), rank(nmax);
+  for (int p = 0; p < nmax; p++) byrank[p] = p;
+  std::sort(byrank.begin(), byrank.end(), max_less);
+  for (int r = 0; r < nmax; r++) rank[byrank[r]] = r;
+  // saddle order: descending (value, id) (saddles2 of computeCP); index 0 = largest
+  auto sad_larger = [&](int a, int b) {  // list positions in S2
+    return hsk[a] > hsk[b] || (hsk[a] == hsk[b] && hS2[a] > hS2[b]);
+  };
+  int sad0 = -1;
+  for (int i = 0; i < n2; i++)
+    if (sad0 < 0 || sad_larger(i, sad0)) sad0 = i;
+  // maximaTriplets in rank order: {saddle position (-1: global minimum), parent rank}
+  std::vector<std::pair<int, int>> mt(nmax);
+  out.max_v.resize(nmax);
+  out.pair_v.resize(nmax);
+  std::vector<char> unpaired(nmax, 0);
+  for (int r = 0; r < nmax; r++) {
+    const int p = byrank[r];
+    out.max_v[r] = hMX[p];
+    if (p == gmax) {
+      out.pair_v[r] = gmin;
+      mt[r] = {-1, r};
+    } else if (hpair[p] >= 0) {
+      out.pair_v[r] = hS2[hpair[p]];
+      mt[r] = {hpair[p], rank[hpar[p]]};
+    } else {  // zero-initialised record of the reference
+      out.pair_v[r] = 0;
+      mt[r] = {sad0, 0};
+      unpaired[r] = 1;
+    }
+  }
+  // constructMergeTree: follow larger saddles to the parent
+  for (bool same = false; !same;) {
+    same = true;
+    for (int i = 0; i < nmax; i++) {
+      const auto next = mt[mt[i].second];
+      if (next.second != mt[i].second && next.first >= 0 && mt[i].first >= 0 &&
+          next.first != mt[i].first && sad_larger(next.first, mt[i].first)) {
+        mt[i].second = next.second;
+        same = false;
+      }
+    }
+  }
+  // branches: the maximum, its saddle (or the root marker) and the saddles
+  // of its children, descending by (value, id)
+  struct Ent {
+    long long key;
+    int id;
+  };
+  std::vector<std::vector<Ent>> br(nmax);
+  const Ent marker{std::numeric_limits<long long>::min(), 0};
+  // a maximum paired with a saddle whose triplet held only itself is its own
+  // parent; the reference then lists the saddle's index in saddles2 (sorted
+  // by descending (value, id)) as the root marker
+  std::vector<int> sad_index;
+  auto index_of = [&](int sp) {
+    if (sad_index.empty()) {
+      std::vector<int> o(n2);
+      for (int i = 0; i < n2; i++) o[i] = i;
+      std::sort(o.begin(), o.end(), sad_larger);
+      sad_index.resize(n2);
+      for (int i = 0; i < n2; i++) sad_index[o[i]] = i;
+    }
+    return sad_index[sp];
+  };
+  for (int b = 0; b < nmax; b++) {
+    br[b].push_back({hmk[byrank[b]], out.max_v[b]});
+    const int par = mt[b].second;
+    if (par != b && mt[b].first >= 0) {
+      const int sp = mt[b].first;
+      const Ent e{hsk[sp], hS2[sp]};
+      br[b].push_back(e);
+      br[par].push_back(e);
+    } else {
+      Ent m = marker;
+      m.id = byrank[b] == gmax ? gmin : (unpaired[b] ? 0 : index_of(mt[b].first));
+      br[b].push_back(m);
+    }
+  }
+  out.parent_v.resize(nmax);
+  out.branch.resize(nmax);
+  for (int b = 0; b < nmax; b++) {
+    out.parent_v[b] = out.max_v[mt[b].second];
+    auto& v = br[b];
+    std::sort(v.begin(), v.end(), [](const Ent& a, const Ent& e) {
+      return a.key > e.key || (a.key == e.key && a.id > e.id);
+    });
+    for (const Ent& e : v) out.branch[b].push_back(e.id);
+  }
 }
 
 int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
@@ -1055,6 +1259,72 @@ int extremz_encode(const double* orig, const double* decp_before, const double* 
     ez::hip_check(hipStreamSynchronize(q), "sync");
     return EXTREMZ_OK;
   });
+}
+
+static int tree_of(const double* field, int W, int H, int D, ez::TreeOut& t) {
+  extremz_ctx* c = ez::cached_ctx(0);
+  if (!c) return EXTREMZ_ENODEV;
+  ez::hip_check(hipSetDevice(c->device), "hipSetDevice");
+  const size_t N = (size_t)W * H * D;
+  c->orig_tmp.ensure(N);
+  ez::hip_check(hipMemcpyAsync(c->orig_tmp.p, field, N * 8, hipMemcpyHostToDevice, c->stream),
+                "h2d");
+  ez::Run r(c, nullptr);
+  r.split_tree(c->orig_tmp.p, W, H, D, t);
+  return EXTREMZ_OK;
+}
+
+int extremz_split_tree(const double* field, int W, int H, int D, int32_t* max_v, int32_t* pair_v,
+                       int32_t* parent_v, int32_t* branch_off, int32_t* branch_v, int cap_max,
+                       int cap_branch, int* rounds) {
+  if (!field || !ez::valid_shape(W, H, D)) return EXTREMZ_EINVAL;
+  std::lock_guard<std::mutex> lk(ez::g_cache_mu);
+  ez::TreeOut t;
+  int rc = ez::guarded([&] { return tree_of(field, W, H, D, t); });
+  if (rc != EXTREMZ_OK) return rc;
+  const int n = (int)t.max_v.size();
+  int total = 0;
+  for (auto& b : t.branch) total += (int)b.size();
+  if (n > cap_max || total > cap_branch || !max_v || !pair_v || !parent_v || !branch_off ||
+      !branch_v)
+    return -std::max(n, total) - 1;
+  int k = 0;
+  for (int i = 0; i < n; i++) {
+    max_v[i] = t.max_v[i];
+    pair_v[i] = t.pair_v[i];
+    parent_v[i] = t.parent_v[i];
+    branch_off[i] = k;
+    for (int v : t.branch[i]) branch_v[k++] = v;
+  }
+  branch_off[n] = k;
+  if (rounds) *rounds = t.rounds;
+  return n;
+}
+
+int extremz_compare_split_trees(const double* a, const double* b, int W, int H, int D,
+                                extremz_tree_cmp* out) {
+  if (!a || !b || !out || !ez::valid_shape(W, H, D)) return EXTREMZ_EINVAL;
+  std::lock_guard<std::mutex> lk(ez::g_cache_mu);
+  ez::TreeOut ta, tb;
+  int rc = ez::guarded([&] {
+    int r = tree_of(a, W, H, D, ta);
+    return r != EXTREMZ_OK ? r : tree_of(b, W, H, D, tb);
+  });
+  if (rc != EXTREMZ_OK) return rc;
+  const size_t na = ta.max_v.size(), nb = tb.max_v.size(), n = std::min(na, nb);
+  *out = extremz_tree_cmp{};
+  out->n_max_a = (int64_t)na;
+  out->n_max_b = (int64_t)nb;
+  out->rounds_a = ta.rounds;
+  out->rounds_b = tb.rounds;
+  const int64_t extra = (int64_t)(std::max(na, nb) - n);
+  out->pair_mismatch = extra;
+  out->branch_mismatch = extra;
+  for (size_t i = 0; i < n; i++) {
+    out->pair_mismatch += ta.pair_v[i] != tb.pair_v[i] || ta.max_v[i] != tb.max_v[i];
+    out->branch_mismatch += ta.branch[i] != tb.branch[i];
+  }
+  return EXTREMZ_OK;
 }
 
 }  // extern "C"

@@ -639,17 +639,6 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
   }
 }
 
-// One in-place pointer-jumping round on both label arrays.
-__global__ void __launch_bounds__(256) k_jump(int n, int* ds, int* as) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  int r = ds[v], rr = ds[r];
-  if (r != rr) ds[v] = rr;
-  r = as[v];
-  rr = as[r];
-  if (r != rr) as[v] = rr;
-}
-
 // Follows every pointer that is not yet at a root to its root (D7: every
 // vertex is compressed).  Other lanes' writes only ever replace a pointer by
 // one further along the same path, so any value read is an ancestor; the
@@ -1728,6 +1717,135 @@ __global__ void __launch_bounds__(1024) k_scan_block(const int* in, int* out, lo
 __global__ void __launch_bounds__(1024) k_scan_add(int* out, long long n, const int* offs) {
   long long i = blockIdx.x * 1024LL + threadIdx.x;
   if (i < n) out[i] += offs[blockIdx.x];
+}
+
+}  // namespace ez
+
+namespace ez {
+
+// ------------------------------------------------- split tree (check) ----
+// compute_MergeT (src/ExtreM.cpp:2930-2952) on the GPU.  Maxima are named by
+// their position p in MX (vertex order); mkey[p] is the ordered key of their
+// value, so (mkey, MX) is the (value, id) order of the reference's ranks.
+
+__device__ __forceinline__ bool max_less(const long long* mkey, const int* MX, int a, int b) {
+  return mkey[a] < mkey[b] || (mkey[a] == mkey[b] && MX[a] < MX[b]);
+}
+
+// findAscPaths (src/ExtreM.cpp:1597-1636): per 2-saddle, the maxima reached
+// by steepest ascent from its upper neighbours, sorted by (value, id) and
+// deduplicated, kNbr slots per saddle.
+__global__ void __launch_bounds__(256) k_tree_triplets(Geo g, const double* f, const int* S2,
+                                                       int n2, const int* ds, const int* ext_id,
+                                                       const long long* mkey, const int* MX,
+                                                       int* T, int* cnt) {
+  GRID_STRIDE(i, n2) {
+    const int v = S2[i];
+    int x, y, z;
+    coords(g, v, x, y, z);
+    const int key = c_cls_key[class_of(g, x, y, z)];
+    const double fv = f[v];
+    int loc[kNbr];
+    int m = 0;
+    for (int d = 0; d < kNbr; d++) {
+      if (!((key >> d) & 1)) continue;
+      const int r = v + g.off[d];
+      const double fr = f[r];
+      if (!(fr > fv || (fr == fv && r > v))) continue;
+      const int p = ext_id[ds[r]] >> 1;
+      int k = 0;
+      bool dup = false;
+      for (; k < m; k++) {
+        if (loc[k] == p) { dup = true; break; }
+        if (max_less(mkey, MX, p, loc[k])) break;
+      }
+      if (dup) continue;
+      for (int j = m; j > k; j--) loc[j] = loc[j - 1];
+      loc[k] = p;
+      m++;
+    }
+    for (int k = 0; k < m; k++) T[i * kNbr + k] = loc[k];
+    cnt[i] = m;
+  }
+}
+
+// constructPersistencePairs, largest saddle per maximum (src/ExtreM.cpp:2146-2166):
+// pass 0 raises the value key, pass 1 the list position among saddles with
+// that key (positions are in vertex order, so the highest position is the
+// highest id).  The global maximum (position gmax) takes no part.
+__global__ void k_tree_best(const int* T, const int* cnt, int n2, const long long* skey, int gmax,
+                            int pass, long long* bestkey, int* bestpos) {
+  GRID_STRIDE(i, n2) {
+    const int m = cnt[i];
+    for (int k = 0; k < m; k++) {
+      const int p = T[i * kNbr + k];
+      if (p == gmax) continue;
+      if (pass == 0) atomicMax(&bestkey[p], skey[i]);
+      else if (skey[i] == bestkey[p]) atomicMax(&bestpos[p], i);
+    }
+  }
+}
+
+// A maximum that is the lowest maximum of its largest saddle is paired with
+// it and points to the saddle's highest maximum (src/ExtreM.cpp:2170-2195).
+__global__ void k_tree_pair(const int* T, const int* cnt, int nmax, const int* bestpos,
+                            int* pair_pos, int* parent, int* ptr, int* changed) {
+  GRID_STRIDE(p, nmax) {
+    const int i = bestpos[p];
+    if (i < 0 || T[i * kNbr] != p) continue;
+    const int hi = T[i * kNbr + cnt[i] - 1];
+    pair_pos[p] = i;
+    parent[p] = hi;
+    ptr[p] = hi;
+    *changed = 1;
+  }
+}
+
+// Path compression of the maximum pointers (src/ExtreM.cpp:2197-2211):
+// pointers only lead to higher maxima, so every walk ends.
+__global__ void k_tree_compress(int n, int* ptr) {
+  GRID_STRIDE(p, n) {
+    int r = ptr[p];
+    int rr = __hip_atomic_load(&ptr[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int step = 0; step < n && rr != r; step++) {
+      r = rr;
+      rr = __hip_atomic_load(&ptr[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    ptr[p] = r;
+  }
+}
+
+// Relabels every triplet through the pointers, sorts and deduplicates it; a
+// triplet left with one maximum is emptied (src/ExtreM.cpp:2213-2222).
+__global__ void k_tree_relabel(int* T, int* cnt, int n2, const int* ptr, const long long* mkey,
+                               const int* MX) {
+  GRID_STRIDE(i, n2) {
+    const int m0 = cnt[i];
+    if (m0 == 0) continue;
+    int loc[kNbr];
+    int m = 0;
+    for (int j = 0; j < m0; j++) {
+      const int p = ptr[T[i * kNbr + j]];
+      int k = 0;
+      bool dup = false;
+      for (; k < m; k++) {
+        if (loc[k] == p) { dup = true; break; }
+        if (max_less(mkey, MX, p, loc[k])) break;
+      }
+      if (dup) continue;
+      for (int q = m; q > k; q--) loc[q] = loc[q - 1];
+      loc[k] = p;
+      m++;
+    }
+    if (m == 1) m = 0;
+    for (int k = 0; k < m; k++) T[i * kNbr + k] = loc[k];
+    cnt[i] = m;
+  }
+}
+
+// Per-position value keys of a vertex list.
+__global__ void k_list_keys(const int* list, int n, const double* f, long long* key) {
+  GRID_STRIDE(i, n) key[i] = okey_c(f[list[i]]);
 }
 
 }  // namespace ez

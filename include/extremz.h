@@ -46,7 +46,7 @@ enum {
   EXTREMZ_STAT_N_SADDLE1,
   EXTREMZ_STAT_N_SADDLE2,
   EXTREMZ_STAT_LABEL_PASSES,    /* full-grid manifold label computations */
-  EXTREMZ_STAT_JUMP_ROUNDS,     /* pointer-jumping rounds, all passes */
+  EXTREMZ_STAT_JUMP_ROUNDS,     /* reserved: 0 (labels are resolved in tiles, then chased) */
   EXTREMZ_STAT_HOST_SYNCS,      /* device->host counter reads */
   EXTREMZ_STAT_FRONTIER_ITEMS,  /* false-critical-point frontier entries */
   EXTREMZ_STAT_COUNT_ = 16
@@ -60,7 +60,7 @@ enum {
   EXTREMZ_K_CHECK_FULL,   /* full-grid false-critical-point scan */
   EXTREMZ_K_CHECK_INCR,   /* frontier / edited-neighbourhood re-check */
   EXTREMZ_K_LABEL_TILE,   /* steepest pointers + lower masks, resolved inside LDS tiles */
-  EXTREMZ_K_LABEL_JUMP,   /* one full-grid pointer-jumping round, both manifolds */
+  EXTREMZ_K_LABEL_JUMP,   /* reserved: no separate pointer-jumping round is launched */
   EXTREMZ_K_LABEL_CHASE,  /* full-grid walk of the remaining pointers to their roots */
   EXTREMZ_K_SADDLE_PASS,  /* saddle-list pass: triplet ends, candidates, changed saddles */
   EXTREMZ_K_SLOT_GATHER,  /* largest / smallest saddle per extremum (gather + merge + ties) */
@@ -178,6 +178,32 @@ int extremz_cost(const double* orig, const double* decp_before, const double* de
 int64_t extremz_zstd_bound(int64_t raw_bytes);
 int64_t extremz_zstd_compress(const void* src, int64_t n, void* dst, int64_t cap, int level);
 int64_t extremz_zstd_decompress(const void* src, int64_t n, void* dst, int64_t cap);
+
+/* Split-tree check: compute_MergeT (src/ExtreM.cpp:2930-2952; persistence
+ * pairs :2132-2249, merge tree and branches :1863-1918) of one float64 field
+ * with its own types, labels and value order.  Per maximum in ascending
+ * (value, id) order: its vertex, the saddle vertex of its persistence pair
+ * (the global minimum for the global maximum), its merge-tree parent's
+ * vertex, and its branch's vertices in descending (value, id) order,
+ * flattened with branch_off[0..n].  Returns the number of maxima, or
+ * -(needed size) - 1 when cap_max / cap_branch is too small.  Host pointers;
+ * runs on the GPU. */
+int extremz_split_tree(const double* field, int W, int H, int D, int32_t* max_v,
+                       int32_t* pair_v, int32_t* parent_v, int32_t* branch_off,
+                       int32_t* branch_v, int cap_max, int cap_branch, int* rounds);
+
+/* The branch comparison of main.cpp:480-504 between two fields (e.g. orig
+ * and the corrected field): branches and pairs are compared rank by rank;
+ * maxima present in one field only count as mismatches. */
+typedef struct extremz_tree_cmp {
+  int64_t n_max_a, n_max_b;
+  int64_t rounds_a, rounds_b;   /* pairing rounds of constructPersistencePairs */
+  int64_t pair_mismatch;        /* ranks whose maximum or pair saddle differs */
+  int64_t branch_mismatch;      /* ranks whose branch vertex list differs */
+} extremz_tree_cmp;
+
+int extremz_compare_split_trees(const double* a, const double* b, int W, int H, int D,
+                                extremz_tree_cmp* out);
 
 /* ---------------------------------------------------------------------------
  * Batching and serving.  No reference counterpart: the reference corrects one

@@ -904,9 +904,177 @@ struct Engine {
   }
 };
 
+// --------------------------------------------------- split tree check ----
+// compute_MergeT of the reference (src/ExtreM.cpp:2930-2952) for one field,
+// as main.cpp:454-463 runs it after the correction: computeCP's value-sorted
+// lists and triplets (src/ExtreM.cpp:1920-2040, 1597-1636),
+// constructPersistencePairs (:2132-2249) and constructMergeTree (:1863-1918).
+//   D14 getSortedPositions (:1840-1860) ranks by value with std::sort, which
+//       leaves ties unordered; ties are ranked by vertex id, as
+//       getSortedIndexes (:1826-1838) does.
+//   D15 each field is processed with its own types, labels and value order
+//       (main.cpp:459-461 sorts the original field's lists by decp).
+struct SplitTree {
+  std::vector<int> maximum;                  // ascending (value, id): rank order
+  std::vector<std::pair<int, int>> pairs;    // per rank: (saddle vertex, max vertex)
+  std::vector<std::pair<int, int>> mtrip;    // per rank: (saddle index, parent rank)
+  std::vector<std::vector<int>> branch;      // per rank: vertex ids, descending order
+  std::vector<int> saddles2;                 // descending (value, id)
+  int global_min = NONE;
+  int rounds = 0;
+};
+
+SplitTree split_tree(const Grid& g, const double* f) {
+  SplitTree t;
+  std::vector<uint8_t> type(g.N);
+  std::vector<uint16_t> low(g.N);
+  for (int64_t i = 0; i < g.N; i++) type[i] = (uint8_t)classify_vertex(g, f, (int)i, &low[i]);
+  Engine e;
+  e.g = g;
+  std::vector<int> DS;
+  e.manifold(f, DS, true);
+  for (int64_t i = 0; i < g.N; i++) {
+    if (type[i] == 2 || type[i] == 3) t.saddles2.push_back((int)i);
+    if (type[i] == 4) t.maximum.push_back((int)i);
+  }
+  std::sort(t.saddles2.begin(), t.saddles2.end(),
+            [&](int v, int u) { return islarger(v, u, f); });
+  std::sort(t.maximum.begin(), t.maximum.end(), [&](int v, int u) { return isless(v, u, f); });
+  const int nMax = (int)t.maximum.size(), nS = (int)t.saddles2.size();
+  std::vector<int> rank(g.N, NONE);  // tempArray
+  for (int i = 0; i < nMax; i++) rank[t.maximum[i]] = i;
+  // findAscPaths: ranks of the maxima reached from the upper neighbours,
+  // sorted and deduplicated (an empty triplet stays empty, D11)
+  std::vector<std::vector<int>> trip(nS);
+  for (int i = 0; i < nS; i++) {
+    int v = t.saddles2[i], nb[kNbr];
+    int m = g.nbrs(v, nb);
+    for (int k = 0; k < m; k++)
+      if (islarger(nb[k], v, f)) trip[i].push_back(rank[DS[nb[k]]]);
+    std::sort(trip[i].begin(), trip[i].end());
+    trip[i].erase(std::unique(trip[i].begin(), trip[i].end()), trip[i].end());
+  }
+  int gmin = 0;
+  for (int64_t i = 1; i < g.N; i++)
+    if (isless((int)i, gmin, f)) gmin = (int)i;
+  t.global_min = gmin;
+  t.pairs.assign(nMax, {0, 0});
+  t.mtrip.assign(nMax, {0, 0});
+  if (nMax == 0) return t;
+  // constructPersistencePairs
+  std::vector<int> ptr(nMax);
+  std::iota(ptr.begin(), ptr.end(), 0);
+  const int globalMax = nMax - 1;
+  bool changed = true;
+  while (changed) {
+    changed = false;
+    std::vector<int> largest(nMax, nS);
+    for (int i = 0; i < nS; i++)
+      for (int m : trip[i])
+        if (m != globalMax) largest[m] = std::min(i, largest[m]);
+    std::vector<int> active;
+    for (int m = 0; m < nMax; m++) {
+      int ls = largest[m];
+      if (ls < nS && trip[ls].front() == m) {
+        changed = true;
+        t.pairs[m] = {t.saddles2[ls], t.maximum[m]};
+        int lmax = trip[ls].back();
+        ptr[m] = lmax;
+        t.mtrip[m] = {ls, lmax};
+        active.push_back(m);
+      }
+    }
+    while (!active.empty()) {
+      std::vector<int> keep;
+      for (int v : active) {
+        ptr[v] = ptr[ptr[v]];
+        if (ptr[v] != ptr[ptr[v]]) keep.push_back(v);
+      }
+      active.swap(keep);
+    }
+    for (auto& tr : trip) {
+      for (int& r : tr) r = ptr[r];
+      std::sort(tr.begin(), tr.end());
+      tr.erase(std::unique(tr.begin(), tr.end()), tr.end());
+      if (tr.size() == 1) tr.clear();
+    }
+    t.rounds++;
+  }
+  t.pairs[nMax - 1] = {gmin, t.maximum[nMax - 1]};
+  t.mtrip[nMax - 1] = {gmin, nMax - 1};
+  // constructMergeTree: follow larger saddles to the parent
+  t.mtrip[nMax - 1].second = nMax - 1;
+  bool same = false;
+  while (!same) {
+    same = true;
+    for (int i = 0; i < nMax; i++) {
+      auto next = t.mtrip[t.mtrip[i].second];
+      if (next.second != t.mtrip[i].second && next.first < t.mtrip[i].first) {
+        t.mtrip[i].second = next.second;
+        same = false;
+      }
+    }
+  }
+  // branches: (order, id) pairs sorted descending; order = rank by (value, id) (D14)
+  std::vector<std::vector<std::pair<int, int>>> br(nMax);
+  std::vector<int> order_of;  // ranks only of the vertices used
+  auto order = [&](int v) {   // rank of v among all vertices by (value, id)
+    int64_t r = 0;
+    for (int64_t u = 0; u < g.N; u++) r += isless((int)u, v, f);
+    return (int)r;
+  };
+  for (int b = 0; b < nMax; b++) {
+    int mx = t.maximum[b];
+    br[b].push_back({order(mx), mx});
+    int parent = t.mtrip[b].second;
+    if (parent != b && !t.saddles2.empty()) {
+      int sad = t.saddles2[t.mtrip[b].first];
+      int os = order(sad);
+      br[b].push_back({os, sad});
+      br[parent].push_back({os, sad});
+    } else {
+      br[b].push_back({-1, t.mtrip[b].first});
+    }
+  }
+  t.branch.resize(nMax);
+  for (int b = 0; b < nMax; b++) {
+    std::sort(br[b].begin(), br[b].end(), std::greater<>());
+    for (auto& pr : br[b]) t.branch[b].push_back(pr.second);
+  }
+  return t;
+}
+
 }  // namespace
 
 extern "C" {
+
+// Split-tree check of one field (compute_MergeT): per maximum in rank order
+// (ascending (value, id)): its vertex, the persistence pair's saddle vertex
+// (global minimum for the global maximum), the merge-tree parent's vertex,
+// and the branch's vertices (descending order) flattened with offsets.
+// Returns the number of maxima, or -(needed) if a buffer is too small.
+int oracle_split_tree(const double* f, int W, int H, int D, int* max_v, int* pair_s,
+                      int* parent_v, int* branch_off, int* branch_v, int cap_max, int cap_branch,
+                      int* rounds) {
+  Grid g{W, H, D, (int64_t)W * H * D};
+  SplitTree t = split_tree(g, f);
+  const int nMax = (int)t.maximum.size();
+  int total = 0;
+  for (auto& b : t.branch) total += (int)b.size();
+  if (nMax > cap_max || total > cap_branch) return -std::max(nMax, total);
+  int k = 0;
+  for (int i = 0; i < nMax; i++) {
+    max_v[i] = t.maximum[i];
+    pair_s[i] = t.pairs[i].first;
+    parent_v[i] = t.maximum[t.mtrip[i].second];
+    branch_off[i] = k;
+    for (int v : t.branch[i]) branch_v[k++] = v;
+  }
+  branch_off[nMax] = k;
+  *rounds = t.rounds;
+  return nMax;
+}
+
 
 // Writes the reference LUT layout (24 800 int32 for the 27 keys of
 // src/ExtreM_cuda.cu:84-110) into out; returns the number of entries, or the

@@ -125,3 +125,26 @@ def cost(orig, before, after, counts, xi):
     st = {"changed": int(out[0]), "snapped": int(out[1]), "edit_ratio": float(out[2]),
           "psnr": float(out[3]), "fixed_psnr": float(out[4])}
     return st, bm[:(3 * o.size + 7) // 8], ed[:st["snapped"]]
+
+
+def split_tree(field):
+    """compute_MergeT restated for one field: dict with per-maximum (rank order)
+    'max', 'pair_saddle', 'parent' vertex ids, 'branch' vertex lists, 'rounds'."""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    L = lib()
+    if not hasattr(L.oracle_split_tree, "_set"):
+        vp, i32 = ctypes.c_void_p, ctypes.c_int
+        L.oracle_split_tree.argtypes = [vp, i32, i32, i32, vp, vp, vp, vp, vp, i32, i32, vp]
+        L.oracle_split_tree._set = True
+    cap = f.size + 1
+    mx, ps, pa = (np.zeros(cap, np.int32) for _ in range(3))
+    off = np.zeros(cap + 1, np.int32)
+    bv = np.zeros(3 * cap, np.int32)
+    rounds = ctypes.c_int(0)
+    n = L.oracle_split_tree(_p(f), *_dims(f), _p(mx), _p(ps), _p(pa), _p(off), _p(bv), cap,
+                            bv.size, ctypes.byref(rounds))
+    if n < 0:
+        raise ValueError("buffer too small")
+    return {"max": mx[:n].copy(), "pair_saddle": ps[:n].copy(), "parent": pa[:n].copy(),
+            "branch": [bv[off[i]:off[i + 1]].tolist() for i in range(n)],
+            "rounds": rounds.value}
