@@ -40,6 +40,10 @@
 //       maximum is not the original global maximum).
 //   D13 fix_wrong_index_saddle is skipped when either index is NONE (kept as
 //       a guard; D12 no longer produces such pairs).
+//   D14 the split-tree check ranks value ties by vertex id
+//       (getSortedPositions, src/ExtreM.cpp:1840-1860, leaves them to std::sort).
+//   D15 the split tree of each field uses that field's own types, labels and
+//       value order (main.cpp:459-461 sorts the original's lists by decp).
 //   LUT extension: fields with an axis of length 1 give boundary classes the
 //   reference has no LUT segment for; they use the same link-component rule.
 //   Iteration caps turn the reference's unbounded loops into an error code.
