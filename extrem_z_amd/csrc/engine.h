@@ -197,9 +197,9 @@ __global__ void k_wn_solve(St s, int2* bufA, int2* bufB, int limit);
 __global__ void k_fixpath(St s);
 __global__ void k_wrong_index(St s, int side, int phase, int do_fix);
 __global__ void k_wrong_saddle(St s, int side, int do_fix);
-__global__ void k_lists_count(St s, int* cnt, int nb);
+__global__ void k_lists_count(St s, int* cnt, int nb, int tiled);
 __global__ void k_lists_scan(St s, int* cnt, int nb);
-__global__ void k_lists_write(St s, const int* cnt, int nb);
+__global__ void k_lists_write(St s, const int* cnt, int nb, int tiled);
 __global__ void k_presnap(St s);
 __global__ void k_argext(const double* f, int n, int* slot_min, int* slot_max);
 __global__ void k_count_edited(const uint8_t* cnt, long long n, int* ctr);
@@ -215,9 +215,10 @@ __global__ void k_scan_add(int* out, long long n, const int* offs);
 __global__ void k_tree_triplets(Geo g, const double* f, const int* S2, int n2, const int* ds,
                                 const int* ext_id, const long long* mkey, const int* MX, int* T,
                                 int* cnt);
-__global__ void k_tree_best(const int* T, const int* cnt, int n2, const long long* skey, int gmax,
-                            int pass, long long* bestkey, int* bestpos);
-__global__ void k_tree_pair(const int* T, const int* cnt, int nmax, const int* bestpos,
+__global__ void k_tree_best(const int* T, const int* cnt, const int* S2, int n2,
+                            const long long* skey, int gmax, int pass, long long* bestkey,
+                            long long* bestpos);
+__global__ void k_tree_pair(const int* T, const int* cnt, int nmax, const long long* bestpos,
                             int* pair_pos, int* parent, int* ptr, int* changed);
 __global__ void k_tree_compress(int n, int* ptr);
 __global__ void k_tree_relabel(int* T, int* cnt, int n2, const int* ptr, const long long* mkey,

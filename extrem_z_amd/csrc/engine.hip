@@ -279,6 +279,7 @@ class Run {
   int h(int idx) const { return c_->h_ctr[idx]; }
 
   void allocate(long long N);
+  void list_compaction();
   void slot_pass(int dec, const double* f, const int* L2, const int* L1, const uint16_t* flow,
                  int* F2, int* La2, int* F1, int* La1, int* smax, int* smin);
   void apply();
@@ -290,6 +291,20 @@ class Run {
   void s_loops(int side, int do_fix);
   void saddle_loops(int side, int do_fix);
 };
+
+// The CP lists S2 / S1 / MX / MN of the types in s_.otype, in the column
+// order of k_lists_count (32 x 32 columns for fields at least that wide).
+void Run::list_compaction() {
+  const Geo& g = s_.g;
+  const int tiled = g.W >= 32 && g.H >= 32;
+  const long long K =
+      tiled ? (long long)((g.W + 31) / 32) * ((g.H + 31) / 32) * 1024LL * g.D : (long long)g.N;
+  const int nb = (int)((K + 255) / 256);
+  c_->lcnt.ensure(4LL * nb);
+  k_lists_count<<<nb, 256, 0, st()>>>(s_, c_->lcnt.p, nb, tiled);
+  k_lists_scan<<<1, 1024, 0, st()>>>(s_, c_->lcnt.p, nb);
+  k_lists_write<<<nb, 256, 0, st()>>>(s_, c_->lcnt.p, nb, tiled);
+}
 
 void Run::allocate(long long N) {
   extremz_ctx* c = c_;
@@ -610,11 +625,7 @@ void Run::split_tree(const double* d_f, int W, int H, int D, TreeOut& out) {
   hip_check(hipMemsetAsync(s_.ctr, 0, sizeof(int) * C_COUNT_, q), "memset");
   k_classify<<<row_blocks(), dim3(64, 4, 1), 0, q>>>(s_.g, s_.lut, d_f, s_.otype, s_.olow);
   label_pass(d_f, c->oDS.p, c->oAS.p, nullptr);
-  const int nb = (int)((N + 255) / 256);
-  c->lcnt.ensure(4LL * nb);
-  k_lists_count<<<nb, 256, 0, q>>>(s_, c->lcnt.p, nb);
-  k_lists_scan<<<1, 1024, 0, q>>>(s_, c->lcnt.p, nb);
-  k_lists_write<<<nb, 256, 0, q>>>(s_, c->lcnt.p, nb);
+  list_compaction();
   set_ctr2(C_GMIN, C_GMAX, kNone);
   k_argext<<<blocks_for(N), 256, 0, q>>>(d_f, (int)N, s_.ctr + C_GMIN, s_.ctr + C_GMAX);
   pull();
@@ -625,14 +636,15 @@ void Run::split_tree(const double* d_f, int W, int H, int D, TreeOut& out) {
   s_.nmin = 0;
   k_fill_i32<<<blocks_for(N), 256, 0, q>>>(s_.ext_id, -1, N);
   k_ext_index<<<blocks_for(nmax), 256, 0, q>>>(s_);
-  DevBuf<long long> mkey, skey, bestkey;
-  DevBuf<int> T, cnt, pair_pos, parent, ptr, bestpos;
+  DevBuf<long long> mkey, skey, bestkey, bestpos;
+  DevBuf<int> T, cnt, pair_pos, parent, ptr;
   mkey.ensure(nmax);
   skey.ensure((size_t)n2 + 1);
   bestkey.ensure(nmax);
   T.ensure((size_t)n2 * kNbr + 1);
   cnt.ensure((size_t)n2 + 1);
-  for (DevBuf<int>* b : {&pair_pos, &parent, &ptr, &bestpos}) b->ensure(nmax);
+  for (DevBuf<int>* b : {&pair_pos, &parent, &ptr}) b->ensure(nmax);
+  bestpos.ensure(nmax);
   k_list_keys<<<blocks_for(nmax), 256, 0, q>>>(s_.MX, nmax, d_f, mkey.p);
   if (n2) k_list_keys<<<blocks_for(n2), 256, 0, q>>>(s_.S2, n2, d_f, skey.p);
   std::vector<int> hMX(nmax), hS2(n2);
@@ -661,13 +673,14 @@ void Run::split_tree(const double* d_f, int W, int H, int D, TreeOut& out) {
   // constructPersistencePairs: rounds until no maximum is paired
   for (int round = 0;; round++) {
     if (round > nmax + 1) throw HipError("split tree pairing did not finish");
-    k_fill_pair<<<blocks_for(nmax), 256, 0, q>>>(bestkey.p, (long long)0x8000000000000000ULL,
-                                                 bestpos.p, -1, nmax);
+    k_fill_i64<<<blocks_for(nmax), 256, 0, q>>>(bestkey.p, (long long)0x8000000000000000ULL,
+                                                nmax);
+    k_fill_i64<<<blocks_for(nmax), 256, 0, q>>>(bestpos.p, -1, nmax);
     if (n2) {
-      k_tree_best<<<blocks_for(n2), 256, 0, q>>>(T.p, cnt.p, n2, skey.p, gmax, 0, bestkey.p,
-                                                 bestpos.p);
-      k_tree_best<<<blocks_for(n2), 256, 0, q>>>(T.p, cnt.p, n2, skey.p, gmax, 1, bestkey.p,
-                                                 bestpos.p);
+      k_tree_best<<<blocks_for(n2), 256, 0, q>>>(T.p, cnt.p, s_.S2, n2, skey.p, gmax, 0,
+                                                 bestkey.p, bestpos.p);
+      k_tree_best<<<blocks_for(n2), 256, 0, q>>>(T.p, cnt.p, s_.S2, n2, skey.p, gmax, 1,
+                                                 bestkey.p, bestpos.p);
     }
     set_ctr(C_CHANGED, 0);
     k_tree_pair<<<blocks_for(nmax), 256, 0, q>>>(T.p, cnt.p, nmax, bestpos.p, pair_pos.p,
@@ -858,13 +871,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   tend();
   label_pass(d_orig, s_.oDS, s_.oAS, nullptr);
   // main.cpp:100-119: CP lists, triplets, largest/smallest saddle maps of orig
-  {
-    const int nb = (int)((N + 255) / 256);
-    c->lcnt.ensure(4LL * nb);
-    k_lists_count<<<nb, 256, 0, q>>>(s_, c->lcnt.p, nb);
-    k_lists_scan<<<1, 1024, 0, q>>>(s_, c->lcnt.p, nb);
-    k_lists_write<<<nb, 256, 0, q>>>(s_, c->lcnt.p, nb);
-  }
+  list_compaction();
   pull();
   s_.n2 = h(C_N2);
   s_.n1 = h(C_N1);

@@ -503,9 +503,12 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
   __shared__ double ring[3][kPN];
   __shared__ int sup[kLTN];
   __shared__ int sdn[kLTN];
-  const int ntx = (g.W + kLTX - 1) / kLTX, nty = (g.H + kLTY - 1) / kLTY;
+  const int ntx = (g.W + kLTX - 1) / kLTX, ntz = (g.D + kLTZ - 1) / kLTZ;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int x0 = (L % ntx) * kLTX, y0 = ((L / ntx) % nty) * kLTY, z0 = (L / (ntx * nty)) * kLTZ;
+  // z fastest: the consecutive tiles of an XCD stack along z and share their
+  // halo planes in that XCD's L2
+  const int col = L / ntz;
+  const int x0 = (col % ntx) * kLTX, y0 = (col / ntx) * kLTY, z0 = (L % ntz) * kLTZ;
   const int lx = threadIdx.x, ly = threadIdx.y, tid = lx + 64 * ly;
   const bool ordered = g.W >= 3;
   double pre[kPPer];
@@ -953,11 +956,10 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const doubl
 }
 
 // Slot gather for the unchanged saddles of a decp pass: for slot index j
-// (maxima, then minima) the best (key, list position) among the extremum's
+// (maxima, then minima) the best (key, vertex id) among the extremum's
 // original adjacent saddles, over the keys pv2 / pv1 that the saddle pass
-// left at the list positions (lists are in vertex order, so position order
-// is id order and (key, position) is the (value, id) order).  pkey = the
-// best key (the empty key if none), pid = its saddle vertex.  Slots with at
+// left at the list positions ((key, id) is the (value, id) order).  pkey =
+// the best key (the empty key if none), pid = its saddle vertex.  Slots with at
 // most kPullShort adjacencies take a group of kPullGroup lanes; longer ones
 // are cut into chunks of kPullChunk (k_long_slots), one block per chunk
 // (k_slot_pull_long), joined by k_slot_pull_join.
@@ -972,22 +974,27 @@ __device__ __forceinline__ bool pull_better(bool mx, long long ka, int pa, long 
 __device__ __forceinline__ void pull_range(const St& s, bool mx, int k0, int k1, int step,
                                            long long& bk, int& bp) {
   const long long* pv = mx ? s.pv2 : s.pv1;
+  const int* S = mx ? s.S2 : s.S1;
   int k = k0;
   for (; k + 3 * step < k1; k += 4 * step) {
-    int p[4];
+    int p[4], id[4];
     long long q[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) p[u] = s.csr[k + u * step];
 #pragma unroll
-    for (int u = 0; u < 4; u++) q[u] = pv[p[u]];
+    for (int u = 0; u < 4; u++) {
+      q[u] = pv[p[u]];
+      id[u] = S[p[u]];
+    }
 #pragma unroll
     for (int u = 0; u < 4; u++)
-      if (bp < 0 || pull_better(mx, q[u], p[u], bk, bp)) { bk = q[u]; bp = p[u]; }
+      if (bp < 0 || pull_better(mx, q[u], id[u], bk, bp)) { bk = q[u]; bp = id[u]; }
   }
   for (; k < k1; k += step) {
     const int p = s.csr[k];
     const long long q = pv[p];
-    if (bp < 0 || pull_better(mx, q, p, bk, bp)) { bk = q; bp = p; }
+    const int id = S[p];
+    if (bp < 0 || pull_better(mx, q, id, bk, bp)) { bk = q; bp = id; }
   }
 }
 
@@ -1004,7 +1011,7 @@ __device__ __forceinline__ void pull_store(const St& s, int j, bool mx, long lon
   const long long empty = mx ? (long long)0x8000000000000000ULL : 0x7fffffffffffffffLL;
   const bool none = bp < 0 || bk == empty;
   pkey[j] = none ? empty : bk;
-  pid[j] = none ? kNone : (mx ? s.S2 : s.S1)[bp];
+  pid[j] = none ? kNone : bp;
 }
 
 __global__ void __launch_bounds__(256) k_slot_pull(St s, long long* pkey, int* pid) {
@@ -1523,22 +1530,43 @@ __global__ void __launch_bounds__(256) k_wrong_saddle(St s, int side, int do_fix
 }
 
 // --------------------------------------------------------------- setup ----
-// The saddle / extremum lists by original type, each in increasing vertex
-// order (an ordered compaction: per-block counts, one scan, ordered writes),
-// so list neighbours are grid neighbours and the per-extremum slot arrays
-// indexed by list position are touched in spatial order.
+// The saddle / extremum lists by original type, in an ordered compaction
+// (per-block counts, one scan, ordered writes) over a traversal of the grid
+// in columns of kListTile x kListTile voxels, each column marched along z
+// (for fields at least that wide; plain vertex order otherwise).  List
+// neighbours are then grid neighbours within a few voxels in every axis, so
+// the per-saddle passes reuse the neighbour rows of a column in L2 instead
+// of streaming whole planes.  No pass depends on the list order: every
+// order-dependent choice compares vertex ids.
+constexpr int kListTile = 32;
+
+__device__ __forceinline__ int list_vertex(const Geo& g, long long k, int tiled) {
+  if (!tiled) return k < g.N ? (int)k : -1;
+  const int ntx = (g.W + kListTile - 1) / kListTile;
+  const long long per_col = (long long)kListTile * kListTile * g.D;
+  const long long col = k / per_col;
+  long long r = k - col * per_col;
+  const int z = (int)(r / (kListTile * kListTile));
+  r -= (long long)z * kListTile * kListTile;
+  const int x = (int)(col % ntx) * kListTile + (int)(r % kListTile);
+  const int y = (int)(col / ntx) * kListTile + (int)(r / kListTile);
+  if (x >= g.W || y >= g.H) return -1;
+  return x + g.W * (y + g.H * z);
+}
+
 __device__ __forceinline__ unsigned list_flags(const St& s, int v) {
-  if (v >= s.g.N) return 0u;
+  if (v < 0) return 0u;
   int t = s.otype[v];
   return (unsigned)(t == 2 || t == 3) | (unsigned)(t == 1 || t == 3) << 1 |
          (unsigned)(t == 4) << 2 | (unsigned)(t == 0) << 3;
 }
 
 // cnt[c * nb + block] = number of category-c vertices in the block (256
-// vertices per block; categories 2-saddle, 1-saddle, maximum, minimum).
-__global__ void __launch_bounds__(256) k_lists_count(St s, int* cnt, int nb) {
+// traversal positions per block; categories 2-saddle, 1-saddle, maximum,
+// minimum).
+__global__ void __launch_bounds__(256) k_lists_count(St s, int* cnt, int nb, int tiled) {
   __shared__ int w[4][4];
-  const int v = blockIdx.x * 256 + threadIdx.x;
+  const int v = list_vertex(s.g, blockIdx.x * 256LL + threadIdx.x, tiled);
   const unsigned fl = list_flags(s, v);
   const int wave = threadIdx.x >> 6;
 #pragma unroll
@@ -1584,9 +1612,9 @@ __global__ void __launch_bounds__(1024) k_lists_scan(St s, int* cnt, int nb) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb) {
+__global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb, int tiled) {
   __shared__ int w[4][4];
-  const int v = blockIdx.x * 256 + threadIdx.x;
+  const int v = list_vertex(s.g, blockIdx.x * 256LL + threadIdx.x, tiled);
   const unsigned fl = list_flags(s, v);
   const int wave = threadIdx.x >> 6, lane = __lane_id();
   unsigned long long m[4];
@@ -1725,7 +1753,7 @@ namespace ez {
 
 // ------------------------------------------------- split tree (check) ----
 // compute_MergeT (src/ExtreM.cpp:2930-2952) on the GPU.  Maxima are named by
-// their position p in MX (vertex order); mkey[p] is the ordered key of their
+// their position p in MX; mkey[p] is the ordered key of their
 // value, so (mkey, MX) is the (value, id) order of the reference's ranks.
 
 __device__ __forceinline__ bool max_less(const long long* mkey, const int* MX, int a, int b) {
@@ -1770,29 +1798,32 @@ __global__ void __launch_bounds__(256) k_tree_triplets(Geo g, const double* f, c
 }
 
 // constructPersistencePairs, largest saddle per maximum (src/ExtreM.cpp:2146-2166):
-// pass 0 raises the value key, pass 1 the list position among saddles with
-// that key (positions are in vertex order, so the highest position is the
-// highest id).  The global maximum (position gmax) takes no part.
-__global__ void k_tree_best(const int* T, const int* cnt, int n2, const long long* skey, int gmax,
-                            int pass, long long* bestkey, int* bestpos) {
+// pass 0 raises the value key, pass 1, among the saddles with that key, the
+// vertex id (high 32 bits) carrying the list position (low 32 bits).  The
+// global maximum (position gmax) takes no part.
+__global__ void k_tree_best(const int* T, const int* cnt, const int* S2, int n2,
+                            const long long* skey, int gmax, int pass, long long* bestkey,
+                            long long* bestpos) {
   GRID_STRIDE(i, n2) {
     const int m = cnt[i];
     for (int k = 0; k < m; k++) {
       const int p = T[i * kNbr + k];
       if (p == gmax) continue;
       if (pass == 0) atomicMax(&bestkey[p], skey[i]);
-      else if (skey[i] == bestkey[p]) atomicMax(&bestpos[p], i);
+      else if (skey[i] == bestkey[p])
+        atomicMax(&bestpos[p], ((long long)S2[i] << 32) | (unsigned)i);
     }
   }
 }
 
 // A maximum that is the lowest maximum of its largest saddle is paired with
 // it and points to the saddle's highest maximum (src/ExtreM.cpp:2170-2195).
-__global__ void k_tree_pair(const int* T, const int* cnt, int nmax, const int* bestpos,
+__global__ void k_tree_pair(const int* T, const int* cnt, int nmax, const long long* bestpos,
                             int* pair_pos, int* parent, int* ptr, int* changed) {
   GRID_STRIDE(p, nmax) {
-    const int i = bestpos[p];
-    if (i < 0 || T[i * kNbr] != p) continue;
+    if (bestpos[p] < 0) continue;
+    const int i = (int)(bestpos[p] & 0xffffffffLL);
+    if (T[i * kNbr] != p) continue;
     const int hi = T[i * kNbr + cnt[i] - 1];
     pair_pos[p] = i;
     parent[p] = hi;
