@@ -184,7 +184,7 @@ __global__ void k_slot_merge(St s, long long* kmax, long long* kmin, int* slot_m
                              int* slot_min, const long long* pkey, const int* pid,
                              int use_pull);
 __global__ void k_scan_single(int* a, int n);
-__global__ void k_fill_pair(long long* a, long long va, int* b, int vb, int n);
+__global__ void k_slot_init(St s, long long* kmax, int* smax, long long* kmin, int* smin);
 __global__ void k_list_fill64(const int* list, int n, long long* arr, long long val);
 __global__ void k_ext_reduce(const int* inmax, int nmax, const int* inmin, int nmin,
                              const double* f, int* outmax, int* outmin);

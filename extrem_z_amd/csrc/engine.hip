@@ -337,11 +337,8 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
   long long* kmax = c_->kmax.p;
   long long* kmin = c_->kmin.p;
   tbegin(EXTREMZ_K_SLOT_GATHER);
-  k_fill_pair<<<blocks_for(s_.nmax), 256, 0, st()>>>(kmax, (long long)0x8000000000000000ULL, smax,
-                                                     kNone, s_.nmax);
-  k_fill_pair<<<blocks_for(s_.nmin), 256, 0, st()>>>(kmin, 0x7fffffffffffffffLL, smin, 0x7fffffff,
-                                                     s_.nmin);
-  set_ctr2(C_ACT0, C_PAIRS, 0);
+  k_slot_init<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(s_, kmax, smax, kmin,
+                                                                          smin);
   tend();
   const int nb2 = (s_.n2 + 255) / 256, nb1 = (s_.n1 + 255) / 256;
   const int ns = s_.nmax + s_.nmin;

@@ -1206,10 +1206,20 @@ __global__ void k_ext_index(St s) {
   GRID_STRIDE(i, s.nmin) s.ext_id[s.MN[i]] = 2 * i + 1;
 }
 
-__global__ void k_fill_pair(long long* a, long long va, int* b, int vb, int n) {
-  GRID_STRIDE(i, n) {
-    a[i] = va;
-    b[i] = vb;
+// Empty slot keys and slots for a saddle pass, and its two list counters
+// (read by the launches before it, appended to by the ones after).
+__global__ void k_slot_init(St s, long long* kmax, int* smax, long long* kmin, int* smin) {
+  GRID_STRIDE(i, s.nmax) {
+    kmax[i] = (long long)0x8000000000000000ULL;
+    smax[i] = kNone;
+  }
+  GRID_STRIDE(i, s.nmin) {
+    kmin[i] = 0x7fffffffffffffffLL;
+    smin[i] = 0x7fffffff;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s.ctr[C_ACT0] = 0;
+    s.ctr[C_PAIRS] = 0;
   }
 }
 
