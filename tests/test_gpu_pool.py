@@ -152,3 +152,28 @@ def test_invalid_job_rejected_and_concurrent_waiters(cases):
         for t in ths:
             t.join(120)
         assert all(results[k] == (0, fixed.tobytes()) for k in range(4))
+
+
+def test_failing_job_does_not_disturb_the_batch(cases):
+    """A job whose input is not finite fails with -22 inside its lane; the
+    other jobs of the batch, and later jobs on the same lanes, are unaffected."""
+    bad_o, bad_d, bad_xi = make_field((20, 20, 20), 42, "noise", 1e-2)
+    bad_d = bad_d.copy()
+    bad_d[3, 4, 5] = np.nan
+    with ez().Pool(0, lanes=2) as pool:
+        jobs, keep = [], []
+        for k, (orig, decp, xi, _, _) in enumerate(cases[:4]):
+            if k == 2:
+                j, kp = pool.host_job(bad_o, bad_d.copy(), bad_xi)
+            else:
+                j, kp = pool.host_job(orig, decp.copy(), xi)
+            jobs.append(j)
+            keep.append(kp)
+        rc, arr = pool.batch(jobs)
+        assert rc == -22
+        assert [arr[k].status for k in range(4)] == [0, 0, -22, 0]
+        for k in (0, 1, 3):
+            assert keep[k][1].tobytes() == cases[k][3].tobytes()
+        j, kp = pool.host_job(cases[4][0], cases[4][1].copy(), cases[4][2])
+        t = pool.submit(j, keep=kp)
+        assert pool.wait(t).status == 0 and kp[1].tobytes() == cases[4][3].tobytes()
