@@ -47,7 +47,7 @@ class ExtremzError(RuntimeError):
 
 class Opts(ctypes.Structure):
     _fields_ = [("max_c_iters", ctypes.c_int), ("max_outer", ctypes.c_int),
-                ("timing", ctypes.c_int), ("reserved", ctypes.c_int)]
+                ("timing", ctypes.c_int), ("fixed_step_q", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -185,11 +185,12 @@ def _timing_mask(timing):
     return sum(1 << KERNEL_FAMILIES.index(n) for n in timing)
 
 
-def correct(orig, decp, xi, max_c_iters=0, max_outer=0, timing=False):
+def correct(orig, decp, xi, max_c_iters=0, max_outer=0, timing=False, fixed_step_q=0):
     """Corrects decp against orig within xi.  Returns (fixed, counts, stats).
 
     orig may be float32 or float64 (promoted exactly); inputs are not modified.
     timing selects the kernel families timed with HIP events (see _timing_mask).
+    fixed_step_q > 0 selects the fixed-step rule set (step -xi/q, threshold q).
     """
     o = np.ascontiguousarray(orig)
     if o.dtype == np.float32:
@@ -202,7 +203,7 @@ def correct(orig, decp, xi, max_c_iters=0, max_outer=0, timing=False):
         raise ValueError("orig and decp shapes differ")
     counts = np.zeros(o.shape, np.uint8)
     st = Stats()
-    opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), 0)
+    opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), int(fixed_step_q))
     rc = lib().extremz_correct(_p(o), dt, _p(d), *_dims(o), float(xi), ctypes.byref(opts),
                                ctypes.byref(st), _p(counts))
     _check(rc)
@@ -218,11 +219,11 @@ class Context:
         self.handle = h
 
     def correct_device(self, d_orig, d_decp, shape, xi, d_counts=0, max_c_iters=0, max_outer=0,
-                       timing=False):
+                       timing=False, fixed_step_q=0):
         """Device pointers (ints): d_orig float64, d_decp float64 edited in place."""
         W, H, D = shape
         st = Stats()
-        opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), 0)
+        opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), int(fixed_step_q))
         rc = lib().extremz_correct_device(self.handle, ctypes.c_void_p(d_orig),
                                           ctypes.c_void_p(d_decp), W, H, D, float(xi),
                                           ctypes.byref(opts), ctypes.byref(st),
@@ -248,9 +249,10 @@ class Pool:
     serve jobs as they arrive.  Every job's result equals the single-field
     correct() result byte for byte."""
 
-    def __init__(self, device=0, lanes=8, hbm_budget=0, max_c_iters=0, max_outer=0, timing=False):
+    def __init__(self, device=0, lanes=8, hbm_budget=0, max_c_iters=0, max_outer=0, timing=False,
+                 fixed_step_q=0):
         h = ctypes.c_void_p()
-        opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), 0)
+        opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), int(fixed_step_q))
         _check(lib().extremz_pool_create(int(device), int(lanes), int(hbm_budget),
                                          ctypes.byref(opts), ctypes.byref(h)))
         self.handle = h

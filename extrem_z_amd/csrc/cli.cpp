@@ -2,7 +2,7 @@
 //
 //   extremz_cli <path,W,H,D> <er> <compressor_id> [--decp decp.bin]
 //               [--orig-f32] [--out fixed.bin] [--counts counts.bin]
-//               [--cost prefix] [--cmp-bytes n]
+//               [--cost prefix] [--cmp-bytes n] [--fixed-step q]
 //
 // Same positional arguments as main.cpp:13-37: a raw float64 field (getdata,
 // src/ExtreM.cpp:95-118; --orig-f32 for float32), the relative bound er
@@ -15,7 +15,8 @@
 // <prefix>index.bin / <prefix>edits.bin and their zstd frames (.zst), and the
 // statistics block of the reference's result file is appended to
 // <prefix>stats.txt; --cmp-bytes is the lossy compressor's output size
-// (SZ3's cmpSize in the reference).
+// (SZ3's cmpSize in the reference).  --fixed-step q selects the GPU fork's
+// fixed-step rule set (its edit_type 0 with q, src/ExtreM_cuda.cu:4958-4963).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -74,6 +75,7 @@ int main(int argc, char** argv) {
   std::string compressor_id = argv[3];
   std::string decp_path, out_path = "fixed.bin", counts_path, cost_prefix;
   long long cmp_bytes = 0;
+  int fixed_q = 0;
   bool f32 = false;
   for (int i = 4; i < argc; i++) {
     std::string a = argv[i];
@@ -83,6 +85,7 @@ int main(int argc, char** argv) {
     else if (a == "--orig-f32") f32 = true;
     else if (a == "--cost" && i + 1 < argc) cost_prefix = argv[++i];
     else if (a == "--cmp-bytes" && i + 1 < argc) cmp_bytes = std::atoll(argv[++i]);
+    else if (a == "--fixed-step" && i + 1 < argc) fixed_q = std::atoi(argv[++i]);
     else {
       std::fprintf(stderr, "unknown argument %s\n", a.c_str());
       return 2;
@@ -119,7 +122,7 @@ int main(int argc, char** argv) {
   std::vector<double> before = decp;
   std::vector<uint8_t> counts(N);
   extremz_stats st;
-  extremz_opts opts = {0, 0, 0, 0};
+  extremz_opts opts = {0, 0, 0, fixed_q};
   int rc = extremz_correct(orig.data(), EXTREMZ_DTYPE_F64, decp.data(), W, H, D, xi, &opts, &st,
                            counts.data());
   if (rc != 0) {

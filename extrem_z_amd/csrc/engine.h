@@ -61,6 +61,8 @@ struct Geo {
 struct St {
   Geo g;
   double xi;
+  int step_q;           // 0: midpoint / snap rules; q > 0: fixed step `step`, threshold q
+  double step;          // -xi / q
   long long sent;       // ordered key of the -4*xi sentinel
   const uint8_t* lut;
   const double* orig;
@@ -115,6 +117,7 @@ void free_ctx(extremz_ctx* c);
 size_t ctx_bytes(extremz_ctx* c);            // device bytes held by the work buffers
 void ctx_release(extremz_ctx* c);            // frees the work buffers (kept: stream, LUT)
 bool valid_shape(int W, int H, int D);
+bool valid_opts(const extremz_opts* o);      // null, or a fixed-step q in [0, 254]
 std::mutex& cache_mutex();                   // guards the per-device contexts of the host entry points
 extremz_ctx* cached_ctx(int device);         // the shared context of a device (caller holds the mutex)
 int ctx_device(extremz_ctx* c);

@@ -203,6 +203,7 @@ class Run {
     max_outer_ = (o && o->max_outer > 0) ? o->max_outer : 10000;
     c_->timing = o ? o->timing : 0;
     c_->ev_used = 0;
+    step_q_ = o ? o->fixed_step_q : 0;
   }
 
   int correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
@@ -217,6 +218,7 @@ class Run {
   extremz_ctx* c_;
   St s_{};
   int max_c_, max_outer_;
+  int step_q_ = 0;
   int epoch_ = 0;
   bool clean_ = false;
   int front_ = C_FRONT_A;  // counter index of the live frontier
@@ -792,6 +794,8 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   allocate(N);
   extremz_ctx* c = c_;
   s_.xi = xi;
+  s_.step_q = step_q_;
+  s_.step = step_q_ ? -xi / step_q_ : 0.0;
   {
     double sd = -4.0 * xi;
     long long b;
@@ -1037,6 +1041,10 @@ static void fill_stats(extremz_ctx* c, const Run& r, int status, double wall_ms,
   }
 }
 
+bool valid_opts(const extremz_opts* o) {
+  return !o || (o->fixed_step_q >= 0 && o->fixed_step_q <= 254);  // counts are uint8
+}
+
 bool valid_shape(int W, int H, int D) {
   if (W <= 0 || H <= 0 || D <= 0) return false;
   long long N = (long long)W * H * D;
@@ -1144,7 +1152,8 @@ int extremz_ctx_destroy(extremz_ctx* ctx) {
 int extremz_correct_device(extremz_ctx* ctx, const double* d_orig, double* d_decp, int W, int H,
                            int D, double xi, const extremz_opts* opts, extremz_stats* stats,
                            uint8_t* d_counts) {
-  if (!ctx || !d_orig || !d_decp || !ez::valid_shape(W, H, D) || !(xi > 0) || !std::isfinite(xi))
+  if (!ctx || !d_orig || !d_decp || !ez::valid_shape(W, H, D) || !(xi > 0) || !std::isfinite(xi) ||
+      !ez::valid_opts(opts))
     return EXTREMZ_EINVAL;
   return ez::run_correct(ctx, d_orig, d_decp, W, H, D, xi, opts, stats, d_counts);
 }
@@ -1152,7 +1161,8 @@ int extremz_correct_device(extremz_ctx* ctx, const double* d_orig, double* d_dec
 int extremz_correct(const void* orig, int orig_dtype, double* decp, int W, int H, int D, double xi,
                     const extremz_opts* opts, extremz_stats* stats, uint8_t* counts) {
   if (!orig || !decp || !ez::valid_shape(W, H, D) || !(xi > 0) || !std::isfinite(xi) ||
-      (orig_dtype != EXTREMZ_DTYPE_F32 && orig_dtype != EXTREMZ_DTYPE_F64))
+      (orig_dtype != EXTREMZ_DTYPE_F32 && orig_dtype != EXTREMZ_DTYPE_F64) ||
+      !ez::valid_opts(opts))
     return EXTREMZ_EINVAL;
   std::lock_guard<std::mutex> lk(ez::g_cache_mu);
   return ez::guarded([&] {

@@ -245,6 +245,7 @@ using Touched = StagedList<int, kStage>;
 // swap + the caller's `d > oldValue` gate, sequentially a max (D5).  The
 // first proposal above the sentinel lists the vertex as touched.
 __device__ __forceinline__ void propose(const St& s, int t, double d, Touched& tl) {
+  if (s.step_q) d = s.step;  // fixed-step rule: every proposal is -xi/q
   long long k = okey(d);
   if (k <= s.sent) return;
   long long old = atomicMax(&s.delta[t], k);
@@ -447,7 +448,8 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
   tl.flush();
 }
 
-// apply_delta, split (src/ExtreM.cpp:1486-1503), over the touched list only;
+// apply_delta, split (src/ExtreM.cpp:1486-1503), or with the fixed-step rule
+// the edit_type 0 branch of applyDeltaBuffer, over the touched list only;
 // resets the delta slot and records changed vertices as dirty.
 __global__ void __launch_bounds__(256) k_apply(St s) {
   STAGED_LIST(dl, int, kStage, s.dirty, &s.ctr[C_DIRTY]);
@@ -459,7 +461,15 @@ __global__ void __launch_bounds__(256) k_apply(St s) {
     double o = s.orig[t], x = s.decp[t];
     int c = s.cnt[t];
     double nx;
-    if (fabs(dd) > 1e-15 && c < 5 && fabs(o - x - dd) <= s.xi) {
+    if (s.step_q) {  // applyDeltaBuffer, edit_type 0 (src/ExtreM_cuda.cu:3065-3076)
+      if (c < s.step_q - 1 && fabs(o - (x + dd)) <= s.xi) {
+        nx = x + dd;
+        c += 1;
+      } else {
+        nx = o - s.xi;
+        c = s.step_q;
+      }
+    } else if (fabs(dd) > 1e-15 && c < 5 && fabs(o - x - dd) <= s.xi) {
       nx = x + dd;
       c += 1;
     } else {

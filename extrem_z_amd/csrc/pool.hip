@@ -230,7 +230,8 @@ extern "C" {
 
 int extremz_pool_create(int device, int lanes, int64_t hbm_budget, const extremz_opts* opts,
                         extremz_pool** out) {
-  if (!out || lanes < 1 || lanes > 64 || hbm_budget < 0) return EXTREMZ_EINVAL;
+  if (!out || lanes < 1 || lanes > 64 || hbm_budget < 0 || !ez::valid_opts(opts))
+    return EXTREMZ_EINVAL;
   *out = nullptr;
   extremz_pool* p = new extremz_pool();
   int rc = ez::guarded([&] {

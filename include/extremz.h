@@ -74,7 +74,11 @@ typedef struct extremz_opts {
   int max_c_iters; /* cap on c_loops inner iterations per run; 0 = 100000 */
   int max_outer;   /* cap on outer iterations; 0 = 10000 */
   int timing;      /* bit f set: time family f with HIP events (0 = none) */
-  int reserved;
+  int fixed_step_q; /* 0: the CPU path's midpoint / snap rules (src/ExtreM.cpp);
+                       q in [1, 254]: the GPU fork's fixed-step rule set
+                       (edit_type 0, src/ExtreM_cuda.cu:2905, 3065-3076): every
+                       proposal is -xi/q, an edit count reaching q - 1 snaps
+                       to orig - xi with count q */
 } extremz_opts;
 
 typedef struct extremz_stats {

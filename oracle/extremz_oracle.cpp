@@ -246,6 +246,7 @@ struct Engine {
   double* decp;        // decp_data
   double bound;        // bound (xi)
   int max_c_iters, max_outer;
+  int step_q = 0;  // 0: the CPU path's rules; q > 0: fixed step -xi/q, threshold q
   std::vector<double> delta;       // d_deltaBuffer
   std::vector<uint8_t> counter;    // delta_counter
   std::vector<uint8_t> vt, dvt;    // vertex_type, dec_vertex_type
@@ -276,6 +277,7 @@ struct Engine {
   // swap/atomicCASDouble sequentially: a proposal wins iff larger (D5),
   // src/ExtreM.cpp:1120-1150 with the caller's `d > oldValue` gate.
   void propose(int v, double d) {
+    if (step_q) d = -bound / step_q;  // fixed-step rule: every proposal is -xi/q
     if (d > delta[v]) delta[v] = d;
   }
   double lower_mid(int v) const {  // ((orig-xi)+decp)/2 - decp
@@ -353,12 +355,24 @@ struct Engine {
     if (dl & ~ol) propose(index, (orig[index] - bound) - decp[index]);
   }
 
-  // apply_delta, split, src/ExtreM.cpp:1486-1503 (D8)
+  // apply_delta, split, src/ExtreM.cpp:1486-1503 (D8); with the fixed-step
+  // rule, applyDeltaBuffer's edit_type 0 branch (src/ExtreM_cuda.cu:3065-3076)
+  // with threshold q.
   void apply_delta() {
     const double sentinel = -4.0 * bound;
     for (int64_t t = 0; t < g.N; t++) {
       double dl = delta[t];
       if (dl == sentinel) continue;
+      if (step_q) {
+        if (counter[t] < step_q - 1 && std::abs(orig[t] - (decp[t] + dl)) <= bound) {
+          decp[t] += dl;
+          counter[t] += 1;
+        } else {
+          counter[t] = (uint8_t)step_q;
+          decp[t] = orig[t] - bound;
+        }
+        continue;
+      }
       if (std::abs(dl) > 1e-15 && counter[t] < 5 &&
           std::abs(orig[t] - decp[t] - dl) <= bound) {
         decp[t] += dl;
@@ -1132,20 +1146,35 @@ int oracle_min_label(const double* f, int W, int H, int D, int v) {
 // status, c_loop_iters, outer_iters, r_loop_calls, s_loop_calls,
 // saddle_loop_calls, c_loops_calls, edited, n_max, n_min, n_saddle1,
 // n_saddle2.  Returns 0, or -3 when an iteration cap is hit.
+int oracle_correct_rule(const double* orig, double* decp, int W, int H, int D, double xi,
+                        int max_c_iters, int max_outer, uint8_t* counts, int64_t* stats,
+                        int threads, int step_q);
 int oracle_correct(const double* orig, double* decp, int W, int H, int D,
                    double xi, int max_c_iters, int max_outer, uint8_t* counts,
                    int64_t* stats, int threads) {
+  return oracle_correct_rule(orig, decp, W, H, D, xi, max_c_iters, max_outer, counts, stats,
+                             threads, 0);
+}
+
+// The same with a rule set: step_q = 0 for the CPU path's midpoint / snap
+// rules, q > 0 for the GPU fork's fixed step -xi/q and threshold q
+// (edit_type 0: src/ExtreM_cuda.cu:2905 and the other `-bound / (q)` sites,
+// applyDeltaBuffer :3065-3076).
+int oracle_correct_rule(const double* orig, double* decp, int W, int H, int D, double xi,
+                        int max_c_iters, int max_outer, uint8_t* counts, int64_t* stats,
+                        int threads, int step_q) {
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #else
   (void)threads;
 #endif
-  if (W <= 0 || H <= 0 || D <= 0 || !(xi > 0)) return -22;
+  if (W <= 0 || H <= 0 || D <= 0 || !(xi > 0) || step_q < 0 || step_q > 254) return -22;
   Engine e;
   e.g = Grid{W, H, D, (int64_t)W * H * D};
   e.orig = orig;
   e.decp = decp;
   e.bound = xi;
+  e.step_q = step_q;
   e.max_c_iters = max_c_iters > 0 ? max_c_iters : 100000;
   e.max_outer = max_outer > 0 ? max_outer : 10000;
   int rc = e.run();

@@ -38,6 +38,7 @@ def lib():
         L.oracle_max_label.argtypes = [vp, i32, i32, i32, i32]
         L.oracle_min_label.argtypes = [vp, i32, i32, i32, i32]
         L.oracle_correct.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, vp, vp, i32]
+        L.oracle_correct_rule.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, vp, vp, i32, i32]
         L.oracle_bitmap3.argtypes = [vp, i64, vp]
         L.oracle_bitmap3.restype = i64
         L.oracle_cost.argtypes = [vp, vp, vp, vp, i64, dbl, vp, vp, vp]
@@ -90,14 +91,16 @@ def min_label(field, v):
     return lib().oracle_min_label(_p(f), *_dims(f), int(v))
 
 
-def correct(orig, decp, xi, max_c_iters=0, max_outer=0, threads=0):
-    """Returns (fixed, counts, stats dict).  Inputs are not modified."""
+def correct(orig, decp, xi, max_c_iters=0, max_outer=0, threads=0, fixed_step_q=0):
+    """Returns (fixed, counts, stats dict).  Inputs are not modified.
+    fixed_step_q > 0 selects the fixed-step rule set (step -xi/q, threshold q)."""
     o = np.ascontiguousarray(orig, dtype=np.float64)
     d = np.array(decp, dtype=np.float64, order="C", copy=True)
     counts = np.zeros(o.shape, np.uint8)
     st = np.zeros(12, np.int64)
-    lib().oracle_correct(_p(o), _p(d), *_dims(o), float(xi), int(max_c_iters),
-                         int(max_outer), _p(counts), _p(st), int(threads))
+    lib().oracle_correct_rule(_p(o), _p(d), *_dims(o), float(xi), int(max_c_iters),
+                              int(max_outer), _p(counts), _p(st), int(threads),
+                              int(fixed_step_q))
     return d, counts, dict(zip(STAT_KEYS, (int(x) for x in st)))
 
 

@@ -147,3 +147,26 @@ def test_full_size_256_digest():
         assert stats[k] == v, k
     assert hashlib.sha256(counts.tobytes()).hexdigest() == ref["sha256_counts"]
     assert hashlib.sha256(fixed.tobytes()).hexdigest() == ref["sha256_fixed"]
+
+
+@pytest.mark.parametrize("q", [3, 4, 6])
+@pytest.mark.parametrize("shape,seed,kind,er", CASES[:4])
+def test_fixed_step_rule_parity(shape, seed, kind, er, q):
+    """The fixed-step rule set (edit_type 0 of the GPU fork): GPU against the
+    oracle, byte for byte, with the counters."""
+    orig, decp, xi = make_field(shape, seed, kind, er)
+    f0, c0, s0 = oracle.correct(orig, decp, xi, fixed_step_q=q)
+    f1, c1, s1 = ez().correct(orig, decp, xi, fixed_step_q=q)
+    assert s0["status"] == 0
+    assert f0.tobytes() == f1.tobytes()
+    assert np.array_equal(c0, c1) and int(c1.max()) <= q
+    assert {k: s0[k] for k in COUNTERS} == {k: s1[k] for k in COUNTERS}
+    tol = xi + np.spacing(np.abs(orig) + xi)
+    assert np.all(np.abs(f1 - orig) <= tol)
+
+
+def test_fixed_step_rule_rejects_bad_q():
+    orig, decp, xi = make_field((8, 8, 8), 1, "noise", 1e-2)
+    with pytest.raises(ez().ExtremzError) as e:
+        ez().correct(orig, decp, xi, fixed_step_q=255)
+    assert e.value.code == -22

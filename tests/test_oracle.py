@@ -53,6 +53,29 @@ def test_known_answer_apply_cap():
     assert f[2] == 0.26 and counts.ravel()[2] == 0
 
 
+def test_known_answer_fixed_step_rule():
+    # The apply-cap field under the fixed-step rule with q = 4 (edit_type 0,
+    # src/ExtreM_cuda.cu:2905, 3065-3076): every proposal is -xi/4; v1 takes
+    # three steps of -0.0625 (count 1..3), still above v2, and the fourth
+    # iteration finds count == q - 1 and snaps it to orig - xi with count q.
+    orig = np.array([0.0, 0.5, 0.5000001, 1.0]).reshape(1, 1, 4)
+    xi = 0.25
+    decp = np.array([0.0, 0.74, 0.26, 1.0]).reshape(1, 1, 4)
+    fixed, counts, st = oracle.correct(orig, decp, xi, fixed_step_q=4)
+    assert st["status"] == 0 and st["c_loop_iters"] == 4
+    f = fixed.ravel()
+    assert f[1] == 0.5 - 0.25 and counts.ravel().tolist() == [0, 4, 0, 0]
+    assert f[2] == 0.26 and f[0] == -0.25 and f[3] == 0.75
+    # q = 8: seven steps cover only 7/8 xi, so the same field snaps after
+    # eight iterations; a false max that one step clears is fixed in one
+    fixed8, counts8, st8 = oracle.correct(orig, decp, xi, fixed_step_q=8)
+    assert fixed8.ravel()[1] == 0.25 and counts8.ravel()[1] == 8 and st8["c_loop_iters"] == 8
+    near = np.array([0.0, 0.31, 0.28, 1.0]).reshape(1, 1, 4)
+    fixed1, counts1, st1 = oracle.correct(orig, near, xi, fixed_step_q=8)
+    assert fixed1.ravel()[1] == 0.31 + (-xi / 8) and counts1.ravel()[1] == 1
+    assert st1["c_loop_iters"] == 1 and fixed1.ravel()[2] == 0.28
+
+
 @pytest.mark.parametrize("shape,seed,kind,er", [
     ((16, 16, 16), 3, "gauss", 1e-3),
     ((20, 14, 10), 4, "noise", 2e-2),
