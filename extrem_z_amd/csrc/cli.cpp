@@ -58,7 +58,8 @@ int main(int argc, char** argv) {
   if (argc < 4) {
     std::fprintf(stderr,
                  "usage: %s <path,W,H,D> <er> <compressor_id> [--decp decp.bin] [--orig-f32] "
-                 "[--out fixed.bin] [--counts counts.bin]\n",
+                 "[--out fixed.bin] [--counts counts.bin] [--cost prefix] [--cmp-bytes n] "
+                 "[--fixed-step q]\n",
                  argv[0]);
     return 2;
   }

@@ -25,6 +25,8 @@ CASES = [
     ((33, 17, 5), 6, "noise", 5e-2),
     ((40, 40, 40), 8, "gauss", 1e-3),
     ((7, 1, 9), 9, "noise", 1e-1),
+    ((48, 40, 1), 10, "gauss", 1e-3),  # depth 1: the reference's planar triangle mesh
+    ((37, 29, 1), 11, "noise", 2e-2),
 ]
 
 

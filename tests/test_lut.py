@@ -121,3 +121,70 @@ def test_saddles_closed_form():
     assert t[2, 2, 2] == 1          # two lower cones, one upper ring
     t, _ = oracle.classify(-f)
     assert t[2, 2, 2] == 2
+
+
+def reference_triangles(W, H):
+    """The planar triangles of a W x H x 1 field as the reference enumerates
+    them (src/ExtreM.cpp:597-627): cell (x, y) holds (x,y),(x+1,y),(x,y+1)
+    and (x+1,y+1),(x,y+1),(x+1,y)."""
+    tris = []
+    for t in range(2 * (W - 1) * (H - 1)):
+        b = t // 2
+        x, y = b % (W - 1), b // (W - 1)
+        if t % 2 == 0:
+            tris.append((x + y * W, x + 1 + y * W, x + (y + 1) * W))
+        else:
+            tris.append((x + 1 + (y + 1) * W, x + (y + 1) * W, x + 1 + y * W))
+    return tris
+
+
+def mesh_type(flat, v, tris):
+    """Critical type of v from its link in the triangle mesh alone: link
+    vertices are the other corners of the triangles around v, link edges the
+    sides opposite v. No stencil directions are used."""
+    link_edges, link = [], set()
+    for t in tris:
+        if v in t:
+            a, b = (u for u in t if u != v)
+            link_edges.append((a, b))
+            link.update((a, b))
+    below = {u for u in link if (flat[u], u) < (flat[v], v)}
+    above = link - below
+
+    def comps(S):
+        parent = {u: u for u in S}
+
+        def f(a):
+            while parent[a] != a:
+                a = parent[a]
+            return a
+        for a, b in link_edges:
+            if a in S and b in S:
+                parent[f(a)] = f(b)
+        return len({f(u) for u in S})
+    lc, uc = comps(below), comps(above)
+    if not above:
+        return 4
+    if not below:
+        return 0
+    return {(True, True): 3, (True, False): 1, (False, True): 2}.get((lc > 1, uc > 1), 5)
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("W,H", [(6, 5), (9, 3), (2, 7), (11, 11)])
+def test_depth1_is_the_reference_triangle_mesh(W, H, seed):
+    """A field of depth 1 is the reference's 2-D mode: its in-plane stencil
+    directions (+-x, +-y, +-(-1,1,0)) are exactly the edges of the reference's
+    planar triangulation, so each vertex's type equals the one its link in
+    that mesh gives (interior saddles have as many lower as upper link
+    components, so they carry both saddle bits)."""
+    rng = np.random.default_rng(100 + seed)
+    f = rng.integers(0, 5, size=(1, H, W)).astype(np.float64)
+    types, _ = oracle.classify(f)
+    flat = f.ravel()
+    tris = reference_triangles(W, H)
+    got = [mesh_type(flat, v, tris) for v in range(W * H)]
+    assert types.ravel().tolist() == got
+    # interior vertices: saddle means both bits (lower and upper cycles alternate)
+    t2 = types.reshape(H, W)[1:-1, 1:-1]
+    assert not np.isin(t2, (1, 2)).any()
