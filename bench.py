@@ -50,13 +50,12 @@ METRIC = "corrected voxels/s + % HBM roofline (byte-exact vs CPU); p50/p99 job l
 #               + write the lower-star mask (2)
 #   saddle_pass per saddle side: one frontier item, 15 x (8 B value + 4 B label)
 #   classify / check_full per voxel: read f64 (8) + type (1) + mask (2)
-#   label_jump / label_chase: extra pointer-jumping rounds carry no algorithmic bytes
+#   label_chase: the walk of the pointers left by the tiles carries no algorithmic bytes
 ALG_BYTES = {
     "label_tile": ("voxel", 18),
     "saddle_pass": ("saddle side", 180),
     "classify": ("voxel", 11),
     "check_full": ("voxel", 11),
-    "label_jump": ("voxel", 0),
     "label_chase": ("voxel", 0),
 }
 

@@ -28,8 +28,8 @@ LIB_PATH = os.path.join(_HERE, "_build", "libextremz.so")
 
 STAT_KEYS = ("status", "c_loop_iters", "outer_iters", "r_loop_calls", "s_loop_calls",
              "saddle_loop_calls", "c_loops_calls", "edited", "n_max", "n_min", "n_saddle1",
-             "n_saddle2", "label_passes", "jump_rounds", "host_syncs", "frontier_items")
-KERNEL_FAMILIES = ("classify", "check_full", "check_incr", "label_tile", "label_jump",
+             "n_saddle2", "label_passes", "inc_passes", "host_syncs", "frontier_items")
+KERNEL_FAMILIES = ("classify", "check_full", "check_incr", "label_tile", "incremental",
                    "label_chase", "saddle_pass", "slot_gather", "propose", "apply", "reserve")
 TIME_ALL = (1 << len(KERNEL_FAMILIES)) - 1
 

@@ -49,6 +49,22 @@ enum Ctr {
   C_NLONG,        // slots with long adjacency ranges
   C_NCHUNK,       // chunks of those ranges
   C_WN_BAIL,      // the one-block reservation solver left the list to the rounds
+  C_LCHG,         // decp changed since the last CP pass of decp (list lchg)
+  C_EXTD,         // ... at a listed extremum
+  C_AL,           // incremental pass lists: lower star changed,
+  C_CU,           // up pointer changed,
+  C_CD,           // down pointer changed,
+  C_QU,           // up root changed,
+  C_QD,           // down root changed,
+  C_DSL2,         // 2-saddle / 1-saddle sides to recompute,
+  C_DSL1,
+  C_IBAIL,        // the incremental pass gave up (the full pass runs instead)
+                  // (C_LCHG..C_IBAIL are cleared at the end of every CP pass of decp)
+  C_DSLOT,        // slots to gather again
+  C_CL2,          // changed sides of the last CP pass
+  C_CL1,
+  C_CLN2,         // changed sides of this CP pass
+  C_CLN1,
   C_COUNT_
 };
 
@@ -93,6 +109,23 @@ struct St {
   int *wnl2, *wnl1;               // wrong_neighbors(_ds)_index
   int *minsad2, *minsad1;         // greedy reservations
   int *win_a, *win_b;             // first-saddle-wins reservations
+  // Incremental CP passes of decp: what changed since the last one, and the
+  // per-pass work lists.
+  uint8_t* pdir;                  // steepest (up + 1) | (down + 1) << 4 per voxel; 0: itself
+  uint8_t* lflag;                 // decp changed since the last CP pass of decp
+  int* lchg;                      // those vertices (C_LCHG)
+  uint8_t* cflag;                 // 1: up pointer changed in this pass, 2: down pointer
+  uint8_t *tu, *td;               // old up / down roots of the changed pointers
+  int *spos2, *spos1;             // list position of a vertex in S2 / S1, else -1
+  int *al, *cu, *cd, *ou, *od, *nu, *nd;  // pass lists (capacity icap)
+  int *qu, *qd;                   // vertices whose up / down root changed (capacity N)
+  int icap;
+  int* sstamp;                    // per side: S2 positions, then n2 + S1 positions
+  int *dsl2, *dsl1;               // sides to recompute (positions)
+  int *cl2, *cl1, *cln2, *cln1;   // changed sides of the last / this pass (positions)
+  int* slstamp;                   // per slot index
+  int* dslot;                     // slots to gather again
+  int iepoch;                     // stamp of this pass
 };
 
 // Modes of the saddle pass.  kModeOrig: original field, every saddle pushes
@@ -169,7 +202,8 @@ __global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int
 __global__ void k_clear_dirty(St s);
 __global__ void k_propose_c(St s, const int* front, int fctr);
 __global__ void k_apply(St s);
-__global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t* lower);
+__global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t* lower,
+                              uint8_t* pdir);
 __global__ void k_chase(int n, int* ds, int* as);
 __global__ void k_list_fill(const int* list, int n, int* arr, int val);
 __global__ void k_saddle_list(St s, int mode, const double* f, const int* L2, const int* L1,
@@ -190,7 +224,21 @@ __global__ void k_scan_single(int* a, int n);
 __global__ void k_slot_init(St s, long long* kmax, int* smax, long long* kmin, int* smin);
 __global__ void k_list_fill64(const int* list, int n, long long* arr, long long val);
 __global__ void k_ext_reduce(const int* inmax, int nmax, const int* inmin, int nmin,
-                             const double* f, int* outmax, int* outmin);
+                             const double* f, int* outmax, int* outmin, const int* gate);
+__global__ void k_spos(St s);
+__global__ void k_inc_ptrs(St s, int epoch);
+__global__ void k_inc_taint(St s);
+__global__ void k_inc_walk(St s);
+__global__ void k_inc_prop(St s, int qcap);
+__global__ void k_inc_mark(St s);
+__global__ void k_inc_mark_csr(St s);
+__global__ void k_inc_saddles(St s, const double* f, int2* pairs, long long* kmax,
+                              long long* kmin);
+__global__ void k_inc_pull(St s, long long* pkey, int* pid);
+__global__ void k_inc_clear(St s);
+__global__ void k_ctr_copy2(int* ctr, int a, int from_a, int b, int from_b);
+__global__ void k_ctr_fill(int* ctr, int a, int b, int v);
+__global__ void k_diff_count(const void* a, const void* b, long long nbytes, int* out);
 __global__ void k_act_copy(St s, int2* dst, int dctr);
 __global__ void k_wn_register(St s, int2* active, int actr);
 __global__ void k_wn_commit(St s, int2* active, int actr);

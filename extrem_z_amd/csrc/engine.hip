@@ -159,6 +159,10 @@ struct extremz_ctx {
   ez::DevBuf<int> ctr;
   ez::DevBuf<double> orig_tmp, decp_tmp;
   ez::DevBuf<uint8_t> counts_tmp;
+  // incremental CP passes of decp
+  ez::DevBuf<uint8_t> pdir, lflag, cflag, tu, td;
+  ez::DevBuf<int> lchg, spos2, spos1, ilist, qu, qd, sstamp, slists, slstamp, dslot;
+  ez::DevBuf<uint8_t> vsnap;  // snapshots of the self-check (EXTREMZ_VERIFY_INC)
   int* h_ctr = nullptr;
   std::vector<ez::EventPair> ev_pool;
   size_t ev_used = 0;
@@ -173,10 +177,13 @@ struct extremz_ctx {
          {&oDS, &oAS, &dDS, &dAS, &stamp, &touched, &dirty, &frontA, &frontB, &S2, &S1, &MX, &MN,
           &olarg, &osmall, &dlarg, &dsmall, &minsad2, &minsad1, &win_a, &win_b, &lcnt, &csr,
           &csr_off, &csr_cur, &pid, &cpos, &ext_id, &of2, &ol2, &of1, &ol1, &df2, &dl2, &df1,
-          &dl1, &wnl2, &wnl1, &bmax2, &bmin2, &ctr})
+          &dl1, &wnl2, &wnl1, &bmax2, &bmin2, &ctr, &lchg, &spos2, &spos1, &ilist, &qu, &qd,
+          &sstamp, &slists, &slstamp, &dslot})
       f(*b);
     for (ez::DevBuf<long long>* b : {&delta, &kmax, &kmin, &pkey, &ckey, &pv2, &pv1}) f(*b);
-    for (ez::DevBuf<uint8_t>* b : {&cnt, &otype, &dflag, &wn2, &wn1, &counts_tmp}) f(*b);
+    for (ez::DevBuf<uint8_t>* b :
+         {&cnt, &otype, &dflag, &wn2, &wn1, &counts_tmp, &pdir, &lflag, &cflag, &tu, &td, &vsnap})
+      f(*b);
     for (ez::DevBuf<uint16_t>* b : {&olow, &dlow}) f(*b);
     for (ez::DevBuf<int2>* b : {&longs, &act0, &actA, &actB, &pairs}) f(*b);
     f(chunks);
@@ -204,6 +211,10 @@ class Run {
     c_->timing = o ? o->timing : 0;
     c_->ev_used = 0;
     step_q_ = o ? o->fixed_step_q : 0;
+    const char* e = std::getenv("EXTREMZ_FULL_PASSES");
+    inc_off_ = e && *e && *e != '0';
+    e = std::getenv("EXTREMZ_VERIFY_INC");
+    inc_verify_ = e && *e && *e != '0';
   }
 
   int correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
@@ -224,6 +235,9 @@ class Run {
   int front_ = C_FRONT_A;  // counter index of the live frontier
   int* front_buf_ = nullptr;
   int* back_buf_ = nullptr;
+  bool inc_ok_ = false;      // the decp CP state is complete (a CP pass of decp has run)
+  bool inc_off_ = false;     // EXTREMZ_FULL_PASSES: every CP pass of decp is a full pass
+  bool inc_verify_ = false;  // EXTREMZ_VERIFY_INC: each incremental pass is checked
 
   hipStream_t st() const { return c_->stream; }
 
@@ -241,6 +255,12 @@ class Run {
   int tile_blocks() const {
     return ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4) * ((s_.g.D + 3) / 4);
   }
+  // Capacity of the incremental pass's vertex lists: a pass with nch changed
+  // vertices touches at most 15 * nch vertices, and runs only when that fits.
+  static long long inc_cap(long long N) { return N / 4 + 1024; }
+  // Largest share of changed vertices (1 / kIncDiv of the grid) taken
+  // incrementally; more, and the full pass is faster.
+  static constexpr long long kIncDiv = 256;
   static int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     return (int)std::max<long long>(1, std::min<long long>(b, 2048));
@@ -285,9 +305,13 @@ class Run {
   void slot_pass(int dec, const double* f, const int* L2, const int* L1, const uint16_t* flow,
                  int* F2, int* La2, int* F1, int* La1, int* smax, int* smin);
   void apply();
-  void label_pass(const double* f, int* ds, int* as, uint16_t* lower);
+  void label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir = nullptr);
   void c_loops();
   void cp_dec(bool force);
+  bool cp_inc(long long nch);
+  void cp_end();
+  void cp_full();
+  void verify_inc();
   void wrong_neighbors();
   void r_loops();
   void s_loops(int side, int do_fix);
@@ -323,6 +347,9 @@ void Run::allocate(long long N) {
                          &c->win_a, &c->win_b, &c->of2, &c->ol2, &c->of1, &c->ol1, &c->df2,
                          &c->dl2, &c->df1, &c->dl1, &c->wnl2, &c->wnl1})
     b->ensure(N);
+  for (DevBuf<uint8_t>* b : {&c->pdir, &c->lflag, &c->cflag, &c->tu, &c->td}) b->ensure(N);
+  for (DevBuf<int>* b : {&c->lchg, &c->spos2, &c->spos1, &c->qu, &c->qd}) b->ensure(N);
+  c->ilist.ensure(7 * inc_cap(N));
   c->bmax2.ensure(256);
   c->bmin2.ensure(256);
   c->act0.ensure(2 * N);
@@ -400,9 +427,9 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
   if (dec) {
     // decp global maximum / minimum among the listed extrema
     k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, f, c_->bmax2.p,
-                                        c_->bmin2.p);
+                                        c_->bmin2.p, nullptr);
     k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, 256, c_->bmin2.p, 256, f, s_.ctr + C_DGMAX,
-                                      s_.ctr + C_DGMIN);
+                                      s_.ctr + C_DGMIN, nullptr);
   }
   tend();
 }
@@ -419,10 +446,10 @@ void Run::apply() {
 // resolution, then a chase of whatever is left (a separate pointer-jumping
 // round before the chase costs more than the chase saves).  No host round
 // trip.
-void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower) {
+void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir) {
   counts_[EXTREMZ_STAT_LABEL_PASSES]++;
   tbegin(EXTREMZ_K_LABEL_TILE);
-  k_labels_tile<<<tile_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as, lower);
+  k_labels_tile<<<tile_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as, lower, pdir);
   tend();
   int nb = (s_.g.N + 255) / 256;
   tbegin(EXTREMZ_K_LABEL_CHASE);
@@ -485,14 +512,141 @@ void Run::c_loops() {
 // Skipped when decp has not changed since the last call: everything it
 // computes depends on decp only.
 void Run::cp_dec(bool force) {
+  long long nch = 0;
   if (!force) {
     pull();
     if (!h(C_ANY_CHANGE)) return;
+    nch = h(C_LCHG);
   }
-  label_pass(s_.decp, s_.dDS, s_.dAS, s_.dlow);
+  // few changes since the last pass: only what they can reach is recomputed
+  // (the self-check takes every pass that fits the lists, to exercise them)
+  const bool inc = inc_ok_ && !inc_off_ && 15 * nch <= s_.icap &&
+                   (inc_verify_ || nch <= s_.g.N / kIncDiv);
+  if (!(inc && cp_inc(nch))) cp_full();
+  cp_end();
+  inc_ok_ = true;
+  set_ctr(C_ANY_CHANGE, 0);
+}
+
+void Run::cp_full() {
+  label_pass(s_.decp, s_.dDS, s_.dAS, s_.dlow, s_.pdir);
   slot_pass(1, s_.decp, s_.dDS, s_.dAS, s_.dlow, s_.df2, s_.dl2, s_.df1, s_.dl1, s_.dlarg,
             s_.dsmall);
-  set_ctr(C_ANY_CHANGE, 0);
+}
+
+// The incremental CP pass (the kernels' section "incremental CP pass"):
+// false when it gave up (the caller runs the full pass).  The saddle part
+// falls back to the full saddle pass when a quarter of the sides are queued.
+bool Run::cp_inc(long long nch) {
+  counts_[EXTREMZ_STAT_INC_PASSES]++;
+  epoch_++;
+  s_.iepoch = epoch_;
+  const int b1 = blocks_for(15 * nch);
+  tbegin(EXTREMZ_K_INCREMENTAL);
+  k_inc_ptrs<<<b1, 256, 0, st()>>>(s_, epoch_);
+  k_inc_taint<<<b1, 256, 0, st()>>>(s_);
+  k_inc_walk<<<b1, 256, 0, st()>>>(s_);
+  k_inc_prop<<<1, 1024, 0, st()>>>(s_, s_.g.N);
+  k_inc_mark<<<1024, 256, 0, st()>>>(s_);
+  k_inc_mark_csr<<<256, 256, 0, st()>>>(s_);
+  tend();
+  pull();
+  if (h(C_IBAIL)) return false;
+  const long long nd = (long long)h(C_DSL2) + h(C_DSL1);
+  if (4 * nd > (long long)s_.n2 + s_.n1) {
+    slot_pass(1, s_.decp, s_.dDS, s_.dAS, s_.dlow, s_.df2, s_.dl2, s_.df1, s_.dl1, s_.dlarg,
+              s_.dsmall);
+  } else {
+    long long* kmax = c_->kmax.p;
+    long long* kmin = c_->kmin.p;
+    tbegin(EXTREMZ_K_INCREMENTAL);
+    k_slot_init<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(s_, kmax, s_.dlarg,
+                                                                            kmin, s_.dsmall);
+    k_inc_saddles<<<blocks_for(nd), 256, 0, st()>>>(s_, s_.decp, c_->pairs.p, kmax, kmin);
+    k_inc_pull<<<256, 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
+    k_slot_merge<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(
+        s_, kmax, kmin, s_.dlarg, s_.dsmall, c_->pkey.p, c_->pid.p, 1);
+    k_slot_ids<<<256, 256, 0, st()>>>(s_, c_->pairs.p, s_.decp, kmax, kmin, s_.dlarg,
+                                      s_.dsmall);
+    // decp global maximum / minimum, when a listed extremum changed value
+    k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, s_.decp, c_->bmax2.p,
+                                        c_->bmin2.p, s_.ctr + C_EXTD);
+    k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, 256, c_->bmin2.p, 256, s_.decp,
+                                      s_.ctr + C_DGMAX, s_.ctr + C_DGMIN, s_.ctr + C_EXTD);
+    tend();
+  }
+  if (inc_verify_) verify_inc();
+  return true;
+}
+
+// End of every CP pass of decp: the pass's flags are cleared, the changed
+// sides become the last pass's, and the change list starts empty.
+void Run::cp_end() {
+  k_inc_clear<<<blocks_for(15 * (long long)s_.icap), 256, 0, st()>>>(s_);
+  k_ctr_fill<<<1, 64, 0, st()>>>(s_.ctr, C_LCHG, C_IBAIL + 1, 0);
+  k_ctr_copy2<<<1, 64, 0, st()>>>(s_.ctr, C_CL2, C_CLN2, C_CL1, C_CLN1);
+  std::swap(s_.cl2, s_.cln2);
+  std::swap(s_.cl1, s_.cln1);
+}
+
+// Self-check of an incremental pass: its state against the full pass's on
+// the same decp (which then stands).  Throws on any difference.
+void Run::verify_inc() {
+  struct Arr {
+    const char* name;
+    void* p;
+    long long bytes;
+  };
+  const long long N = s_.g.N, ns = (long long)s_.nmax + s_.nmin;
+  const Arr arrs[] = {{"dDS", s_.dDS, 4 * N},         {"dAS", s_.dAS, 4 * N},
+                      {"dlow", s_.dlow, 2 * N},       {"pdir", s_.pdir, N},
+                      {"df2", s_.df2, 4LL * s_.n2},   {"dl2", s_.dl2, 4LL * s_.n2},
+                      {"df1", s_.df1, 4LL * s_.n1},   {"dl1", s_.dl1, 4LL * s_.n1},
+                      {"pv2", s_.pv2, 8LL * s_.n2},   {"pv1", s_.pv1, 8LL * s_.n1},
+                      {"dlarg", s_.dlarg, 4LL * s_.nmax}, {"dsmall", s_.dsmall, 4LL * s_.nmin},
+                      {"pkey", c_->pkey.p, 8 * ns},   {"pid", c_->pid.p, 4 * ns}};
+  const int na = (int)(sizeof(arrs) / sizeof(arrs[0]));
+  long long total = 0;
+  for (const Arr& a : arrs) total += a.bytes;
+  c_->vsnap.ensure((size_t)total + 64);
+  hipStream_t q = st();
+  long long off = 0;
+  for (const Arr& a : arrs) {
+    if (a.bytes) hip_check(hipMemcpyAsync(c_->vsnap.p + off, a.p, a.bytes,
+                                          hipMemcpyDeviceToDevice, q), "hipMemcpyAsync");
+    off += a.bytes;
+  }
+  // (the number of recorded slot pairs depends on the order of the atomics,
+  // only the slots they settle do not)
+  int hc[3];
+  const int keep[3] = {C_DGMAX, C_DGMIN, C_ACT0};
+  pull();
+  for (int k = 0; k < 3; k++) hc[k] = h(keep[k]);
+  // the full pass on the same decp, over the same change lists
+  label_pass(s_.decp, s_.dDS, s_.dAS, s_.dlow, s_.pdir);
+  slot_pass(1, s_.decp, s_.dDS, s_.dAS, s_.dlow, s_.df2, s_.dl2, s_.df1, s_.dl1, s_.dlarg,
+            s_.dsmall);
+  DevBuf<int> diff;
+  diff.ensure(na);
+  hip_check(hipMemsetAsync(diff.p, 0, sizeof(int) * na, q), "memset");
+  off = 0;
+  for (int k = 0; k < na; k++) {
+    if (arrs[k].bytes)
+      k_diff_count<<<blocks_for(arrs[k].bytes), 256, 0, q>>>(c_->vsnap.p + off, arrs[k].p,
+                                                             arrs[k].bytes, diff.p + k);
+    off += arrs[k].bytes;
+  }
+  std::vector<int> hd(na);
+  hip_check(hipMemcpyAsync(hd.data(), diff.p, sizeof(int) * na, hipMemcpyDeviceToHost, q),
+            "hipMemcpyAsync");
+  pull();
+  std::string bad;
+  for (int k = 0; k < na; k++)
+    if (hd[k]) bad += std::string(" ") + arrs[k].name + "(" + std::to_string(hd[k]) + " B)";
+  const char* cn[3] = {"dgmax", "dgmin", "act0"};
+  for (int k = 0; k < 3; k++)
+    if (hc[k] != h(keep[k])) bad += std::string(" ") + cn[k];
+  if (!bad.empty()) throw HipError("incremental CP pass differs from the full pass:" + bad);
 }
 
 // get_wrong_split_neighbors / get_wrong_join_neighbors over all saddles in
@@ -845,6 +999,25 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   s_.minsad1 = c->minsad1.p;
   s_.win_a = c->win_a.p;
   s_.win_b = c->win_b.p;
+  s_.pdir = c->pdir.p;
+  s_.lflag = c->lflag.p;
+  s_.lchg = c->lchg.p;
+  s_.cflag = c->cflag.p;
+  s_.tu = c->tu.p;
+  s_.td = c->td.p;
+  s_.spos2 = c->spos2.p;
+  s_.spos1 = c->spos1.p;
+  s_.qu = c->qu.p;
+  s_.qd = c->qd.p;
+  {
+    const long long cap = inc_cap(N);
+    s_.icap = (int)cap;
+    int* il = c->ilist.p;
+    for (int** l : {&s_.al, &s_.cu, &s_.cd, &s_.ou, &s_.od, &s_.nu, &s_.nd}) {
+      *l = il;
+      il += cap;
+    }
+  }
 
   hipStream_t q = st();
   const int bN = blocks_for(N);
@@ -857,6 +1030,8 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   ez::hip_check(hipMemsetAsync(s_.dflag, 0, N, q), "memset");
   ez::hip_check(hipMemsetAsync(s_.wn2, 0, N, q), "memset");
   ez::hip_check(hipMemsetAsync(s_.wn1, 0, N, q), "memset");
+  for (uint8_t* p : {s_.lflag, s_.cflag, s_.tu, s_.td})
+    ez::hip_check(hipMemsetAsync(p, 0, N, q), "memset");
   for (int* p : {s_.minsad2, s_.minsad1, s_.win_a, s_.win_b})
     k_fill_i32<<<bN, 256, 0, q>>>(p, 0x7fffffff, N);
   k_fill_i32<<<bN, 256, 0, q>>>(s_.ext_id, -1, N);
@@ -884,6 +1059,28 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   counts_[EXTREMZ_STAT_N_MIN] = s_.nmin;
   // every saddle records at most one pair per distinct root of its star
   c->pairs.ensure(14LL * (s_.n2 + s_.n1) + 1);
+  {  // incremental CP passes: saddle positions, side and slot lists and stamps
+    const long long n21 = (long long)s_.n2 + s_.n1, ns = (long long)s_.nmax + s_.nmin;
+    c->sstamp.ensure(n21 + 1);
+    c->slists.ensure(3 * n21 + 1);
+    c->slstamp.ensure(ns + 1);
+    c->dslot.ensure(ns + 1);
+    s_.sstamp = c->sstamp.p;
+    s_.slstamp = c->slstamp.p;
+    s_.dslot = c->dslot.p;
+    int* sl = c->slists.p;
+    s_.dsl2 = sl;
+    s_.dsl1 = sl + s_.n2;
+    s_.cl2 = sl + n21;
+    s_.cl1 = s_.cl2 + s_.n2;
+    s_.cln2 = sl + 2 * n21;
+    s_.cln1 = s_.cln2 + s_.n2;
+    ez::hip_check(hipMemsetAsync(s_.sstamp, 0, sizeof(int) * (n21 + 1), q), "memset");
+    ez::hip_check(hipMemsetAsync(s_.slstamp, 0, sizeof(int) * (ns + 1), q), "memset");
+    k_fill_i32<<<bN, 256, 0, q>>>(s_.spos2, -1, N);
+    k_fill_i32<<<bN, 256, 0, q>>>(s_.spos1, -1, N);
+    k_spos<<<blocks_for(std::max(s_.n2, s_.n1)), 256, 0, q>>>(s_);
+  }
   k_ext_index<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, q>>>(s_);
   {
     const long long ns = (long long)s_.nmax + s_.nmin;

@@ -450,9 +450,11 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
 
 // apply_delta, split (src/ExtreM.cpp:1486-1503), or with the fixed-step rule
 // the edit_type 0 branch of applyDeltaBuffer, over the touched list only;
-// resets the delta slot and records changed vertices as dirty.
+// resets the delta slot and records changed vertices as dirty (for the next
+// false-critical-point check) and as changed since the last CP pass of decp.
 __global__ void __launch_bounds__(256) k_apply(St s) {
   STAGED_LIST(dl, int, kStage, s.dirty, &s.ctr[C_DIRTY]);
+  STAGED_LIST(cl, int, kStage, s.lchg, &s.ctr[C_LCHG]);
   int n = s.ctr[C_TOUCHED];
   GRID_STRIDE(i, n) {
     int t = s.touched[i];
@@ -484,9 +486,14 @@ __global__ void __launch_bounds__(256) k_apply(St s) {
         s.dflag[t] = 1;
         dl.push(t);
       }
+      if (!s.lflag[t]) {  // one touched-list entry per vertex: no other lane sets it
+        s.lflag[t] = 1;
+        cl.push(t);
+      }
     }
   }
   dl.flush();
+  cl.flush();
 }
 
 // ----------------------------------------------------- manifold labels ----
@@ -496,7 +503,9 @@ __global__ void __launch_bounds__(256) k_apply(St s) {
 // pointers (first loops of src/ExtreM.cpp:146-157 and 504-515) are computed
 // from LDS and resolved inside the tile by pointer doubling; a pointer then
 // names the root or the first vertex outside the tile on the steepest path.
-// With lower != nullptr the lower-star mask of every voxel is written too.
+// With lower != nullptr the lower-star mask of every voxel is written too,
+// with pdir != nullptr its steepest directions ((up + 1) | (down + 1) << 4,
+// 0 for the voxel itself), the state the incremental CP pass starts from.
 //
 // Fast path: for an interior voxel of a field with W >= 3 the 15 candidate
 // ids (self + 14 neighbours) have the fixed order of their offsets
@@ -509,7 +518,7 @@ constexpr int kPPer = (kPN + 255) / 256;
 constexpr int kPairBuf = 2048;  // (position, saddle) pairs staged in LDS per block
 
 __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int* ds, int* as,
-                                                     uint16_t* lower) {
+                                                     uint16_t* lower, uint8_t* pdir) {
   __shared__ double ring[3][kPN];
   __shared__ int sup[kLTN];
   __shared__ int sdn[kLTN];
@@ -606,6 +615,7 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
         }
       }
       if (lower) lower[v] = (uint16_t)lm;
+      if (pdir) pdir[v] = (uint8_t)((du + 1) | ((dd + 1) << 4));
       if (du >= 0) {
         int ux = lx + c_dx[du], uy = ly + c_dy[du], uz = lz + c_dz[du];
         bool inside = ux >= 0 && ux < kLTX && uy >= 0 && uy < kLTY && uz >= 0 && uz < kLTZ;
@@ -870,12 +880,38 @@ __device__ __forceinline__ int slot_index(const St& s, int e) {
   return (e & 1) ? s.nmax + (e >> 1) : (e >> 1);
 }
 
+// Queues the slots whose original adjacent saddles include the saddle v
+// (its original roots of this side, the extremum->saddle index of the
+// kModeFill pass) for a new gather: v's gather key changed.
+template <int SIDE>
+__device__ __forceinline__ void regather_slots(const St& s, int v) {
+  int x, y, z;
+  coords(s.g, v, x, y, z);
+  const unsigned key = c_cls_key[class_of(s.g, x, y, z)];
+  const unsigned ol = s.olow[v];
+  const unsigned ostar = SIDE == 2 ? key & ~ol : ol;
+  const int* Lo = SIDE == 2 ? s.oDS : s.oAS;
+#pragma unroll
+  for (int d = 0; d < kNbr; d++) {
+    if (!((ostar >> d) & 1)) continue;
+    const int e = s.ext_id[Lo[v + s.g.off[d]]];
+    if (e < 0 || (e & 1) != (SIDE == 1)) continue;
+    const int j = slot_index(s, e);
+    if (atomicExch(&s.slstamp[j], s.iepoch) != s.iepoch)
+      s.dslot[atomicAdd(&s.ctr[C_DSLOT], 1)] = j;
+  }
+}
+
 // One side for one saddle: the list entry i of S2 (SIDE 2) or S1 (SIDE 1).
+// In kModeDec the changed sides are listed (cln2 / cln1) for the next
+// incremental pass; with inc, a side whose gather key changes queues its
+// slots for a new gather.
 template <int SIDE>
 __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double* f, const int* L,
                                              const uint16_t* flow, int* F, int* La, int i,
                                              bool act, const int* s_off, int2* s_pairs,
-                                             int* s_np, int2* pairs, long long* key_slot) {
+                                             int* s_np, int2* pairs, long long* key_slot,
+                                             int inc) {
   const Geo& g = s.g;
   const int dec = mode == kModeDec;
   int v = 0;
@@ -922,13 +958,21 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
     // same star and the same root at every star neighbour: the root set is
     // the original one, gathered per extremum instead of pushed (its key at
     // its list position; a changed saddle leaves the empty key there)
-    if (act)
-      (SIDE == 2 ? s.pv2 : s.pv1)[i] =
+    if (act) {
+      long long* pv = SIDE == 2 ? s.pv2 : s.pv1;
+      const long long nk =
           changed ? (SIDE == 2 ? (long long)0x8000000000000000ULL : 0x7fffffffffffffffLL)
                   : okey_c(fs);
+      if (inc && pv[i] != nk) regather_slots<SIDE>(s, v);
+      pv[i] = nk;
+    }
   }
   emit_side<SIDE>(s, v, okey_c(fs), E, key_slot, s_pairs, s_np, pairs);
-  if (dec) wave_append(s.act0, &s.ctr[C_ACT0], cand != 0, make_int2(2 * v + (SIDE == 1), (int)cand));
+  if (dec) {
+    wave_append(s.act0, &s.ctr[C_ACT0], cand != 0, make_int2(2 * v + (SIDE == 1), (int)cand));
+    wave_append(SIDE == 2 ? s.cln2 : s.cln1, &s.ctr[SIDE == 2 ? C_CLN2 : C_CLN1], act && changed,
+                i);
+  }
 }
 
 // Blocks [0, nb2) take S2 entries, the rest S1 entries (block order remapped
@@ -952,11 +996,11 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const doubl
   if (b < nb2) {
     const int i = b * 256 + tid;
     saddle_entry<2>(s, mode, f, L2, flow, F2, La2, i, i < s.n2, s_off, s_pairs, &s_np, pairs,
-                    kmax);
+                    kmax, 0);
   } else {
     const int i = (b - nb2) * 256 + tid;
     saddle_entry<1>(s, mode, f, L1, flow, F1, La1, i, i < s.n1, s_off, s_pairs, &s_np, pairs,
-                    kmin);
+                    kmin, 0);
   }
   __syncthreads();
   const int np = s_np < kPairBuf ? s_np : kPairBuf;
@@ -1161,10 +1205,12 @@ __global__ void __launch_bounds__(1024) k_scan_single(int* a, int n) {
 
 // Best (value, id) of two vertex lists under f: out[block] = the highest of
 // inmax's grid-stride share and the lowest of inmin's (kNone entries skipped).
-// Run over MX / MN, then over the per-block results with one block.
+// Run over MX / MN, then over the per-block results with one block.  With a
+// gate, nothing is done while *gate is 0 (no listed extremum changed value).
 __global__ void __launch_bounds__(256) k_ext_reduce(const int* inmax, int nmax, const int* inmin,
                                                     int nmin, const double* f, int* outmax,
-                                                    int* outmin) {
+                                                    int* outmin, const int* gate) {
+  if (gate && *gate == 0) return;
   int cmax = kNone, cmin = kNone;
   double fmax = 0, fmin = 0;
   GRID_STRIDE(i, nmax) {
@@ -1216,8 +1262,8 @@ __global__ void k_ext_index(St s) {
   GRID_STRIDE(i, s.nmin) s.ext_id[s.MN[i]] = 2 * i + 1;
 }
 
-// Empty slot keys and slots for a saddle pass, and its two list counters
-// (read by the launches before it, appended to by the ones after).
+// Empty slot keys and slots for a saddle pass, and its list counters (read
+// by the launches before it, appended to by the ones after).
 __global__ void k_slot_init(St s, long long* kmax, int* smax, long long* kmin, int* smin) {
   GRID_STRIDE(i, s.nmax) {
     kmax[i] = (long long)0x8000000000000000ULL;
@@ -1230,11 +1276,381 @@ __global__ void k_slot_init(St s, long long* kmax, int* smax, long long* kmin, i
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     s.ctr[C_ACT0] = 0;
     s.ctr[C_PAIRS] = 0;
+    s.ctr[C_CLN2] = 0;
+    s.ctr[C_CLN1] = 0;
+    s.ctr[C_DSLOT] = 0;
   }
 }
 
 __global__ void k_list_fill64(const int* list, int n, long long* arr, long long val) {
   GRID_STRIDE(i, n) arr[list[i]] = val;
+}
+
+// ------------------------------------------------ incremental CP pass ----
+// A CP pass of decp after few edits recomputes only what the edits can
+// reach, and leaves the state the full pass (label_pass + slot_pass) would:
+//  1. k_inc_ptrs: lower-star masks and steepest directions of the changed
+//     vertices and their neighbours, the only vertices whose star or
+//     pointers can differ.
+//  2. k_inc_taint + k_inc_walk: the new root of every vertex whose pointer
+//     changed, by walking its new path up to a vertex whose old root is not
+//     the old root of any changed pointer: no changed pointer lies on that
+//     vertex's path, so its old root stands.
+//  3. k_inc_prop: every other vertex whose root changes has a path that
+//     enters a changed pointer whose root changed, and takes that root;
+//     these are found breadth-first over the pointer tree (the children of
+//     u are the neighbours whose pointer names u), stopping at changed
+//     pointers, which carry their own root.
+//  4. k_inc_mark / k_inc_mark_csr: the saddle sides whose inputs changed --
+//     their star (lower mask), their value, the root of a star neighbour, the
+//     value of one of their roots -- and every side that differed from the
+//     original in the last pass (slot pushes and wrong-neighbour candidates
+//     are rebuilt each pass).
+//  5. k_inc_saddles: those sides through saddle_entry; k_inc_pull: the slots
+//     whose gathered keys changed.
+
+// Saddle positions per vertex (-1 elsewhere, filled first).
+__global__ void k_spos(St s) {
+  GRID_STRIDE(i, s.n2) s.spos2[s.S2[i]] = i;
+  GRID_STRIDE(i, s.n1) s.spos1[s.S1[i]] = i;
+}
+
+// One lane per (changed vertex, candidate): candidate 0 is the vertex,
+// 1 + d its neighbour in direction d; each vertex once per epoch.  New lower
+// masks and steepest directions (the order of k_labels_tile); lists the
+// vertices whose mask (al), up pointer (cu) or down pointer (cd) changed.
+__global__ void __launch_bounds__(256) k_inc_ptrs(St s, int epoch) {
+  STAGED_LIST(la, int, 512, s.al, &s.ctr[C_AL]);
+  STAGED_LIST(lu, int, 512, s.cu, &s.ctr[C_CU]);
+  STAGED_LIST(ld, int, 512, s.cd, &s.ctr[C_CD]);
+  const long long n = (long long)s.ctr[C_LCHG] * (kNbr + 1);
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(k / (kNbr + 1)), j = (int)(k - (long long)i * (kNbr + 1));
+    int u = s.lchg[i];
+    int x, y, z;
+    coords(s.g, u, x, y, z);
+    int cls = class_of(s.g, x, y, z);
+    if (j == 0) {
+      if (s.ext_id[u] >= 0) s.ctr[C_EXTD] = 1;
+    } else {
+      if (!((c_cls_key[cls] >> (j - 1)) & 1)) continue;
+      u += s.g.off[j - 1];
+      cls = class_of(s.g, x + c_dx[j - 1], y + c_dy[j - 1], z + c_dz[j - 1]);
+    }
+    if (atomicExch(&s.stamp[u], epoch) == epoch) continue;
+    const int key = c_cls_key[cls];
+    const double c = s.decp[u];
+    unsigned lm = 0;
+    int up = u, dn = u, du = -1, dd = -1;
+    double fu = c, fd = c;
+#pragma unroll
+    for (int d = 0; d < kNbr; d++) {
+      if (!((key >> d) & 1)) continue;
+      const int r = u + s.g.off[d];
+      const double fr = s.decp[r];
+      if (fr < c || (fr == c && r < u)) lm |= 1u << d;
+      if (fu < fr || (fu == fr && up < r)) { up = r; fu = fr; du = d; }
+      if (fd > fr || (fd == fr && dn > r)) { dn = r; fd = fr; dd = d; }
+    }
+    const unsigned nd = (unsigned)(du + 1) | (unsigned)(dd + 1) << 4;
+    const unsigned od = s.pdir[u];
+    if (lm != s.dlow[u]) {
+      s.dlow[u] = (uint16_t)lm;
+      la.push(u);
+    }
+    unsigned cf = 0;
+    if ((nd ^ od) & 15u) {
+      cf |= 1;
+      lu.push(u);
+    }
+    if ((nd ^ od) >> 4) {
+      cf |= 2;
+      ld.push(u);
+    }
+    if (cf) {
+      s.pdir[u] = (uint8_t)nd;
+      s.cflag[u] = (uint8_t)cf;
+    }
+  }
+  la.flush();
+  lu.flush();
+  ld.flush();
+}
+
+// The old roots of the changed pointers (kept in ou / od, flagged in tu / td).
+__global__ void k_inc_taint(St s) {
+  const int nu = s.ctr[C_CU], nd = s.ctr[C_CD];
+  GRID_STRIDE(k, nu) {
+    const int r = s.dDS[s.cu[k]];
+    s.ou[k] = r;
+    s.tu[r] = 1;
+  }
+  GRID_STRIDE(k, nd) {
+    const int r = s.dAS[s.cd[k]];
+    s.od[k] = r;
+    s.td[r] = 1;
+  }
+}
+
+// New root of each changed pointer (nu / nd).  A path strictly increases
+// (decreases) in (value, id), so it ends within N steps.
+__global__ void k_inc_walk(St s) {
+  const int nu = s.ctr[C_CU], n = nu + s.ctr[C_CD];
+  GRID_STRIDE(k, n) {
+    const bool up = k < nu;
+    const int c = up ? s.cu[k] : s.cd[k - nu];
+    const int* R = up ? s.dDS : s.dAS;
+    const uint8_t* T = up ? s.tu : s.td;
+    const int sh = up ? 0 : 4;
+    int a = c, root = -1;
+    for (int step = 0; step < s.g.N && root < 0; step++) {
+      const int nib = (s.pdir[a] >> sh) & 15;
+      if (nib == 0) {
+        root = a;
+        break;
+      }
+      a += s.g.off[nib - 1];
+      const int ra = R[a];
+      if (!T[ra]) root = ra;
+    }
+    if (root < 0) {
+      s.ctr[C_IBAIL] = 1;
+      root = a;
+    }
+    if (up) s.nu[k] = root;
+    else s.nd[k - nu] = root;
+  }
+}
+
+// One block: the changed pointers take their new roots, then the vertices
+// below each pointer whose root changed take it too, level by level (a
+// vertex has one parent, so none is queued twice).  qu / qd list every
+// vertex whose root changed.
+__global__ void __launch_bounds__(1024) k_inc_prop(St s, int qcap) {
+  __shared__ int qn;
+  const int tid = threadIdx.x;
+  for (int side = 0; side < 2; side++) {
+    const bool up = side == 0;
+    int* R = up ? s.dDS : s.dAS;
+    const int* C = up ? s.cu : s.cd;
+    const int* NR = up ? s.nu : s.nd;
+    const int* OR = up ? s.ou : s.od;
+    int* Q = up ? s.qu : s.qd;
+    const int nc = s.ctr[up ? C_CU : C_CD];
+    const int sh = up ? 0 : 4;
+    const unsigned cbit = up ? 1u : 2u;
+    if (tid == 0) qn = 0;
+    __syncthreads();
+    for (int k = tid; k < nc; k += 1024) {
+      const int c = C[k], r = NR[k];
+      R[c] = r;
+      if (r != OR[k]) {
+        const int q = atomicAdd(&qn, 1);
+        if (q < qcap) Q[q] = c;
+      }
+    }
+    __syncthreads();
+    int lo = 0, hi = min(qn, qcap);
+    __syncthreads();
+    while (lo < hi) {
+      for (int k = lo + tid; k < hi; k += 1024) {
+        const int u = Q[k];
+        const int root = R[u];
+        int x, y, z;
+        coords(s.g, u, x, y, z);
+        const int key = c_cls_key[class_of(s.g, x, y, z)];
+        unsigned pd[kNbr];
+#pragma unroll
+        for (int d = 0; d < kNbr; d++)  // all loads issued before any is used
+          pd[d] = ((key >> d) & 1) ? s.pdir[u + s.g.off[d]] : 0u;
+#pragma unroll
+        for (int d = 0; d < kNbr; d++) {
+          if ((int)((pd[d] >> sh) & 15) != (d ^ 1) + 1) continue;
+          const int nb = u + s.g.off[d];
+          if (s.cflag[nb] & cbit) continue;
+          R[nb] = root;
+          const int q = atomicAdd(&qn, 1);
+          if (q < qcap) Q[q] = nb;
+        }
+      }
+      __syncthreads();
+      lo = hi;
+      hi = min(qn, qcap);
+      __syncthreads();
+    }
+    if (tid == 0) {
+      s.ctr[up ? C_QU : C_QD] = min(qn, qcap);
+      if (qn > qcap) s.ctr[C_IBAIL] = 1;
+    }
+    __syncthreads();
+  }
+}
+
+// Queues side q2 of S2 (two) or S1 for the saddle recompute, once per pass.
+template <class L>
+__device__ __forceinline__ void mark_side(const St& s, int q2, bool two, L& l2, L& l1) {
+  const int q = two ? q2 : s.n2 + q2;
+  if (atomicExch(&s.sstamp[q], s.iepoch) == s.iepoch) return;
+  if (two) l2.push(q2);
+  else l1.push(q2);
+}
+
+// Sides of saddles whose star or value changed (al, lchg), of saddles next to
+// a vertex whose root changed (qu: 2-saddles read up roots, qd: 1-saddles
+// down roots), and the changed sides of the last pass (cl2, cl1).
+__global__ void __launch_bounds__(256) k_inc_mark(St s) {
+  if (s.ctr[C_IBAIL]) return;
+  STAGED_LIST(l2, int, 1024, s.dsl2, &s.ctr[C_DSL2]);
+  STAGED_LIST(l1, int, 1024, s.dsl1, &s.ctr[C_DSL1]);
+  const long long nal = s.ctr[C_AL], nch = s.ctr[C_LCHG], nqu = s.ctr[C_QU],
+                  nqd = s.ctr[C_QD], ncl2 = s.ctr[C_CL2], ncl1 = s.ctr[C_CL1];
+  const long long n = nal + nch + nqu + nqd + ncl2 + ncl1;
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    long long t = k;
+    if (t < nal + nch) {
+      const int u = t < nal ? s.al[t] : s.lchg[t - nal];
+      const int p2 = s.spos2[u], p1 = s.spos1[u];
+      if (p2 >= 0) mark_side(s, p2, true, l2, l1);
+      if (p1 >= 0) mark_side(s, p1, false, l2, l1);
+      continue;
+    }
+    t -= nal + nch;
+    if (t < nqu + nqd) {
+      const bool up = t < nqu;
+      const int u = up ? s.qu[t] : s.qd[t - nqu];
+      const int* P = up ? s.spos2 : s.spos1;
+      int x, y, z;
+      coords(s.g, u, x, y, z);
+      const int key = c_cls_key[class_of(s.g, x, y, z)];
+      int p[kNbr];
+#pragma unroll
+      for (int d = 0; d < kNbr; d++) p[d] = ((key >> d) & 1) ? P[u + s.g.off[d]] : -1;
+#pragma unroll
+      for (int d = 0; d < kNbr; d++)
+        if (p[d] >= 0) mark_side(s, p[d], up, l2, l1);
+      continue;
+    }
+    t -= nqu + nqd;
+    if (t < ncl2) mark_side(s, s.cl2[t], true, l2, l1);
+    else mark_side(s, s.cl1[t - ncl2], false, l2, l1);
+  }
+  l2.flush();
+  l1.flush();
+}
+
+// One wavefront per changed vertex that is a listed extremum and still a
+// root of decp: the saddles adjacent to it in the original (the slot's
+// extremum->saddle index) compare its value, so their sides are queued.  The
+// other saddles with this root differ from the original and are queued as
+// last pass's changed sides or by a root change next to them.
+__global__ void __launch_bounds__(256) k_inc_mark_csr(St s) {
+  if (s.ctr[C_IBAIL]) return;
+  STAGED_LIST(l2, int, 1024, s.dsl2, &s.ctr[C_DSL2]);
+  STAGED_LIST(l1, int, 1024, s.dsl1, &s.ctr[C_DSL1]);
+  const int n = s.ctr[C_LCHG];
+  const int lane = threadIdx.x & 63;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n; w += nw) {
+    const int r = s.lchg[w];
+    const int e = s.ext_id[r];
+    if (e < 0) continue;
+    const bool mx = (e & 1) == 0;
+    if ((mx ? s.dDS[r] : s.dAS[r]) != r) continue;
+    const int j = slot_index(s, e);
+    const int k1 = s.csr_off[j + 1];
+    for (int k = s.csr_off[j] + lane; k < k1; k += 64) mark_side(s, s.csr[k], mx, l2, l1);
+  }
+  l2.flush();
+  l1.flush();
+}
+
+// The queued sides through saddle_entry (kModeDec, with slot re-gather
+// marking).  Every lane of a block walks the same rounds, so the wavefront
+// collectives inside saddle_entry see whole wavefronts.
+__global__ void __launch_bounds__(256) k_inc_saddles(St s, const double* f, int2* pairs,
+                                                     long long* kmax, long long* kmin) {
+  __shared__ int2 s_pairs[kPairBuf];
+  __shared__ int s_off[kNbr];
+  __shared__ int s_np, s_pbase;
+  const int tid = threadIdx.x;
+  if (tid < kNbr) s_off[tid] = s.g.off[tid];
+  if (tid == 0) s_np = 0;
+  __syncthreads();
+  const int n2 = s.ctr[C_DSL2], n1 = s.ctr[C_DSL1];
+  const int r2 = (n2 + 255) & ~255, r1 = (n1 + 255) & ~255;
+  for (int base = blockIdx.x * 256; base < r2 + r1; base += gridDim.x * 256) {
+    if (base < r2) {
+      const int k = base + tid;
+      const bool act = k < n2;
+      saddle_entry<2>(s, kModeDec, f, s.dDS, s.dlow, s.df2, s.dl2, act ? s.dsl2[k] : 0, act,
+                      s_off, s_pairs, &s_np, pairs, kmax, 1);
+    } else {
+      const int k = base - r2 + tid;
+      const bool act = k < n1;
+      saddle_entry<1>(s, kModeDec, f, s.dAS, s.dlow, s.df1, s.dl1, act ? s.dsl1[k] : 0, act,
+                      s_off, s_pairs, &s_np, pairs, kmin, 1);
+    }
+  }
+  __syncthreads();
+  const int np = s_np < kPairBuf ? s_np : kPairBuf;
+  if (tid == 0) s_pbase = np ? atomicAdd(&s.ctr[C_PAIRS], np) : 0;
+  __syncthreads();
+  for (int k = tid; k < np; k += 256) pairs[s_pbase + k] = s_pairs[k];
+}
+
+// The queued slots' gathers (one wavefront per slot).
+__global__ void __launch_bounds__(256) k_inc_pull(St s, long long* pkey, int* pid) {
+  const int n = s.ctr[C_DSLOT];
+  const int lane = threadIdx.x & 63;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n; w += nw) {
+    const int j = s.dslot[w];
+    const bool mx = j < s.nmax;
+    long long bk = 0;
+    int bp = -1;
+    pull_range(s, mx, s.csr_off[j] + lane, s.csr_off[j + 1], 64, bk, bp);
+    pull_group(mx, 64, bk, bp);
+    if (lane == 0) pull_store(s, j, mx, bk, bp, pkey, pid);
+  }
+}
+
+// End of every CP pass of decp: clears the pass's per-vertex flags through
+// its lists.
+__global__ void k_inc_clear(St s) {
+  const int nu = s.ctr[C_CU], nd = s.ctr[C_CD], nl = s.ctr[C_LCHG];
+  GRID_STRIDE(k, nu) {
+    s.cflag[s.cu[k]] = 0;
+    s.tu[s.ou[k]] = 0;
+  }
+  GRID_STRIDE(k, nd) {
+    s.cflag[s.cd[k]] = 0;
+    s.td[s.od[k]] = 0;
+  }
+  GRID_STRIDE(k, nl) s.lflag[s.lchg[k]] = 0;
+}
+
+__global__ void k_ctr_copy2(int* ctr, int a, int from_a, int b, int from_b) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctr[a] = ctr[from_a];
+    ctr[b] = ctr[from_b];
+  }
+}
+
+__global__ void k_ctr_fill(int* ctr, int a, int b, int v) {
+  for (int i = a + (int)threadIdx.x; i < b; i += blockDim.x) ctr[i] = v;
+}
+
+// Number of differing bytes of two buffers (out[0], accumulated).
+__global__ void k_diff_count(const void* a, const void* b, long long nbytes, int* out) {
+  const uint8_t* x = (const uint8_t*)a;
+  const uint8_t* y = (const uint8_t*)b;
+  int c = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nbytes;
+       i += (long long)gridDim.x * blockDim.x)
+    c += x[i] != y[i];
+  if (c) atomicAdd(out, c);
 }
 
 // ------------------------------------------- wrong neighbours (greedy) ----

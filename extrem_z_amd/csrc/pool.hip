@@ -31,17 +31,19 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 // Device bytes of one correction of N voxels, in two parts.  fixed: the
-// per-voxel work arrays every correction sizes by N (209 B) plus the small
-// per-extremum arrays.  var: what depends on the data or the caller's buffers
-// -- the saddle pair records, slot indices and keys (80 B/voxel covers the
-// 0.43 saddle sides per voxel of white noise; smooth fields have fewer) --
-// and the staging of host buffers / float32 originals.
+// per-voxel work arrays every correction sizes by N (241 B, 32 of them the
+// change lists, pointers and flags of the incremental CP pass) plus the
+// small per-extremum arrays.  var: what depends on the data or the caller's
+// buffers -- the saddle pair records, side and slot lists, stamps, indices
+// and keys (88 B/voxel covers the 0.43 saddle sides per voxel of white noise;
+// smooth fields have fewer) -- and the staging of host buffers / float32
+// originals.
 struct Need {
   int64_t fixed, var;
 };
 Need working_set(const extremz_job& j) {
   const int64_t N = (int64_t)j.W * j.H * j.D;
-  Need n{209 * N + (1 << 20), 80 * N};
+  Need n{241 * N + (1 << 20), 88 * N};
   if (j.on_host || j.orig_dtype == EXTREMZ_DTYPE_F32) n.var += 8 * N;  // orig as float64
   if (j.on_host) n.var += 9 * N;                                       // decp + counts
   return n;

@@ -46,7 +46,7 @@ enum {
   EXTREMZ_STAT_N_SADDLE1,
   EXTREMZ_STAT_N_SADDLE2,
   EXTREMZ_STAT_LABEL_PASSES,    /* full-grid manifold label computations */
-  EXTREMZ_STAT_JUMP_ROUNDS,     /* reserved: 0 (labels are resolved in tiles, then chased) */
+  EXTREMZ_STAT_INC_PASSES,      /* CP passes of decp done incrementally (no full-grid pass) */
   EXTREMZ_STAT_HOST_SYNCS,      /* device->host counter reads */
   EXTREMZ_STAT_FRONTIER_ITEMS,  /* false-critical-point frontier entries */
   EXTREMZ_STAT_COUNT_ = 16
@@ -60,7 +60,7 @@ enum {
   EXTREMZ_K_CHECK_FULL,   /* full-grid false-critical-point scan */
   EXTREMZ_K_CHECK_INCR,   /* frontier / edited-neighbourhood re-check */
   EXTREMZ_K_LABEL_TILE,   /* steepest pointers + lower masks, resolved inside LDS tiles */
-  EXTREMZ_K_LABEL_JUMP,   /* reserved: no separate pointer-jumping round is launched */
+  EXTREMZ_K_INCREMENTAL,  /* incremental CP pass of decp: pointer, root and saddle updates */
   EXTREMZ_K_LABEL_CHASE,  /* full-grid walk of the remaining pointers to their roots */
   EXTREMZ_K_SADDLE_PASS,  /* saddle-list pass: triplet ends, candidates, changed saddles */
   EXTREMZ_K_SLOT_GATHER,  /* largest / smallest saddle per extremum (gather + merge + ties) */

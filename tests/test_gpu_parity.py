@@ -172,3 +172,30 @@ def test_fixed_step_rule_rejects_bad_q():
     with pytest.raises(ez().ExtremzError) as e:
         ez().correct(orig, decp, xi, fixed_step_q=255)
     assert e.value.code == -22
+
+
+INC_CASES = CASES + [((64, 64, 64), 12, "gauss", 1e-3), ((48, 40, 36), 13, "noise", 1e-2)]
+
+
+@pytest.mark.parametrize("shape,seed,kind,er", INC_CASES)
+def test_incremental_passes_equal_full_passes(shape, seed, kind, er, monkeypatch):
+    """With EXTREMZ_VERIFY_INC every CP pass of decp that runs incrementally is
+    followed inside the engine by the full pass on the same decp, and any
+    difference in labels, masks, pointers, triplet ends, gather keys, slots,
+    global extrema or the wrong-neighbour candidate count fails the call.  The result equals the run
+    with full passes only, and the oracle."""
+    orig, decp, xi = make_field(shape, seed, kind, er)
+    monkeypatch.setenv("EXTREMZ_VERIFY_INC", "1")
+    f1, c1, s1 = ez().correct(orig, decp, xi)
+    assert s1["status"] == 0
+    monkeypatch.setenv("EXTREMZ_VERIFY_INC", "0")
+    monkeypatch.setenv("EXTREMZ_FULL_PASSES", "1")
+    f2, c2, s2 = ez().correct(orig, decp, xi)
+    assert s2["inc_passes"] == 0
+    assert f1.tobytes() == f2.tobytes() and np.array_equal(c1, c2)
+    assert {k: s1[k] for k in COUNTERS} == {k: s2[k] for k in COUNTERS}
+    if s2["label_passes"] > 2:
+        assert s1["inc_passes"] > 0
+    if shape[0] * shape[1] * shape[2] <= 40 ** 3:
+        f0, c0, _ = oracle.correct(orig, decp, xi)
+        assert f0.tobytes() == f1.tobytes()
