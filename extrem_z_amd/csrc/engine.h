@@ -58,6 +58,7 @@ enum Ctr {
   C_QD,           // down root changed,
   C_DSL2,         // 2-saddle / 1-saddle sides to recompute,
   C_DSL1,
+  C_NITEM,        // chunks of the extremum->saddle index to queue
   C_IBAIL,        // the incremental pass gave up (the full pass runs instead)
                   // (C_LCHG..C_IBAIL are cleared at the end of every CP pass of decp)
   C_DSLOT,        // slots to gather again
@@ -125,6 +126,7 @@ struct St {
   int *cl2, *cl1, *cln2, *cln1;   // changed sides of the last / this pass (positions)
   int* slstamp;                   // per slot index
   int* dslot;                     // slots to gather again
+  int4* citems;                   // {k0, k1, maxima?, -}: index chunks whose saddles are queued
   int iepoch;                     // stamp of this pass
 };
 
@@ -231,7 +233,9 @@ __global__ void k_inc_taint(St s);
 __global__ void k_inc_walk(St s);
 __global__ void k_inc_prop(St s, int qcap);
 __global__ void k_inc_mark(St s);
+__global__ void k_inc_csr_items(St s);
 __global__ void k_inc_mark_csr(St s);
+__global__ void k_inc_ext(St s);
 __global__ void k_inc_saddles(St s, const double* f, int2* pairs, long long* kmax,
                               long long* kmin);
 __global__ void k_inc_pull(St s, long long* pkey, int* pid);

@@ -152,7 +152,7 @@ struct extremz_ctx {
       lcnt, csr, csr_off, csr_cur, pid, cpos;
   ez::DevBuf<long long> pkey, ckey, pv2, pv1;
   ez::DevBuf<int2> longs;
-  ez::DevBuf<int4> chunks;
+  ez::DevBuf<int4> chunks, citems;
   ez::DevBuf<int> ext_id;
   ez::DevBuf<int> of2, ol2, of1, ol1, df2, dl2, df1, dl1, wnl2, wnl1, bmax2, bmin2;
   ez::DevBuf<int2> act0, actA, actB, pairs;
@@ -187,6 +187,7 @@ struct extremz_ctx {
     for (ez::DevBuf<uint16_t>* b : {&olow, &dlow}) f(*b);
     for (ez::DevBuf<int2>* b : {&longs, &act0, &actA, &actB, &pairs}) f(*b);
     f(chunks);
+    f(citems);
     for (ez::DevBuf<double>* b : {&orig_tmp, &decp_tmp}) f(*b);
   }
 };
@@ -404,6 +405,10 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
     c_->chunks.ensure(nch);
     c_->ckey.ensure(nch);
     c_->cpos.ensure(nch);
+    // the incremental pass cuts the index ranges of changed extrema into
+    // chunks of 1024: at most one per extremum plus one per 1024 entries
+    c_->citems.ensure((size_t)ns + total / 1024 + 2);
+    s_.citems = c_->citems.p;
     s_.chunks = c_->chunks.p;
     // after the buffers above exist: the launch captures their pointers
     set_ctr2(C_NLONG, C_NCHUNK, 0);
@@ -544,11 +549,13 @@ bool Run::cp_inc(long long nch) {
   const int b1 = blocks_for(15 * nch);
   tbegin(EXTREMZ_K_INCREMENTAL);
   k_inc_ptrs<<<b1, 256, 0, st()>>>(s_, epoch_);
+  k_inc_ext<<<blocks_for(nch), 256, 0, st()>>>(s_);
   k_inc_taint<<<b1, 256, 0, st()>>>(s_);
   k_inc_walk<<<b1, 256, 0, st()>>>(s_);
   k_inc_prop<<<1, 1024, 0, st()>>>(s_, s_.g.N);
   k_inc_mark<<<1024, 256, 0, st()>>>(s_);
-  k_inc_mark_csr<<<256, 256, 0, st()>>>(s_);
+  k_inc_csr_items<<<blocks_for(nch), 256, 0, st()>>>(s_);
+  k_inc_mark_csr<<<512, 256, 0, st()>>>(s_);
   tend();
   pull();
   if (h(C_IBAIL)) return false;
@@ -568,7 +575,7 @@ bool Run::cp_inc(long long nch) {
         s_, kmax, kmin, s_.dlarg, s_.dsmall, c_->pkey.p, c_->pid.p, 1);
     k_slot_ids<<<256, 256, 0, st()>>>(s_, c_->pairs.p, s_.decp, kmax, kmin, s_.dlarg,
                                       s_.dsmall);
-    // decp global maximum / minimum, when a listed extremum changed value
+    // decp global maximum / minimum, when the current one changed value
     k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, s_.decp, c_->bmax2.p,
                                         c_->bmin2.p, s_.ctr + C_EXTD);
     k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, 256, c_->bmin2.p, 256, s_.decp,

@@ -1331,8 +1331,8 @@ __global__ void __launch_bounds__(256) k_inc_ptrs(St s, int epoch) {
     int x, y, z;
     coords(s.g, u, x, y, z);
     int cls = class_of(s.g, x, y, z);
-    if (j == 0) {
-      if (s.ext_id[u] >= 0) s.ctr[C_EXTD] = 1;
+    if (j == 0) {  // the global extremum of decp changed value: k_ext_reduce runs
+      if (u == s.ctr[C_DGMAX] || u == s.ctr[C_DGMIN]) s.ctr[C_EXTD] = 1;
     } else {
       if (!((c_cls_key[cls] >> (j - 1)) & 1)) continue;
       u += s.g.off[j - 1];
@@ -1423,10 +1423,30 @@ __global__ void k_inc_walk(St s) {
   }
 }
 
+// Appends v (when pred) to the list Q whose length is the LDS counter qn, with
+// one LDS atomic per wavefront; all lanes of the wave must call.
+__device__ __forceinline__ void wave_push(int* Q, int* qn, int qcap, int cnt,
+                                          const int (&v)[kNbr]) {
+  int pre = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(pre, o);
+    if (__lane_id() >= o) pre += y;
+  }
+  const int total = __shfl(pre, 63);
+  if (total == 0) return;
+  int base = 0;
+  if (__lane_id() == 63) base = atomicAdd(qn, total);
+  base = __shfl(base, 63) + pre - cnt;
+  for (int j = 0; j < cnt; j++)
+    if (base + j < qcap) Q[base + j] = v[j];
+}
+
 // One block: the changed pointers take their new roots, then the vertices
 // below each pointer whose root changed take it too, level by level (a
 // vertex has one parent, so none is queued twice).  qu / qd list every
-// vertex whose root changed.
+// vertex whose root changed.  Every lane of the block walks the same rounds
+// (the queue appends are wavefront collectives).
 __global__ void __launch_bounds__(1024) k_inc_prop(St s, int qcap) {
   __shared__ int qn;
   const int tid = threadIdx.x;
@@ -1442,37 +1462,51 @@ __global__ void __launch_bounds__(1024) k_inc_prop(St s, int qcap) {
     const unsigned cbit = up ? 1u : 2u;
     if (tid == 0) qn = 0;
     __syncthreads();
-    for (int k = tid; k < nc; k += 1024) {
-      const int c = C[k], r = NR[k];
-      R[c] = r;
-      if (r != OR[k]) {
-        const int q = atomicAdd(&qn, 1);
-        if (q < qcap) Q[q] = c;
+    for (int k0 = 0; k0 < nc; k0 += 1024) {
+      const int k = k0 + tid;
+      int v[kNbr];
+      int cnt = 0;
+      if (k < nc) {
+        const int c = C[k], r = NR[k];
+        R[c] = r;
+        if (r != OR[k]) v[cnt++] = c;
       }
+      wave_push(Q, &qn, qcap, cnt, v);
     }
     __syncthreads();
     int lo = 0, hi = min(qn, qcap);
     __syncthreads();
     while (lo < hi) {
-      for (int k = lo + tid; k < hi; k += 1024) {
-        const int u = Q[k];
-        const int root = R[u];
-        int x, y, z;
-        coords(s.g, u, x, y, z);
-        const int key = c_cls_key[class_of(s.g, x, y, z)];
-        unsigned pd[kNbr];
+      for (int k0 = lo; k0 < hi; k0 += 1024) {
+        const int k = k0 + tid;
+        int v[kNbr];
+        int cnt = 0;
+        if (k < hi) {
+          const int u = Q[k];
+          const int root = R[u];
+          int x, y, z;
+          coords(s.g, u, x, y, z);
+          const int key = c_cls_key[class_of(s.g, x, y, z)];
+          unsigned pd[kNbr];
 #pragma unroll
-        for (int d = 0; d < kNbr; d++)  // all loads issued before any is used
-          pd[d] = ((key >> d) & 1) ? s.pdir[u + s.g.off[d]] : 0u;
+          for (int d = 0; d < kNbr; d++)  // all loads issued before any is used
+            pd[d] = ((key >> d) & 1) ? s.pdir[u + s.g.off[d]] : 0u;
+          unsigned kids = 0;
 #pragma unroll
-        for (int d = 0; d < kNbr; d++) {
-          if ((int)((pd[d] >> sh) & 15) != (d ^ 1) + 1) continue;
-          const int nb = u + s.g.off[d];
-          if (s.cflag[nb] & cbit) continue;
-          R[nb] = root;
-          const int q = atomicAdd(&qn, 1);
-          if (q < qcap) Q[q] = nb;
+          for (int d = 0; d < kNbr; d++)
+            if ((int)((pd[d] >> sh) & 15) == (d ^ 1) + 1) kids |= 1u << d;
+          unsigned cf[kNbr];
+#pragma unroll
+          for (int d = 0; d < kNbr; d++) cf[d] = ((kids >> d) & 1) ? s.cflag[u + s.g.off[d]] : 0u;
+#pragma unroll
+          for (int d = 0; d < kNbr; d++) {
+            if (!((kids >> d) & 1) || (cf[d] & cbit)) continue;
+            const int nb = u + s.g.off[d];
+            R[nb] = root;
+            v[cnt++] = nb;
+          }
         }
+        wave_push(Q, &qn, qcap, cnt, v);
       }
       __syncthreads();
       lo = hi;
@@ -1540,30 +1574,78 @@ __global__ void __launch_bounds__(256) k_inc_mark(St s) {
   l1.flush();
 }
 
-// One wavefront per changed vertex that is a listed extremum and still a
-// root of decp: the saddles adjacent to it in the original (the slot's
-// extremum->saddle index) compare its value, so their sides are queued.  The
-// other saddles with this root differ from the original and are queued as
-// last pass's changed sides or by a root change next to them.
+// The saddles adjacent in the original to a changed vertex that is a listed
+// extremum and still a root of decp compare its value, so their sides are
+// queued; the other saddles with this root differ from the original and are
+// queued as last pass's changed sides or by a root change next to them.
+// k_inc_csr_items cuts those extrema's index ranges into chunks of 1024,
+// k_inc_mark_csr queues one chunk per wavefront.
+__global__ void __launch_bounds__(256) k_inc_csr_items(St s) {
+  if (s.ctr[C_IBAIL]) return;
+  const int n = s.ctr[C_LCHG];
+  const int stride = gridDim.x * blockDim.x;
+  // uniform rounds: the chunk counts are summed per wavefront
+  for (int w0 = blockIdx.x * blockDim.x; w0 < n; w0 += stride) {
+    const int w = w0 + threadIdx.x;
+    int k0 = 0, k1 = 0, mx = 0;
+    if (w < n) {
+      const int r = s.lchg[w];
+      const int e = s.ext_id[r];
+      if (e >= 0) {
+        mx = (e & 1) == 0;
+        if ((mx ? s.dDS[r] : s.dAS[r]) == r) {
+          const int j = slot_index(s, e);
+          k0 = s.csr_off[j];
+          k1 = s.csr_off[j + 1];
+        }
+      }
+    }
+    const int nch = k1 > k0 ? (k1 - k0 + 1023) >> 10 : 0;
+    int pre = nch;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(pre, o);
+      if (__lane_id() >= o) pre += y;
+    }
+    const int total = __shfl(pre, 63);
+    if (total == 0) continue;
+    int b = 0;
+    if (__lane_id() == 63) b = atomicAdd(&s.ctr[C_NITEM], total);
+    b = __shfl(b, 63) + pre - nch;
+    for (int c = 0; c < nch; c++)
+      s.citems[b + c] = make_int4(k0 + (c << 10), min(k1, k0 + ((c + 1) << 10)), mx, 0);
+  }
+}
+
 __global__ void __launch_bounds__(256) k_inc_mark_csr(St s) {
   if (s.ctr[C_IBAIL]) return;
   STAGED_LIST(l2, int, 1024, s.dsl2, &s.ctr[C_DSL2]);
   STAGED_LIST(l1, int, 1024, s.dsl1, &s.ctr[C_DSL1]);
-  const int n = s.ctr[C_LCHG];
+  const int n = s.ctr[C_NITEM];
   const int lane = threadIdx.x & 63;
   const int nw = (gridDim.x * blockDim.x) >> 6;
   for (int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n; w += nw) {
-    const int r = s.lchg[w];
-    const int e = s.ext_id[r];
-    if (e < 0) continue;
-    const bool mx = (e & 1) == 0;
-    if ((mx ? s.dDS[r] : s.dAS[r]) != r) continue;
-    const int j = slot_index(s, e);
-    const int k1 = s.csr_off[j + 1];
-    for (int k = s.csr_off[j] + lane; k < k1; k += 64) mark_side(s, s.csr[k], mx, l2, l1);
+    const int4 it = s.citems[w];
+    for (int k = it.x + lane; k < it.y; k += 64) mark_side(s, s.csr[k], it.z != 0, l2, l1);
   }
   l2.flush();
   l1.flush();
+}
+
+// A listed extremum other than the current global one changed value: the
+// global maximum / minimum of decp is the better of it and the current one.
+// (When the current one changed, k_inc_ptrs set C_EXTD and k_ext_reduce
+// recomputes both.)
+__global__ void k_inc_ext(St s) {
+  if (s.ctr[C_EXTD]) return;
+  const int n = s.ctr[C_LCHG];
+  GRID_STRIDE(w, n) {
+    const int u = s.lchg[w];
+    const int e = s.ext_id[u];
+    if (e < 0) continue;
+    if ((e & 1) == 0) cas_best(&s.ctr[C_DGMAX], u, s.decp, true);
+    else cas_best(&s.ctr[C_DGMIN], u, s.decp, false);
+  }
 }
 
 // The queued sides through saddle_entry (kModeDec, with slot re-gather
