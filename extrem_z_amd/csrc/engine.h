@@ -59,6 +59,11 @@ enum Ctr {
   C_DSL2,         // 2-saddle / 1-saddle sides to recompute,
   C_DSL1,
   C_NITEM,        // chunks of the extremum->saddle index to queue
+  C_QLU,          // current level [C_QLU, C_QHU) of the up-root propagation queue,
+  C_QHU,
+  C_QLD,          // ... of the down-root one
+  C_QHD,
+  C_SWEEP,        // blocks of a propagation sweep that finished
   C_IBAIL,        // the incremental pass gave up (the full pass runs instead)
                   // (C_LCHG..C_IBAIL are cleared at the end of every CP pass of decp)
   C_DSLOT,        // slots to gather again
@@ -118,7 +123,7 @@ struct St {
   uint8_t* cflag;                 // 1: up pointer changed in this pass, 2: down pointer
   uint8_t *tu, *td;               // old up / down roots of the changed pointers
   int *spos2, *spos1;             // list position of a vertex in S2 / S1, else -1
-  int *al, *cu, *cd, *ou, *od, *nu, *nd;  // pass lists (capacity icap)
+  int *al, *cu, *cd, *ou, *od;    // pass lists (capacity icap)
   int *qu, *qd;                   // vertices whose up / down root changed (capacity N)
   int icap;
   int* sstamp;                    // per side: S2 positions, then n2 + S1 positions
@@ -231,6 +236,7 @@ __global__ void k_spos(St s);
 __global__ void k_inc_ptrs(St s, int epoch);
 __global__ void k_inc_taint(St s);
 __global__ void k_inc_walk(St s);
+__global__ void k_inc_sweep(St s);
 __global__ void k_inc_prop(St s, int qcap);
 __global__ void k_inc_mark(St s);
 __global__ void k_inc_csr_items(St s);

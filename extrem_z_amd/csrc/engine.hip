@@ -262,6 +262,8 @@ class Run {
   // Largest share of changed vertices (1 / kIncDiv of the grid) taken
   // incrementally; more, and the full pass is faster.
   static constexpr long long kIncDiv = 256;
+  // Whole-grid levels of the root propagation before the one-block remainder.
+  static constexpr int kSweeps = 3, kSweepBlocks = 256;
   static int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     return (int)std::max<long long>(1, std::min<long long>(b, 2048));
@@ -350,7 +352,7 @@ void Run::allocate(long long N) {
     b->ensure(N);
   for (DevBuf<uint8_t>* b : {&c->pdir, &c->lflag, &c->cflag, &c->tu, &c->td}) b->ensure(N);
   for (DevBuf<int>* b : {&c->lchg, &c->spos2, &c->spos1, &c->qu, &c->qd}) b->ensure(N);
-  c->ilist.ensure(7 * inc_cap(N));
+  c->ilist.ensure(5 * inc_cap(N));
   c->bmax2.ensure(256);
   c->bmin2.ensure(256);
   c->act0.ensure(2 * N);
@@ -552,6 +554,10 @@ bool Run::cp_inc(long long nch) {
   k_inc_ext<<<blocks_for(nch), 256, 0, st()>>>(s_);
   k_inc_taint<<<b1, 256, 0, st()>>>(s_);
   k_inc_walk<<<b1, 256, 0, st()>>>(s_);
+  // the first levels of the root propagation over the whole grid, the rest in
+  // one block
+  k_ctr_copy2<<<1, 64, 0, st()>>>(s_.ctr, C_QHU, C_QU, C_QHD, C_QD);
+  for (int l = 0; l < kSweeps; l++) k_inc_sweep<<<kSweepBlocks, 256, 0, st()>>>(s_);
   k_inc_prop<<<1, 1024, 0, st()>>>(s_, s_.g.N);
   k_inc_mark<<<1024, 256, 0, st()>>>(s_);
   k_inc_csr_items<<<blocks_for(nch), 256, 0, st()>>>(s_);
@@ -1020,7 +1026,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
     const long long cap = inc_cap(N);
     s_.icap = (int)cap;
     int* il = c->ilist.p;
-    for (int** l : {&s_.al, &s_.cu, &s_.cd, &s_.ou, &s_.od, &s_.nu, &s_.nd}) {
+    for (int** l : {&s_.al, &s_.cu, &s_.cd, &s_.ou, &s_.od}) {
       *l = il;
       il += cap;
     }

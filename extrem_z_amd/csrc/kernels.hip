@@ -189,7 +189,7 @@ struct StagedList {
   T* glist;
   int* gcount;
   // Callable from divergent code: the active lanes of the wavefront share
-  // one LDS atomic.
+  // one LDS atomic (and, once the block's buffer is full, one global atomic).
   __device__ __forceinline__ void push(T v) {
     unsigned long long m = __ballot(1);
     int lane = __lane_id();
@@ -198,8 +198,16 @@ struct StagedList {
     if (lane == leader) base = atomicAdd(n, __popcll(m));
     base = __shfl(base, leader);
     int i = base + __popcll(m & ((1ull << lane) - 1ull));
-    if (i < K) buf[i] = v;
-    else glist[atomicAdd(gcount, 1)] = v;
+    if (i < K) {
+      buf[i] = v;
+      return;
+    }
+    unsigned long long o = __ballot(1);  // the lanes past the buffer
+    int ol = __ffsll((long long)o) - 1;
+    int ob = 0;
+    if (lane == ol) ob = atomicAdd(gcount, __popcll(o));
+    ob = __shfl(ob, ol);
+    glist[ob + __popcll(o & ((1ull << lane) - 1ull))] = v;
   }
   __device__ __forceinline__ void flush() {
     __syncthreads();
@@ -1294,8 +1302,9 @@ __global__ void k_list_fill64(const int* list, int n, long long* arr, long long 
 //     pointers can differ.
 //  2. k_inc_taint + k_inc_walk: the new root of every vertex whose pointer
 //     changed, by walking its new path up to a vertex whose old root is not
-//     the old root of any changed pointer: no changed pointer lies on that
-//     vertex's path, so its old root stands.
+//     the old root of any changed pointer (no changed pointer lies on that
+//     vertex's path, so its old root stands) or that is a changed pointer
+//     whose new root is known.
 //  3. k_inc_prop: every other vertex whose root changes has a path that
 //     enters a changed pointer whose root changed, and takes that root;
 //     these are found breadth-first over the pointer tree (the children of
@@ -1393,14 +1402,20 @@ __global__ void k_inc_taint(St s) {
   }
 }
 
-// New root of each changed pointer (nu / nd).  A path strictly increases
-// (decreases) in (value, id), so it ends within N steps.
-__global__ void k_inc_walk(St s) {
+// New root of each changed pointer, written to its root array; the pointers
+// whose root changed start the propagation queues (qu / qd).  A path strictly
+// increases (decreases) in (value, id), so it ends within N steps.  A root
+// array entry read on the way is either the old root (tainted when the
+// vertex's path holds a changed pointer, so the walk goes on) or the final new
+// root of a changed pointer, so either answer it gives is exact.
+__global__ void __launch_bounds__(256) k_inc_walk(St s) {
+  STAGED_LIST(su, int, 512, s.qu, &s.ctr[C_QU]);
+  STAGED_LIST(sd, int, 512, s.qd, &s.ctr[C_QD]);
   const int nu = s.ctr[C_CU], n = nu + s.ctr[C_CD];
   GRID_STRIDE(k, n) {
     const bool up = k < nu;
     const int c = up ? s.cu[k] : s.cd[k - nu];
-    const int* R = up ? s.dDS : s.dAS;
+    int* R = up ? s.dDS : s.dAS;
     const uint8_t* T = up ? s.tu : s.td;
     const int sh = up ? 0 : 4;
     int a = c, root = -1;
@@ -1418,15 +1433,20 @@ __global__ void k_inc_walk(St s) {
       s.ctr[C_IBAIL] = 1;
       root = a;
     }
-    if (up) s.nu[k] = root;
-    else s.nd[k - nu] = root;
+    R[c] = root;
+    if (root != (up ? s.ou[k] : s.od[k - nu])) {
+      if (up) su.push(c);
+      else sd.push(c);
+    }
   }
+  su.flush();
+  sd.flush();
 }
 
-// Appends v (when pred) to the list Q whose length is the LDS counter qn, with
-// one LDS atomic per wavefront; all lanes of the wave must call.
-__device__ __forceinline__ void wave_push(int* Q, int* qn, int qcap, int cnt,
-                                          const int (&v)[kNbr]) {
+// Wavefront-collective append of cnt entries per lane to the list Q whose
+// length is the LDS counter qn (one LDS atomic per wavefront); returns the
+// lane's first index.  All lanes of the wave must call.
+__device__ __forceinline__ int wave_reserve(int* qn, int cnt) {
   int pre = cnt;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -1434,79 +1454,122 @@ __device__ __forceinline__ void wave_push(int* Q, int* qn, int qcap, int cnt,
     if (__lane_id() >= o) pre += y;
   }
   const int total = __shfl(pre, 63);
-  if (total == 0) return;
   int base = 0;
-  if (__lane_id() == 63) base = atomicAdd(qn, total);
-  base = __shfl(base, 63) + pre - cnt;
-  for (int j = 0; j < cnt; j++)
-    if (base + j < qcap) Q[base + j] = v[j];
+  if (__lane_id() == 63 && total) base = atomicAdd(qn, total);
+  return __shfl(base, 63) + pre - cnt;
 }
 
-// One block: the changed pointers take their new roots, then the vertices
-// below each pointer whose root changed take it too, level by level (a
-// vertex has one parent, so none is queued twice).  qu / qd list every
-// vertex whose root changed.  Every lane of the block walks the same rounds
-// (the queue appends are wavefront collectives).
+// The children of u (neighbours whose pointer of this side names u) that are
+// not changed pointers themselves.
+__device__ __forceinline__ unsigned prop_children(const St& s, int u, int sh, unsigned cbit) {
+  int x, y, z;
+  coords(s.g, u, x, y, z);
+  const int key = c_cls_key[class_of(s.g, x, y, z)];
+  unsigned pd[kNbr];
+#pragma unroll
+  for (int d = 0; d < kNbr; d++)  // all loads issued before any is used
+    pd[d] = ((key >> d) & 1) ? s.pdir[u + s.g.off[d]] : 0u;
+  unsigned kids = 0;
+#pragma unroll
+  for (int d = 0; d < kNbr; d++)
+    if ((int)((pd[d] >> sh) & 15) == (d ^ 1) + 1) kids |= 1u << d;
+  unsigned cf[kNbr];
+#pragma unroll
+  for (int d = 0; d < kNbr; d++) cf[d] = ((kids >> d) & 1) ? s.cflag[u + s.g.off[d]] : 0u;
+#pragma unroll
+  for (int d = 0; d < kNbr; d++)
+    if (cf[d] & cbit) kids &= ~(1u << d);
+  return kids;
+}
+
+// The vertices below each changed pointer whose root changed take its new
+// root.  Each queued vertex already holds its final root, so the queue may be
+// worked in any order; it is worked level by level, a level being what the
+// queue held when the previous one was done: first by whole-grid sweeps (one
+// level each, the last block to finish opens the next level), then by
+// k_inc_prop in one block for the deep remainder.  A vertex has one parent,
+// so none is queued twice.
+__global__ void __launch_bounds__(256) k_inc_sweep(St s) {
+  const int lou = s.ctr[C_QLU], hiu = s.ctr[C_QHU], lod = s.ctr[C_QLD], hid = s.ctr[C_QHD];
+  const int nu = hiu - lou, n = nu + (hid - lod);
+  const int stride = gridDim.x * blockDim.x;
+  for (int k0 = blockIdx.x * blockDim.x; k0 < n; k0 += stride) {  // uniform per block
+    const int k = k0 + threadIdx.x;
+    const bool up = k < nu;  // lanes of one wave may differ: appends go per side
+    int u = 0, root = 0;
+    unsigned kids = 0;
+    if (k < n) {
+      u = up ? s.qu[lou + k] : s.qd[lod + k - nu];
+      root = (up ? s.dDS : s.dAS)[u];
+      kids = prop_children(s, u, up ? 0 : 4, up ? 1u : 2u);
+    }
+    int* R = up ? s.dDS : s.dAS;
+#pragma unroll
+    for (int d = 0; d < kNbr; d++)
+      if ((kids >> d) & 1) R[u + s.g.off[d]] = root;
+    const int cu = up ? __popc(kids) : 0, cd = up ? 0 : __popc(kids);
+    int bu = wave_reserve(&s.ctr[C_QU], cu);
+    int bd = wave_reserve(&s.ctr[C_QD], cd);
+    int* Q = up ? s.qu : s.qd;
+    int b = up ? bu : bd;
+#pragma unroll
+    for (int d = 0; d < kNbr; d++)
+      if ((kids >> d) & 1) {
+        if (b < s.g.N) Q[b] = u + s.g.off[d];
+        else s.ctr[C_IBAIL] = 1;
+        b++;
+      }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&s.ctr[C_SWEEP], 1) == (int)gridDim.x - 1) {  // the last block
+      __threadfence();
+      s.ctr[C_QLU] = hiu;
+      s.ctr[C_QHU] = __hip_atomic_load(&s.ctr[C_QU], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s.ctr[C_QLD] = hid;
+      s.ctr[C_QHD] = __hip_atomic_load(&s.ctr[C_QD], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s.ctr[C_SWEEP] = 0;
+    }
+  }
+}
+
+// The remainder of the propagation in one block, level by level from
+// [C_QLx, C_QHx) to an empty level.  Every lane of the block walks the same
+// rounds (the queue appends are wavefront collectives).  qu / qd end up
+// listing every vertex whose root changed.
 __global__ void __launch_bounds__(1024) k_inc_prop(St s, int qcap) {
   __shared__ int qn;
   const int tid = threadIdx.x;
   for (int side = 0; side < 2; side++) {
     const bool up = side == 0;
     int* R = up ? s.dDS : s.dAS;
-    const int* C = up ? s.cu : s.cd;
-    const int* NR = up ? s.nu : s.nd;
-    const int* OR = up ? s.ou : s.od;
     int* Q = up ? s.qu : s.qd;
-    const int nc = s.ctr[up ? C_CU : C_CD];
     const int sh = up ? 0 : 4;
     const unsigned cbit = up ? 1u : 2u;
-    if (tid == 0) qn = 0;
+    if (tid == 0) qn = s.ctr[up ? C_QU : C_QD];
     __syncthreads();
-    for (int k0 = 0; k0 < nc; k0 += 1024) {
-      const int k = k0 + tid;
-      int v[kNbr];
-      int cnt = 0;
-      if (k < nc) {
-        const int c = C[k], r = NR[k];
-        R[c] = r;
-        if (r != OR[k]) v[cnt++] = c;
-      }
-      wave_push(Q, &qn, qcap, cnt, v);
-    }
-    __syncthreads();
-    int lo = 0, hi = min(qn, qcap);
+    int lo = s.ctr[up ? C_QLU : C_QLD], hi = min(qn, qcap);
     __syncthreads();
     while (lo < hi) {
       for (int k0 = lo; k0 < hi; k0 += 1024) {
         const int k = k0 + tid;
-        int v[kNbr];
-        int cnt = 0;
+        int u = 0, root = 0;
+        unsigned kids = 0;
         if (k < hi) {
-          const int u = Q[k];
-          const int root = R[u];
-          int x, y, z;
-          coords(s.g, u, x, y, z);
-          const int key = c_cls_key[class_of(s.g, x, y, z)];
-          unsigned pd[kNbr];
-#pragma unroll
-          for (int d = 0; d < kNbr; d++)  // all loads issued before any is used
-            pd[d] = ((key >> d) & 1) ? s.pdir[u + s.g.off[d]] : 0u;
-          unsigned kids = 0;
-#pragma unroll
-          for (int d = 0; d < kNbr; d++)
-            if ((int)((pd[d] >> sh) & 15) == (d ^ 1) + 1) kids |= 1u << d;
-          unsigned cf[kNbr];
-#pragma unroll
-          for (int d = 0; d < kNbr; d++) cf[d] = ((kids >> d) & 1) ? s.cflag[u + s.g.off[d]] : 0u;
-#pragma unroll
-          for (int d = 0; d < kNbr; d++) {
-            if (!((kids >> d) & 1) || (cf[d] & cbit)) continue;
-            const int nb = u + s.g.off[d];
-            R[nb] = root;
-            v[cnt++] = nb;
-          }
+          u = Q[k];
+          root = R[u];
+          kids = prop_children(s, u, sh, cbit);
         }
-        wave_push(Q, &qn, qcap, cnt, v);
+        int b = wave_reserve(&qn, __popc(kids));
+#pragma unroll
+        for (int d = 0; d < kNbr; d++) {
+          if (!((kids >> d) & 1)) continue;
+          const int nb = u + s.g.off[d];
+          R[nb] = root;
+          if (b < qcap) Q[b] = nb;
+          b++;
+        }
       }
       __syncthreads();
       lo = hi;
