@@ -543,7 +543,7 @@ void Run::cp_full() {
 
 // The incremental CP pass (the kernels' section "incremental CP pass"):
 // false when it gave up (the caller runs the full pass).  The saddle part
-// falls back to the full saddle pass when a quarter of the sides are queued.
+// falls back to the full saddle pass when half of the sides are queued.
 bool Run::cp_inc(long long nch) {
   counts_[EXTREMZ_STAT_INC_PASSES]++;
   epoch_++;
@@ -566,7 +566,7 @@ bool Run::cp_inc(long long nch) {
   pull();
   if (h(C_IBAIL)) return false;
   const long long nd = (long long)h(C_DSL2) + h(C_DSL1);
-  if (4 * nd > (long long)s_.n2 + s_.n1) {
+  if (2 * nd > (long long)s_.n2 + s_.n1) {
     slot_pass(1, s_.decp, s_.dDS, s_.dAS, s_.dlow, s_.df2, s_.dl2, s_.df1, s_.dl1, s_.dlarg,
               s_.dsmall);
   } else {
