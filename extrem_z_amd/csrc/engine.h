@@ -15,6 +15,9 @@ namespace ez {
 
 constexpr int kNbr = 14;
 constexpr int kNone = -1;
+// Slot pairs of one 256-side chunk of a saddle launch: at most one per
+// distinct root of each side's star.
+constexpr int kChunkPairs = 256 * kNbr;
 
 // Device counters (one int array, read back with one copy).
 enum Ctr {
@@ -45,7 +48,7 @@ enum Ctr {
   C_FAIL,
   C_EDITED,
   C_ACT0,         // wrong-neighbour candidates of the last CP pass
-  C_PAIRS,        // (extremum, saddle) pairs of the slot pass
+  C_PCHUNK,       // chunk regions of slot pairs written by the last saddle launch
   C_NLONG,        // slots with long adjacency ranges
   C_NCHUNK,       // chunks of those ranges
   C_WN_BAIL,      // the one-block reservation solver left the list to the rounds
@@ -203,7 +206,7 @@ __global__ void k_f32_to_f64(const float* in, double* out, long long n);
 __global__ void k_check_finite(const double* a, long long n, int* ctr);
 __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* type,
                            uint16_t* lower);
-__global__ void k_check_full(St s, int* front, int fctr);
+__global__ void k_check_full(St s, int* front, int fctr, int ntiles);
 __global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int* front,
                              int fctr, int epoch);
 __global__ void k_clear_dirty(St s);
@@ -215,9 +218,11 @@ __global__ void k_chase(int n, int* ds, int* as);
 __global__ void k_list_fill(const int* list, int n, int* arr, int val);
 __global__ void k_saddle_list(St s, int mode, const double* f, const int* L2, const int* L1,
                               const uint16_t* flow, int* F2, int* La2, int* F1, int* La1,
-                              int2* pairs, long long* kmax, long long* kmin, int nb2);
-__global__ void k_slot_ids(St s, const int2* pairs, const double* f, const long long* kmax,
-                           const long long* kmin, int* slot_max, int* slot_min);
+                              int2* pairs, int* pcnt, long long* kmax, long long* kmin, int nb2,
+                              int nb);
+__global__ void k_slot_ids(St s, const int2* pairs, const int* pcnt, const double* f,
+                           const long long* kmax, const long long* kmin, int* slot_max,
+                           int* slot_min);
 __global__ void k_ext_index(St s);
 __global__ void k_slot_pull(St s, long long* pkey, int* pid);
 __global__ void k_slot_pull_long(St s, long long* ckey, int* cpos);
@@ -242,7 +247,7 @@ __global__ void k_inc_mark(St s);
 __global__ void k_inc_csr_items(St s);
 __global__ void k_inc_mark_csr(St s);
 __global__ void k_inc_ext(St s);
-__global__ void k_inc_saddles(St s, const double* f, int2* pairs, long long* kmax,
+__global__ void k_inc_saddles(St s, const double* f, int2* pairs, int* pcnt, long long* kmax,
                               long long* kmin);
 __global__ void k_inc_pull(St s, long long* pkey, int* pid);
 __global__ void k_inc_clear(St s);
@@ -259,7 +264,7 @@ __global__ void k_fixpath(St s);
 __global__ void k_wrong_index(St s, int side, int phase, int do_fix);
 __global__ void k_wrong_saddle(St s, int side, int do_fix);
 __global__ void k_lists_count(St s, int* cnt, int nb, int tiled);
-__global__ void k_lists_scan(St s, int* cnt, int nb);
+__global__ void k_lists_totals(St s, const int* cnt, int nb);
 __global__ void k_lists_write(St s, const int* cnt, int nb, int tiled);
 __global__ void k_presnap(St s);
 __global__ void k_argext(const double* f, int n, int* slot_min, int* slot_max);

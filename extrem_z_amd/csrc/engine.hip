@@ -161,7 +161,7 @@ struct extremz_ctx {
   ez::DevBuf<uint8_t> counts_tmp;
   // incremental CP passes of decp
   ez::DevBuf<uint8_t> pdir, lflag, cflag, tu, td;
-  ez::DevBuf<int> lchg, spos2, spos1, ilist, qu, qd, sstamp, slists, slstamp, dslot;
+  ez::DevBuf<int> lchg, spos2, spos1, ilist, qu, qd, sstamp, slists, slstamp, dslot, pcnt;
   ez::DevBuf<uint8_t> vsnap;  // snapshots of the self-check (EXTREMZ_VERIFY_INC)
   int* h_ctr = nullptr;
   std::vector<ez::EventPair> ev_pool;
@@ -178,7 +178,7 @@ struct extremz_ctx {
           &olarg, &osmall, &dlarg, &dsmall, &minsad2, &minsad1, &win_a, &win_b, &lcnt, &csr,
           &csr_off, &csr_cur, &pid, &cpos, &ext_id, &of2, &ol2, &of1, &ol1, &df2, &dl2, &df1,
           &dl1, &wnl2, &wnl1, &bmax2, &bmin2, &ctr, &lchg, &spos2, &spos1, &ilist, &qu, &qd,
-          &sstamp, &slists, &slstamp, &dslot})
+          &sstamp, &slists, &slstamp, &dslot, &pcnt})
       f(*b);
     for (ez::DevBuf<long long>* b : {&delta, &kmax, &kmin, &pkey, &ckey, &pv2, &pv1}) f(*b);
     for (ez::DevBuf<uint8_t>* b :
@@ -264,6 +264,9 @@ class Run {
   static constexpr long long kIncDiv = 256;
   // Whole-grid levels of the root propagation before the one-block remainder.
   static constexpr int kSweeps = 3, kSweepBlocks = 256;
+  // Blocks of a saddle launch (each takes a contiguous range of 256-side
+  // chunks).
+  static constexpr int kSaddleBlocks = 4096;
   static int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     return (int)std::max<long long>(1, std::min<long long>(b, 2048));
@@ -329,10 +332,20 @@ void Run::list_compaction() {
   const long long K =
       tiled ? (long long)((g.W + 31) / 32) * ((g.H + 31) / 32) * 1024LL * g.D : (long long)g.N;
   const int nb = (int)((K + 255) / 256);
-  c_->lcnt.ensure(4LL * nb);
-  k_lists_count<<<nb, 256, 0, st()>>>(s_, c_->lcnt.p, nb, tiled);
-  k_lists_scan<<<1, 1024, 0, st()>>>(s_, c_->lcnt.p, nb);
-  k_lists_write<<<nb, 256, 0, st()>>>(s_, c_->lcnt.p, nb, tiled);
+  // the 4 * nb block counts (plus a zero that receives the total) scanned as
+  // one array; block sums after them
+  const long long n1 = 4LL * nb + 1;
+  const int nblk = (int)((n1 + 1023) / 1024);
+  c_->lcnt.ensure(n1 + nblk + 1);
+  int* cnt = c_->lcnt.p;
+  int* sums = cnt + n1;
+  k_fill_i32<<<1, 64, 0, st()>>>(cnt + 4LL * nb, 0, 1);
+  k_lists_count<<<nb, 256, 0, st()>>>(s_, cnt, nb, tiled);
+  k_scan_block<<<nblk, 1024, 0, st()>>>(cnt, cnt, n1, sums);
+  k_scan_single<<<1, 1024, 0, st()>>>(sums, nblk);
+  k_scan_add<<<nblk, 1024, 0, st()>>>(cnt, n1, sums);
+  k_lists_totals<<<1, 64, 0, st()>>>(s_, cnt, nb);
+  k_lists_write<<<nb, 256, 0, st()>>>(s_, cnt, nb, tiled);
 }
 
 void Run::allocate(long long N) {
@@ -375,10 +388,14 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
   const int nb2 = (s_.n2 + 255) / 256, nb1 = (s_.n1 + 255) / 256;
   const int ns = s_.nmax + s_.nmin;
   auto saddle_pass = [&](int mode) {
-    if (!(nb2 + nb1)) return;
+    if (!(nb2 + nb1)) {
+      if (mode != kModeFill) set_ctr(C_PCHUNK, 0);
+      return;
+    }
     tbegin(EXTREMZ_K_SADDLE_PASS);
-    k_saddle_list<<<nb2 + nb1, 256, 0, st()>>>(s_, mode, f, L2, L1, flow, F2, La2, F1, La1,
-                                               c_->pairs.p, kmax, kmin, nb2);
+    k_saddle_list<<<std::min(nb2 + nb1, kSaddleBlocks), 256, 0, st()>>>(
+        s_, mode, f, L2, L1, flow, F2, La2, F1, La1, c_->pairs.p, c_->pcnt.p, kmax, kmin, nb2,
+        nb2 + nb1);
     tend();
   };
   if (!dec) {
@@ -429,8 +446,9 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
   }
   k_slot_merge<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(
       s_, kmax, kmin, smax, smin, c_->pkey.p, c_->pid.p, dec);
-  k_slot_ids<<<blocks_for(14LL * (s_.n2 + s_.n1)), 256, 0, st()>>>(s_, c_->pairs.p, f, kmax, kmin,
-                                                                   smax, smin);
+  // one wavefront per chunk region
+  k_slot_ids<<<blocks_for(64LL * (nb2 + nb1)), 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p, f,
+                                                               kmax, kmin, smax, smin);
   if (dec) {
     // decp global maximum / minimum among the listed extrema
     k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, f, c_->bmax2.p,
@@ -475,7 +493,8 @@ void Run::c_loops() {
   epoch_++;
   if (!clean_) {
     tbegin(EXTREMZ_K_CHECK_FULL);
-    k_check_full<<<row_blocks(), dim3(64, 4, 1), 0, st()>>>(s_, front_buf_, front_);
+    k_check_full<<<std::min(row_blocks(), 8192), dim3(64, 4, 1), 0, st()>>>(
+        s_, front_buf_, front_, row_blocks());
     tend();
   } else {
     tbegin(EXTREMZ_K_CHECK_INCR);
@@ -575,12 +594,14 @@ bool Run::cp_inc(long long nch) {
     tbegin(EXTREMZ_K_INCREMENTAL);
     k_slot_init<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(s_, kmax, s_.dlarg,
                                                                             kmin, s_.dsmall);
-    k_inc_saddles<<<blocks_for(nd), 256, 0, st()>>>(s_, s_.decp, c_->pairs.p, kmax, kmin);
+    const int nbi = (h(C_DSL2) + 255) / 256 + (h(C_DSL1) + 255) / 256;
+    k_inc_saddles<<<std::max(1, std::min(nbi, kSaddleBlocks)), 256, 0, st()>>>(
+        s_, s_.decp, c_->pairs.p, c_->pcnt.p, kmax, kmin);
     k_inc_pull<<<256, 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
     k_slot_merge<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(
         s_, kmax, kmin, s_.dlarg, s_.dsmall, c_->pkey.p, c_->pid.p, 1);
-    k_slot_ids<<<256, 256, 0, st()>>>(s_, c_->pairs.p, s_.decp, kmax, kmin, s_.dlarg,
-                                      s_.dsmall);
+    k_slot_ids<<<blocks_for(64LL * nbi), 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p, s_.decp,
+                                                         kmax, kmin, s_.dlarg, s_.dsmall);
     // decp global maximum / minimum, when the current one changed value
     k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, s_.decp, c_->bmax2.p,
                                         c_->bmin2.p, s_.ctr + C_EXTD);
@@ -1070,8 +1091,11 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   counts_[EXTREMZ_STAT_N_SADDLE1] = s_.n1;
   counts_[EXTREMZ_STAT_N_MAX] = s_.nmax;
   counts_[EXTREMZ_STAT_N_MIN] = s_.nmin;
-  // every saddle records at most one pair per distinct root of its star
-  c->pairs.ensure(14LL * (s_.n2 + s_.n1) + 1);
+  // every saddle records at most one pair per distinct root of its star, in
+  // the region of its 256-side chunk
+  const long long nchunk = ((long long)s_.n2 + s_.n1) / 256 + 4;
+  c->pairs.ensure(nchunk * kChunkPairs);
+  c->pcnt.ensure(nchunk);
   {  // incremental CP passes: saddle positions, side and slot lists and stamps
     const long long n21 = (long long)s_.n2 + s_.n1, ns = (long long)s_.nmax + s_.nmin;
     c->sstamp.ensure(n21 + 1);

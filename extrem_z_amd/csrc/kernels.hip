@@ -81,28 +81,6 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
-// Appends val when pred with one atomic per 256-thread block; every thread
-// of the block must call.
-template <class T>
-__device__ __forceinline__ void block_append(T* list, int* counter, bool pred, T val) {
-  __shared__ int s_cnt[4];
-  __shared__ int s_base;
-  int tid = threadIdx.x + blockDim.x * threadIdx.y;
-  int wave = tid >> 6, lane = tid & 63;
-  unsigned long long m = __ballot(pred);
-  if (lane == 0) s_cnt[wave] = __popcll(m);
-  __syncthreads();
-  if (tid == 0) {
-    int tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-    s_base = tot ? atomicAdd(counter, tot) : 0;
-  }
-  __syncthreads();
-  int off = s_base;
-  for (int w = 0; w < wave; w++) off += s_cnt[w];
-  if (pred) list[off + __popcll(m & ((1ull << lane) - 1ull))] = val;
-  __syncthreads();
-}
-
 // Lower-star mask (direction space), compact upper config and the vertex type
 // of classifyVertex_CUDA, src/ExtreM_cuda.cu:3796-3853.
 __device__ __forceinline__ int classify_at(const Geo& g, const uint8_t* lut,
@@ -308,15 +286,17 @@ struct RowPos {
   int x, y, z;
   bool in;
 };
-__device__ __forceinline__ RowPos row_pos(const Geo& g) {
+__device__ __forceinline__ RowPos row_pos_at(const Geo& g, int L) {
   const int nbx = (g.W + 63) / 64, nby = (g.H + 3) / 4;
-  const int L = xcd_remap(blockIdx.x, gridDim.x);
   RowPos p;
   p.x = (L % nbx) * 64 + threadIdx.x;
   p.y = ((L / nbx) % nby) * 4 + threadIdx.y;
   p.z = L / (nbx * nby);
   p.in = p.x < g.W && p.y < g.H;
   return p;
+}
+__device__ __forceinline__ RowPos row_pos(const Geo& g) {
+  return row_pos_at(g, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Full-grid classification of a field (classifyAllVertices,
@@ -334,21 +314,27 @@ __global__ void __launch_bounds__(256) k_classify(Geo g, const uint8_t* lut, con
 
 // Full-grid false-critical-point scan (the first get_false_criticle_points
 // of c_loops, src/ExtreM.cpp:1513-1519); appends to the frontier.
-__global__ void __launch_bounds__(256) k_check_full(St s, int* front, int fctr) {
-  RowPos p = row_pos(s.g);
-  int v = p.in ? p.x + s.g.W * (p.y + s.g.H * p.z) : 0;
-  bool mis = false;
-  if (p.in) {
-    int cls = class_of(s.g, p.x, p.y, p.z);
-    unsigned dl = lower_mask_at(s.g, s.decp, v, cls);
-    unsigned ol = s.olow[v];
+// Each block takes a contiguous range of the 64x4 row tiles (one frontier
+// append per block, not per tile: a shared counter taken by every tile's
+// block serialises).
+__global__ void __launch_bounds__(256) k_check_full(St s, int* front, int fctr, int ntiles) {
+  STAGED_LIST(fl, int, 2048, front, &s.ctr[fctr]);
+  const int per = (ntiles + gridDim.x - 1) / gridDim.x;
+  const int t0 = xcd_remap(blockIdx.x, gridDim.x) * per, t1 = min(ntiles, t0 + per);
+  for (int t = t0; t < t1; t++) {
+    const RowPos p = row_pos_at(s.g, t);
+    if (!p.in) continue;
+    const int v = p.x + s.g.W * (p.y + s.g.H * p.z);
+    const int cls = class_of(s.g, p.x, p.y, p.z);
+    const unsigned dl = lower_mask_at(s.g, s.decp, v, cls);
+    const unsigned ol = s.olow[v];
     if (dl != ol) {
-      int ot = s.otype[v];
-      int dt = type_of_mask(s.lut, cls, dl);
-      mis = dt != ot || is_saddle_type(ot);
+      const int ot = s.otype[v];
+      const int dt = type_of_mask(s.lut, cls, dl);
+      if (dt != ot || is_saddle_type(ot)) fl.push(v);
     }
   }
-  block_append(front, &s.ctr[fctr], mis, v);
+  fl.flush();
 }
 
 // Re-check after edits: every list entry (and, with nbrs, its 14 neighbours)
@@ -523,7 +509,6 @@ __global__ void __launch_bounds__(256) k_apply(St s) {
 constexpr int kLTX = 64, kLTY = 4, kLTZ = 4, kLTN = kLTX * kLTY * kLTZ;
 constexpr int kPX = kLTX + 2, kPY = kLTY + 2, kPN = kPX * kPY;
 constexpr int kPPer = (kPN + 255) / 256;
-constexpr int kPairBuf = 2048;  // (position, saddle) pairs staged in LDS per block
 
 __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int* ds, int* as,
                                                      uint16_t* lower, uint8_t* pdir) {
@@ -839,13 +824,13 @@ __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* 
 // Slot pass A for one side: raises (maxima, even positions) or lowers (minima,
 // odd positions) each distinct root's best saddle key.  A saddle whose key
 // is worse than the key it found there can never hold the slot; every other
-// one stages its (position, saddle) pair in LDS (one LDS atomic per
-// wavefront), so every saddle that ties the final key is recorded.  Called by
-// every lane of the block.
+// one records its (position, saddle) pair in the region of its 256-side chunk
+// (cpairs, kChunkPairs entries, count in the LDS counter s_np: one LDS
+// atomic per wavefront), so every saddle that ties the final key is
+// recorded.  Called by every lane of the block.
 template <int SIDE>
 __device__ __forceinline__ void emit_side(const St& s, int v, long long kv, int (&E)[kNbr],
-                                          long long* key_slot, int2* s_pairs, int* s_np,
-                                          int2* pairs) {
+                                          long long* key_slot, int2* cpairs, int* s_np) {
   long long old[kNbr];
 #pragma unroll
   for (int d = 0; d < kNbr; d++) {
@@ -875,9 +860,7 @@ __device__ __forceinline__ void emit_side(const St& s, int v, long long kv, int 
 #pragma unroll
   for (int d = 0; d < kNbr; d++) {
     if (E[d] >= 0) {
-      int2 pr = make_int2(E[d], v);
-      if (base < kPairBuf) s_pairs[base] = pr;
-      else pairs[atomicAdd(&s.ctr[C_PAIRS], 1)] = pr;
+      cpairs[base] = make_int2(E[d], v);
       base++;
     }
   }
@@ -911,15 +894,15 @@ __device__ __forceinline__ void regather_slots(const St& s, int v) {
 }
 
 // One side for one saddle: the list entry i of S2 (SIDE 2) or S1 (SIDE 1).
-// In kModeDec the changed sides are listed (cln2 / cln1) for the next
-// incremental pass; with inc, a side whose gather key changes queues its
-// slots for a new gather.
-template <int SIDE>
+// In kModeDec the wrong-neighbour candidates go to acl (act0) and the changed
+// sides to cl (cln2 / cln1, for the next incremental pass); with inc, a side
+// whose gather key changes queues its slots for a new gather.
+template <int SIDE, class LA, class LC>
 __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double* f, const int* L,
                                              const uint16_t* flow, int* F, int* La, int i,
-                                             bool act, const int* s_off, int2* s_pairs,
-                                             int* s_np, int2* pairs, long long* key_slot,
-                                             int inc) {
+                                             bool act, const int* s_off, int2* cpairs,
+                                             int* s_np, long long* key_slot, int inc, LA& acl,
+                                             LC& cl) {
   const Geo& g = s.g;
   const int dec = mode == kModeDec;
   int v = 0;
@@ -975,46 +958,56 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
       pv[i] = nk;
     }
   }
-  emit_side<SIDE>(s, v, okey_c(fs), E, key_slot, s_pairs, s_np, pairs);
+  emit_side<SIDE>(s, v, okey_c(fs), E, key_slot, cpairs, s_np);
   if (dec) {
-    wave_append(s.act0, &s.ctr[C_ACT0], cand != 0, make_int2(2 * v + (SIDE == 1), (int)cand));
-    wave_append(SIDE == 2 ? s.cln2 : s.cln1, &s.ctr[SIDE == 2 ? C_CLN2 : C_CLN1], act && changed,
-                i);
+    if (cand != 0) acl.push(make_int2(2 * v + (SIDE == 1), (int)cand));
+    if (act && changed) cl.push(i);
   }
 }
 
-// Blocks [0, nb2) take S2 entries, the rest S1 entries (block order remapped
-// so each XCD walks one contiguous range: list order is spatial order, so the
-// neighbour rows of a range stay in that XCD's L2).  Triplet ends F2 / La2 /
-// F1 / La1 are written at the list position.
+// The lists in chunks of 256 sides: chunks [0, nb2) are S2 entries, the rest
+// S1 entries.  Each block takes a contiguous range of chunks, and the block
+// order is remapped so each XCD walks one contiguous range (list order is
+// spatial order, so the neighbour rows of a range stay in that XCD's L2).
+// Triplet ends F2 / La2 / F1 / La1 are written at the list position, the
+// slot pairs of chunk c to its own region (count pcnt[c]), so no two blocks
+// share a counter for them; kModeFill records no pairs and leaves pcnt as
+// the kModeOrig pass wrote it.
 __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const double* f,
                                                      const int* L2, const int* L1,
                                                      const uint16_t* flow, int* F2, int* La2,
-                                                     int* F1, int* La1, int2* pairs,
-                                                     long long* kmax, long long* kmin, int nb2) {
-  __shared__ int2 s_pairs[kPairBuf];
+                                                     int* F1, int* La1, int2* pairs, int* pcnt,
+                                                     long long* kmax, long long* kmin, int nb2,
+                                                     int nb) {
   __shared__ int s_off[kNbr];
-  __shared__ int s_np, s_pbase;
-  const Geo& g = s.g;
+  __shared__ int s_np;
+  STAGED_LIST(sa, int2, 1024, s.act0, &s.ctr[C_ACT0]);
+  STAGED_LIST(sc2, int, 1024, s.cln2, &s.ctr[C_CLN2]);
+  STAGED_LIST(sc1, int, 1024, s.cln1, &s.ctr[C_CLN1]);
   const int tid = threadIdx.x;
-  if (tid < kNbr) s_off[tid] = c_dx[tid] + g.W * (c_dy[tid] + g.H * c_dz[tid]);
-  if (tid == 0) s_np = 0;
-  __syncthreads();
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  if (b < nb2) {
-    const int i = b * 256 + tid;
-    saddle_entry<2>(s, mode, f, L2, flow, F2, La2, i, i < s.n2, s_off, s_pairs, &s_np, pairs,
-                    kmax, 0);
-  } else {
-    const int i = (b - nb2) * 256 + tid;
-    saddle_entry<1>(s, mode, f, L1, flow, F1, La1, i, i < s.n1, s_off, s_pairs, &s_np, pairs,
-                    kmin, 0);
+  if (tid < kNbr) s_off[tid] = s.g.off[tid];
+  if (blockIdx.x == 0 && tid == 0 && mode != kModeFill) s.ctr[C_PCHUNK] = nb;
+  const int per = (nb + gridDim.x - 1) / gridDim.x;
+  const int c0 = xcd_remap(blockIdx.x, gridDim.x) * per, c1 = min(nb, c0 + per);
+  for (int c = c0; c < c1; c++) {
+    if (tid == 0) s_np = 0;
+    __syncthreads();
+    int2* cp = pairs + (long long)c * kChunkPairs;
+    if (c < nb2) {
+      const int i = c * 256 + tid;
+      saddle_entry<2>(s, mode, f, L2, flow, F2, La2, i, i < s.n2, s_off, cp, &s_np, kmax, 0, sa,
+                      sc2);
+    } else {
+      const int i = (c - nb2) * 256 + tid;
+      saddle_entry<1>(s, mode, f, L1, flow, F1, La1, i, i < s.n1, s_off, cp, &s_np, kmin, 0, sa,
+                      sc1);
+    }
+    __syncthreads();
+    if (tid == 0 && mode != kModeFill) pcnt[c] = s_np;
   }
-  __syncthreads();
-  const int np = s_np < kPairBuf ? s_np : kPairBuf;
-  if (tid == 0) s_pbase = np ? atomicAdd(&s.ctr[C_PAIRS], np) : 0;
-  __syncthreads();
-  for (int k = tid; k < np; k += 256) pairs[s_pbase + k] = s_pairs[k];
+  sa.flush();
+  sc2.flush();
+  sc1.flush();
 }
 
 // Slot gather for the unchanged saddles of a decp pass: for slot index j
@@ -1246,19 +1239,27 @@ __global__ void __launch_bounds__(256) k_ext_reduce(const int* inmax, int nmax, 
 // Slot pass B: among the recorded (position, saddle) pairs, the saddles
 // whose value equals the position's best value key settle the id (highest id
 // for a maximum's slot, lowest for a minimum's), giving the (value, id) order
-// of computelargestSaddlesForMax / computesmallestSaddlesForMin.
-__global__ void __launch_bounds__(256) k_slot_ids(St s, const int2* pairs, const double* f,
-                                                  const long long* kmax, const long long* kmin,
-                                                  int* slot_max, int* slot_min) {
-  int n = s.ctr[C_PAIRS];
-  GRID_STRIDE(i, n) {
-    int2 pr = pairs[i];
-    int e = pr.x >> 1;
-    long long k = okey_c(f[pr.y]);
-    if (pr.x & 1) {
-      if (k == kmin[e]) atomicMin(&slot_min[e], pr.y);
-    } else {
-      if (k == kmax[e]) atomicMax(&slot_max[e], pr.y);
+// of computelargestSaddlesForMax / computesmallestSaddlesForMin.  One
+// wavefront per chunk region of the last saddle launch.
+__global__ void __launch_bounds__(256) k_slot_ids(St s, const int2* pairs, const int* pcnt,
+                                                  const double* f, const long long* kmax,
+                                                  const long long* kmin, int* slot_max,
+                                                  int* slot_min) {
+  const int nc = s.ctr[C_PCHUNK];
+  const int lane = threadIdx.x & 63;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < nc; c += nw) {
+    const int n = pcnt[c];
+    const int2* P = pairs + (long long)c * kChunkPairs;
+    for (int j = lane; j < n; j += 64) {
+      const int2 pr = P[j];
+      const int e = pr.x >> 1;
+      const long long k = okey_c(f[pr.y]);
+      if (pr.x & 1) {
+        if (k == kmin[e]) atomicMin(&slot_min[e], pr.y);
+      } else {
+        if (k == kmax[e]) atomicMax(&slot_max[e], pr.y);
+      }
     }
   }
 }
@@ -1283,7 +1284,6 @@ __global__ void k_slot_init(St s, long long* kmax, int* smax, long long* kmin, i
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     s.ctr[C_ACT0] = 0;
-    s.ctr[C_PAIRS] = 0;
     s.ctr[C_CLN2] = 0;
     s.ctr[C_CLN1] = 0;
     s.ctr[C_DSLOT] = 0;
@@ -1712,37 +1712,44 @@ __global__ void k_inc_ext(St s) {
 }
 
 // The queued sides through saddle_entry (kModeDec, with slot re-gather
-// marking).  Every lane of a block walks the same rounds, so the wavefront
-// collectives inside saddle_entry see whole wavefronts.
-__global__ void __launch_bounds__(256) k_inc_saddles(St s, const double* f, int2* pairs,
+// marking), in chunks of 256 queued sides (S2 sides first, each part rounded
+// up to whole chunks), a contiguous range of chunks per block, slot pairs in
+// the chunks' own regions as in k_saddle_list.
+__global__ void __launch_bounds__(256) k_inc_saddles(St s, const double* f, int2* pairs, int* pcnt,
                                                      long long* kmax, long long* kmin) {
-  __shared__ int2 s_pairs[kPairBuf];
   __shared__ int s_off[kNbr];
-  __shared__ int s_np, s_pbase;
+  __shared__ int s_np;
+  STAGED_LIST(sa, int2, 1024, s.act0, &s.ctr[C_ACT0]);
+  STAGED_LIST(sc2, int, 1024, s.cln2, &s.ctr[C_CLN2]);
+  STAGED_LIST(sc1, int, 1024, s.cln1, &s.ctr[C_CLN1]);
   const int tid = threadIdx.x;
   if (tid < kNbr) s_off[tid] = s.g.off[tid];
-  if (tid == 0) s_np = 0;
-  __syncthreads();
   const int n2 = s.ctr[C_DSL2], n1 = s.ctr[C_DSL1];
-  const int r2 = (n2 + 255) & ~255, r1 = (n1 + 255) & ~255;
-  for (int base = blockIdx.x * 256; base < r2 + r1; base += gridDim.x * 256) {
-    if (base < r2) {
-      const int k = base + tid;
+  const int nb2 = (n2 + 255) >> 8, nb = nb2 + ((n1 + 255) >> 8);
+  if (blockIdx.x == 0 && tid == 0) s.ctr[C_PCHUNK] = nb;
+  const int per = (nb + gridDim.x - 1) / gridDim.x;
+  const int c0 = blockIdx.x * per, c1 = min(nb, c0 + per);
+  for (int c = c0; c < c1; c++) {
+    if (tid == 0) s_np = 0;
+    __syncthreads();
+    int2* cp = pairs + (long long)c * kChunkPairs;
+    if (c < nb2) {
+      const int k = c * 256 + tid;
       const bool act = k < n2;
       saddle_entry<2>(s, kModeDec, f, s.dDS, s.dlow, s.df2, s.dl2, act ? s.dsl2[k] : 0, act,
-                      s_off, s_pairs, &s_np, pairs, kmax, 1);
+                      s_off, cp, &s_np, kmax, 1, sa, sc2);
     } else {
-      const int k = base - r2 + tid;
+      const int k = (c - nb2) * 256 + tid;
       const bool act = k < n1;
       saddle_entry<1>(s, kModeDec, f, s.dAS, s.dlow, s.df1, s.dl1, act ? s.dsl1[k] : 0, act,
-                      s_off, s_pairs, &s_np, pairs, kmin, 1);
+                      s_off, cp, &s_np, kmin, 1, sa, sc1);
     }
+    __syncthreads();
+    if (tid == 0) pcnt[c] = s_np;
   }
-  __syncthreads();
-  const int np = s_np < kPairBuf ? s_np : kPairBuf;
-  if (tid == 0) s_pbase = np ? atomicAdd(&s.ctr[C_PAIRS], np) : 0;
-  __syncthreads();
-  for (int k = tid; k < np; k += 256) pairs[s_pbase + k] = s_pairs[k];
+  sa.flush();
+  sc2.flush();
+  sc1.flush();
 }
 
 // The queued slots' gathers (one wavefront per slot).
@@ -2162,34 +2169,13 @@ __global__ void __launch_bounds__(256) k_lists_count(St s, int* cnt, int nb, int
   }
 }
 
-// In-place exclusive scan of each category's nb block counts (one block of
-// 1024 threads, a contiguous chunk per thread); totals to the list-length
-// counters C_N2, C_N1, C_NMAX, C_NMIN.
-__global__ void __launch_bounds__(1024) k_lists_scan(St s, int* cnt, int nb) {
-  __shared__ int sh[1024];
-  const int t = threadIdx.x;
-  const int per = (nb + 1023) / 1024;
-  const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
-  for (int c = 0; c < 4; c++) {
-    int* a = cnt + (long long)c * nb;
-    int sum = 0;
-    for (int b = b0; b < b1; b++) sum += a[b];
-    sh[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      int x = t >= o ? sh[t - o] : 0;
-      __syncthreads();
-      sh[t] += x;
-      __syncthreads();
-    }
-    int run = sh[t] - sum;
-    for (int b = b0; b < b1; b++) {
-      int x = a[b];
-      a[b] = run;
-      run += x;
-    }
-    if (t == 1023) s.ctr[C_N2 + c] = sh[1023];
-    __syncthreads();
+// List lengths from the exclusive scan of the 4 * nb block counts taken as
+// one array (entry 4 * nb holds the grand total): category c's blocks start
+// at cnt[c * nb], so its length is cnt[(c + 1) * nb] - cnt[c * nb].
+__global__ void k_lists_totals(St s, const int* cnt, int nb) {
+  if (blockIdx.x == 0 && threadIdx.x < 4) {
+    const int c = threadIdx.x;
+    s.ctr[C_N2 + c] = cnt[(long long)(c + 1) * nb] - cnt[(long long)c * nb];
   }
 }
 
@@ -2209,7 +2195,7 @@ __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int n
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     if (!((fl >> c) & 1u)) continue;
-    int base = cnt[c * nb + blockIdx.x];
+    int base = cnt[(long long)c * nb + blockIdx.x] - cnt[(long long)c * nb];
     for (int k = 0; k < wave; k++) base += w[c][k];
     lists[c][base + __popcll(m[c] & ((1ull << lane) - 1ull))] = v;
   }
