@@ -74,6 +74,8 @@ enum Ctr {
   C_CL1,
   C_CLN2,         // changed sides of this CP pass
   C_CLN1,
+  C_WL2,          // sides whose decp triplet ends may differ from the original ones
+  C_WL1,
   C_COUNT_
 };
 
@@ -135,6 +137,8 @@ struct St {
   int* slstamp;                   // per slot index
   int* dslot;                     // slots to gather again
   int4* citems;                   // {k0, k1, maxima?, -}: index chunks whose saddles are queued
+  uint8_t* wflag;                 // per side: listed in wl2 / wl1
+  int *wl2, *wl1;                 // sides whose triplet ends differed when last computed
   int iepoch;                     // stamp of this pass
 };
 

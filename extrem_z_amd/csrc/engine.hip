@@ -160,7 +160,7 @@ struct extremz_ctx {
   ez::DevBuf<double> orig_tmp, decp_tmp;
   ez::DevBuf<uint8_t> counts_tmp;
   // incremental CP passes of decp
-  ez::DevBuf<uint8_t> pdir, lflag, cflag, tu, td;
+  ez::DevBuf<uint8_t> pdir, lflag, cflag, tu, td, wflag;
   ez::DevBuf<int> lchg, spos2, spos1, ilist, qu, qd, sstamp, slists, slstamp, dslot, pcnt;
   ez::DevBuf<uint8_t> vsnap;  // snapshots of the self-check (EXTREMZ_VERIFY_INC)
   int* h_ctr = nullptr;
@@ -182,7 +182,8 @@ struct extremz_ctx {
       f(*b);
     for (ez::DevBuf<long long>* b : {&delta, &kmax, &kmin, &pkey, &ckey, &pv2, &pv1}) f(*b);
     for (ez::DevBuf<uint8_t>* b :
-         {&cnt, &otype, &dflag, &wn2, &wn1, &counts_tmp, &pdir, &lflag, &cflag, &tu, &td, &vsnap})
+         {&cnt, &otype, &dflag, &wn2, &wn1, &counts_tmp, &pdir, &lflag, &cflag, &tu, &td, &wflag,
+          &vsnap})
       f(*b);
     for (ez::DevBuf<uint16_t>* b : {&olow, &dlow}) f(*b);
     for (ez::DevBuf<int2>* b : {&longs, &act0, &actA, &actB, &pairs}) f(*b);
@@ -266,7 +267,7 @@ class Run {
   static constexpr int kSweeps = 3, kSweepBlocks = 256;
   // Blocks of a saddle launch (each takes a contiguous range of 256-side
   // chunks).
-  static constexpr int kSaddleBlocks = 4096;
+  static constexpr int kSaddleBlocks = 65536;
   static int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     return (int)std::max<long long>(1, std::min<long long>(b, 2048));
@@ -438,6 +439,10 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
     saddle_pass(kModeFill);
     tbegin(EXTREMZ_K_SLOT_GATHER);
   } else {
+    // every side is recomputed: the list of sides with differing triplet ends
+    // starts empty
+    hip_check(hipMemsetAsync(s_.wflag, 0, (size_t)s_.n2 + s_.n1 + 1, st()), "memset");
+    set_ctr2(C_WL2, C_WL1, 0);
     saddle_pass(kModeDec);
     tbegin(EXTREMZ_K_SLOT_GATHER);
     k_slot_pull<<<std::max(1, (ns + 31) / 32), 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
@@ -736,7 +741,7 @@ void Run::s_loops(int side, int do_fix) {
   else set_ctr2(C_WMINC, C_WMINC2, 0);
   if (do_fix) counts_[EXTREMZ_STAT_S_LOOP_CALLS]++;
   if (n) {
-    int b = blocks_for(n);
+    const int b = std::min(blocks_for(n), 512);  // over the listed sides only
     k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 0, 0);
     tbegin(EXTREMZ_K_PROPOSE);
     k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 1, do_fix);
@@ -1099,7 +1104,9 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   {  // incremental CP passes: saddle positions, side and slot lists and stamps
     const long long n21 = (long long)s_.n2 + s_.n1, ns = (long long)s_.nmax + s_.nmin;
     c->sstamp.ensure(n21 + 1);
-    c->slists.ensure(3 * n21 + 1);
+    c->slists.ensure(4 * n21 + 1);
+    c->wflag.ensure(n21 + 1);
+    s_.wflag = c->wflag.p;
     c->slstamp.ensure(ns + 1);
     c->dslot.ensure(ns + 1);
     s_.sstamp = c->sstamp.p;
@@ -1112,6 +1119,8 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
     s_.cl1 = s_.cl2 + s_.n2;
     s_.cln2 = sl + 2 * n21;
     s_.cln1 = s_.cln2 + s_.n2;
+    s_.wl2 = sl + 3 * n21;
+    s_.wl1 = s_.wl2 + s_.n2;
     ez::hip_check(hipMemsetAsync(s_.sstamp, 0, sizeof(int) * (n21 + 1), q), "memset");
     ez::hip_check(hipMemsetAsync(s_.slstamp, 0, sizeof(int) * (ns + 1), q), "memset");
     k_fill_i32<<<bN, 256, 0, q>>>(s_.spos2, -1, N);

@@ -902,7 +902,7 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
                                              const uint16_t* flow, int* F, int* La, int i,
                                              bool act, const int* s_off, int2* cpairs,
                                              int* s_np, long long* key_slot, int inc, LA& acl,
-                                             LC& cl) {
+                                             LC& cl, LC& wl) {
   const Geo& g = s.g;
   const int dec = mode == kModeDec;
   int v = 0;
@@ -962,6 +962,15 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
   if (dec) {
     if (cand != 0) acl.push(make_int2(2 * v + (SIDE == 1), (int)cand));
     if (act && changed) cl.push(i);
+    if (act) {  // the sides k_wrong_index has to look at
+      const bool wm = F[i] != (SIDE == 2 ? s.of2 : s.of1)[i] ||
+                      La[i] != (SIDE == 2 ? s.ol2 : s.ol1)[i];
+      const int q = SIDE == 2 ? i : s.n2 + i;
+      if (wm && !s.wflag[q]) {
+        s.wflag[q] = 1;
+        wl.push(i);
+      }
+    }
   }
 }
 
@@ -984,6 +993,8 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const doubl
   STAGED_LIST(sa, int2, 1024, s.act0, &s.ctr[C_ACT0]);
   STAGED_LIST(sc2, int, 1024, s.cln2, &s.ctr[C_CLN2]);
   STAGED_LIST(sc1, int, 1024, s.cln1, &s.ctr[C_CLN1]);
+  STAGED_LIST(sw2, int, 1024, s.wl2, &s.ctr[C_WL2]);
+  STAGED_LIST(sw1, int, 1024, s.wl1, &s.ctr[C_WL1]);
   const int tid = threadIdx.x;
   if (tid < kNbr) s_off[tid] = s.g.off[tid];
   if (blockIdx.x == 0 && tid == 0 && mode != kModeFill) s.ctr[C_PCHUNK] = nb;
@@ -996,11 +1007,11 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const doubl
     if (c < nb2) {
       const int i = c * 256 + tid;
       saddle_entry<2>(s, mode, f, L2, flow, F2, La2, i, i < s.n2, s_off, cp, &s_np, kmax, 0, sa,
-                      sc2);
+                      sc2, sw2);
     } else {
       const int i = (c - nb2) * 256 + tid;
       saddle_entry<1>(s, mode, f, L1, flow, F1, La1, i, i < s.n1, s_off, cp, &s_np, kmin, 0, sa,
-                      sc1);
+                      sc1, sw1);
     }
     __syncthreads();
     if (tid == 0 && mode != kModeFill) pcnt[c] = s_np;
@@ -1008,6 +1019,8 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const doubl
   sa.flush();
   sc2.flush();
   sc1.flush();
+  sw2.flush();
+  sw1.flush();
 }
 
 // Slot gather for the unchanged saddles of a decp pass: for slot index j
@@ -1722,6 +1735,8 @@ __global__ void __launch_bounds__(256) k_inc_saddles(St s, const double* f, int2
   STAGED_LIST(sa, int2, 1024, s.act0, &s.ctr[C_ACT0]);
   STAGED_LIST(sc2, int, 1024, s.cln2, &s.ctr[C_CLN2]);
   STAGED_LIST(sc1, int, 1024, s.cln1, &s.ctr[C_CLN1]);
+  STAGED_LIST(sw2, int, 1024, s.wl2, &s.ctr[C_WL2]);
+  STAGED_LIST(sw1, int, 1024, s.wl1, &s.ctr[C_WL1]);
   const int tid = threadIdx.x;
   if (tid < kNbr) s_off[tid] = s.g.off[tid];
   const int n2 = s.ctr[C_DSL2], n1 = s.ctr[C_DSL1];
@@ -1737,12 +1752,12 @@ __global__ void __launch_bounds__(256) k_inc_saddles(St s, const double* f, int2
       const int k = c * 256 + tid;
       const bool act = k < n2;
       saddle_entry<2>(s, kModeDec, f, s.dDS, s.dlow, s.df2, s.dl2, act ? s.dsl2[k] : 0, act,
-                      s_off, cp, &s_np, kmax, 1, sa, sc2);
+                      s_off, cp, &s_np, kmax, 1, sa, sc2, sw2);
     } else {
       const int k = (c - nb2) * 256 + tid;
       const bool act = k < n1;
       saddle_entry<1>(s, kModeDec, f, s.dAS, s.dlow, s.df1, s.dl1, act ? s.dsl1[k] : 0, act,
-                      s_off, cp, &s_np, kmin, 1, sa, sc1);
+                      s_off, cp, &s_np, kmin, 1, sa, sc1, sw1);
     }
     __syncthreads();
     if (tid == 0) pcnt[c] = s_np;
@@ -1750,6 +1765,8 @@ __global__ void __launch_bounds__(256) k_inc_saddles(St s, const double* f, int2
   sa.flush();
   sc2.flush();
   sc1.flush();
+  sw2.flush();
+  sw1.flush();
 }
 
 // The queued slots' gathers (one wavefront per slot).
@@ -2042,7 +2059,9 @@ __global__ void __launch_bounds__(256) k_fixpath(St s) {
 // vertex id.  phase 0 registers, phase 1 counts (and proposes, s_loops /
 // s_loops_join, src/ExtreM.cpp:2838-2886), phase 2 releases.
 __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, int do_fix) {
-  int n = side == 2 ? s.n2 : s.n1;
+  // only the listed sides can have triplet ends that differ
+  const int n = s.ctr[side == 2 ? C_WL2 : C_WL1];
+  const int* WL = side == 2 ? s.wl2 : s.wl1;
   const int* S = side == 2 ? s.S2 : s.S1;
   const int* of = side == 2 ? s.of2 : s.of1;
   const int* ol = side == 2 ? s.ol2 : s.ol1;
@@ -2051,7 +2070,8 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
   STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[C_TOUCHED]);
   STAGED_COUNT(cf, &s.ctr[side == 2 ? C_WMC : C_WMINC2]);
   STAGED_COUNT(cl, &s.ctr[side == 2 ? C_WMC2 : C_WMINC]);
-  GRID_STRIDE(i, n) {
+  GRID_STRIDE(k, n) {
+    const int i = WL[k];
     int sv = S[i];
     int a = of[i], b = ol[i], c = df[i], d = dl[i];
     bool mf = a != c, ml = b != d;

@@ -35,7 +35,7 @@ using Clock = std::chrono::steady_clock;
 // change lists, pointers and flags of the incremental CP pass) plus the
 // small per-extremum arrays.  var: what depends on the data or the caller's
 // buffers -- the saddle pair records, side and slot lists, stamps, indices
-// and keys (88 B/voxel covers the 0.43 saddle sides per voxel of white noise;
+// and keys (92 B/voxel covers the 0.43 saddle sides per voxel of white noise;
 // smooth fields have fewer) -- and the staging of host buffers / float32
 // originals.
 struct Need {
@@ -43,7 +43,7 @@ struct Need {
 };
 Need working_set(const extremz_job& j) {
   const int64_t N = (int64_t)j.W * j.H * j.D;
-  Need n{239 * N + (1 << 20), 88 * N};
+  Need n{239 * N + (1 << 20), 92 * N};
   if (j.on_host || j.orig_dtype == EXTREMZ_DTYPE_F32) n.var += 8 * N;  // orig as float64
   if (j.on_host) n.var += 9 * N;                                       // decp + counts
   return n;
