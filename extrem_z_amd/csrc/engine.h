@@ -142,11 +142,11 @@ struct St {
   int iepoch;                     // stamp of this pass
 };
 
-// Modes of the saddle pass.  kModeOrig: original field, every saddle pushes
-// its roots to the slots and counts its (extremum, saddle) adjacencies;
-// kModeFill: original field, the adjacencies are written to the per-extremum
-// index; kModeDec: decp, only saddles whose root set differs from the
-// original one push (the others are gathered per extremum by k_slot_pull).
+// Modes of the saddle pass.  kModeOrig: original field, counts each
+// saddle's (extremum, saddle) adjacencies; kModeFill: original field, the
+// adjacencies are written to the per-extremum index and every saddle leaves
+// its key for the gather (k_slot_pull); kModeDec: decp, only saddles whose
+// root set differs from the original one push (the others are gathered).
 constexpr int kModeOrig = 0, kModeFill = 1, kModeDec = 2;
 
 struct HipError : std::runtime_error {
@@ -234,8 +234,7 @@ __global__ void k_slot_pull_join(St s, const long long* ckey, const int* cpos, l
                                  int* pid);
 __global__ void k_long_slots(St s);
 __global__ void k_slot_merge(St s, long long* kmax, long long* kmin, int* slot_max,
-                             int* slot_min, const long long* pkey, const int* pid,
-                             int use_pull);
+                             int* slot_min, const long long* pkey, const int* pid);
 __global__ void k_scan_single(int* a, int n);
 __global__ void k_slot_init(St s, long long* kmax, int* smax, long long* kmin, int* smin);
 __global__ void k_list_fill64(const int* list, int n, long long* arr, long long val);

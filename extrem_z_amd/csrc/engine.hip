@@ -400,8 +400,9 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
     tend();
   };
   if (!dec) {
-    // original field: every saddle pushes, and the (extremum, saddle)
-    // adjacencies are indexed per extremum for the decp passes
+    // original field: the (extremum, saddle) adjacencies are indexed per
+    // extremum (for the decp passes too), and each extremum gathers its
+    // largest / smallest saddle over them
     k_fill_i32<<<blocks_for(ns + 1), 256, 0, st()>>>(s_.csr_off, 0, ns + 1);
     saddle_pass(kModeOrig);
     {  // exclusive scan of the ns counts (entry ns, zero, receives the total)
@@ -438,6 +439,9 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
                   "hipMemcpyAsync");
     saddle_pass(kModeFill);
     tbegin(EXTREMZ_K_SLOT_GATHER);
+    k_slot_pull<<<std::max(1, (ns + 31) / 32), 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
+    k_slot_pull_long<<<512, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p);
+    k_slot_pull_join<<<64, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p, c_->pkey.p, c_->pid.p);
   } else {
     // every side is recomputed: the list of sides with differing triplet ends
     // starts empty
@@ -450,10 +454,11 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
     k_slot_pull_join<<<64, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p, c_->pkey.p, c_->pid.p);
   }
   k_slot_merge<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(
-      s_, kmax, kmin, smax, smin, c_->pkey.p, c_->pid.p, dec);
-  // one wavefront per chunk region
-  k_slot_ids<<<blocks_for(64LL * (nb2 + nb1)), 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p, f,
-                                                               kmax, kmin, smax, smin);
+      s_, kmax, kmin, smax, smin, c_->pkey.p, c_->pid.p);
+  // ties among the pushed saddles: one wavefront per chunk region
+  if (dec)
+    k_slot_ids<<<blocks_for(64LL * (nb2 + nb1)), 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p, f,
+                                                                 kmax, kmin, smax, smin);
   if (dec) {
     // decp global maximum / minimum among the listed extrema
     k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, f, c_->bmax2.p,
@@ -604,7 +609,7 @@ bool Run::cp_inc(long long nch) {
         s_, s_.decp, c_->pairs.p, c_->pcnt.p, kmax, kmin);
     k_inc_pull<<<256, 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
     k_slot_merge<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(
-        s_, kmax, kmin, s_.dlarg, s_.dsmall, c_->pkey.p, c_->pid.p, 1);
+        s_, kmax, kmin, s_.dlarg, s_.dsmall, c_->pkey.p, c_->pid.p);
     k_slot_ids<<<blocks_for(64LL * nbi), 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p, s_.decp,
                                                          kmax, kmin, s_.dlarg, s_.dsmall);
     // decp global maximum / minimum, when the current one changed value

@@ -937,14 +937,18 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
   }
   if (mode == kModeOrig) {
 #pragma unroll
-    for (int d = 0; d < kNbr; d++)
+    for (int d = 0; d < kNbr; d++) {
       if (E[d] >= 0) atomicAdd(&s.csr_off[slot_index(s, E[d])], 1);
+      E[d] = -1;
+    }
   } else if (mode == kModeFill) {
 #pragma unroll
     for (int d = 0; d < kNbr; d++) {
       if (E[d] >= 0) s.csr[atomicAdd(&s.csr_cur[slot_index(s, E[d])], 1)] = i;
       E[d] = -1;
     }
+    // every original saddle is gathered by its extrema
+    if (act) (SIDE == 2 ? s.pv2 : s.pv1)[i] = okey_c(fs);
   } else {
     // same star and the same root at every star neighbour: the root set is
     // the original one, gathered per extremum instead of pushed (its key at
@@ -1171,13 +1175,7 @@ __global__ void k_long_slots(St s) {
 // starts with the gathered saddle when it holds the final key (k_slot_ids
 // then adds the pushed saddles holding it).
 __global__ void k_slot_merge(St s, long long* kmax, long long* kmin, int* slot_max,
-                             int* slot_min, const long long* pkey, const int* pid,
-                             int use_pull) {
-  if (!use_pull) {  // every saddle pushed
-    GRID_STRIDE(j, s.nmax) slot_max[j] = kNone;
-    GRID_STRIDE(j, s.nmin) slot_min[j] = 0x7fffffff;
-    return;
-  }
+                             int* slot_min, const long long* pkey, const int* pid) {
   GRID_STRIDE(j, s.nmax) {
     const long long k = kmax[j] > pkey[j] ? kmax[j] : pkey[j];
     kmax[j] = k;
