@@ -311,7 +311,7 @@ class Run {
   void list_compaction();
   void slot_pass(int dec, const double* f, const int* L2, const int* L1, const uint16_t* flow,
                  int* F2, int* La2, int* F1, int* La1, int* smax, int* smin);
-  void apply();
+  void apply(long long bound = 1LL << 62, int also = -1);
   void label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir = nullptr);
   void c_loops();
   void cp_dec(bool force);
@@ -469,12 +469,14 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
   tend();
 }
 
-// apply_delta over the touched list, then the list is emptied.
-void Run::apply() {
+// apply_delta over the touched list (at most `bound` entries), then the list
+// is emptied (with the counter `also`, when given).
+void Run::apply(long long bound, int also) {
   tbegin(EXTREMZ_K_APPLY);
-  k_apply<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
+  k_apply<<<blocks_for(std::min<long long>(bound, s_.g.N)), 256, 0, st()>>>(s_);
   tend();
-  set_ctr(C_TOUCHED, 0);
+  if (also >= 0) set_ctr2(C_TOUCHED, also, 0);
+  else set_ctr(C_TOUCHED, 0);
 }
 
 // ComputeDescendingManifold + ComputeAscendingManifold on field f (D7): tile
@@ -523,19 +525,21 @@ void Run::c_loops() {
     tbegin(EXTREMZ_K_PROPOSE);
     k_propose_c<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_);
     tend();
-    apply();
+    // a frontier vertex proposes to itself or at most its 14 neighbours, so
+    // at most 15 nf vertices are touched and changed
+    const long long nt = 15LL * nf;
     int back = front_ == C_FRONT_A ? C_FRONT_B : C_FRONT_A;
-    set_ctr(back, 0);
+    apply(nt, back);
     epoch_++;
     tbegin(EXTREMZ_K_CHECK_INCR);
     k_check_list<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_, 0, back_buf_, back,
                                                     epoch_);
     tend();
     tbegin(EXTREMZ_K_CHECK_INCR);
-    k_check_list<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, s_.dirty, C_DIRTY, 1, back_buf_,
-                                                        back, epoch_);
+    k_check_list<<<blocks_for(15 * nt), 256, 0, st()>>>(s_, s_.dirty, C_DIRTY, 1, back_buf_,
+                                                         back, epoch_);
     tend();
-    k_clear_dirty<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
+    k_clear_dirty<<<blocks_for(nt), 256, 0, st()>>>(s_);
     set_ctr(C_DIRTY, 0);
     std::swap(front_buf_, back_buf_);
     front_ = back;

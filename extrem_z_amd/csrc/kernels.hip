@@ -822,29 +822,25 @@ __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* 
 }
 
 // Slot pass A for one side: raises (maxima, even positions) or lowers (minima,
-// odd positions) each distinct root's best saddle key.  A saddle whose key
-// is worse than the key it found there can never hold the slot; every other
-// one records its (position, saddle) pair in the region of its 256-side chunk
-// (cpairs, kChunkPairs entries, count in the LDS counter s_np: one LDS
-// atomic per wavefront), so every saddle that ties the final key is
-// recorded.  Called by every lane of the block.
+// odd positions) each distinct root's best saddle key (atomics whose result
+// is not used, so none waits for its return) and records every
+// (position, saddle) pair in the region of its 256-side chunk (cpairs,
+// kChunkPairs entries, count in the LDS counter s_np: one LDS atomic per
+// wavefront); k_slot_ids keeps the pairs that hold the final key.  Called by
+// every lane of the block.
 template <int SIDE>
 __device__ __forceinline__ void emit_side(const St& s, int v, long long kv, int (&E)[kNbr],
                                           long long* key_slot, int2* cpairs, int* s_np) {
-  long long old[kNbr];
-#pragma unroll
-  for (int d = 0; d < kNbr; d++) {
-    if (E[d] >= 0 && (E[d] & 1) == (SIDE == 1))
-      old[d] = SIDE == 2 ? atomicMax(&key_slot[E[d] >> 1], kv)
-                         : atomicMin(&key_slot[E[d] >> 1], kv);
-    else
-      E[d] = -1;
-  }
   int cnt = 0;
 #pragma unroll
   for (int d = 0; d < kNbr; d++) {
-    if (E[d] >= 0 && (SIDE == 2 ? kv >= old[d] : kv <= old[d])) cnt++;
-    else E[d] = -1;
+    if (E[d] >= 0 && (E[d] & 1) == (SIDE == 1)) {
+      if (SIDE == 2) atomicMax(&key_slot[E[d] >> 1], kv);
+      else atomicMin(&key_slot[E[d] >> 1], kv);
+      cnt++;
+    } else {
+      E[d] = -1;
+    }
   }
   int pre = cnt;
 #pragma unroll
