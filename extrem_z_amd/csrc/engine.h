@@ -66,9 +66,9 @@ enum Ctr {
   C_QHU,
   C_QLD,          // ... of the down-root one
   C_QHD,
-  C_SWEEP,        // blocks of a propagation sweep that finished
-  C_IBAIL,        // the incremental pass gave up (the full pass runs instead)
-                  // (C_LCHG..C_IBAIL are cleared at the end of every CP pass of decp)
+  C_SWEEP,        // blocks of a propagation sweep (or of the walk) that finished
+                  // (C_LCHG..C_SWEEP are cleared at the end of every CP pass of decp)
+  C_IBAIL,        // an incremental pass overran a list (never expected; fails the call)
   C_DSLOT,        // slots to gather again
   C_CL2,          // changed sides of the last CP pass
   C_CL1,
@@ -246,16 +246,14 @@ __global__ void k_inc_taint(St s);
 __global__ void k_inc_walk(St s);
 __global__ void k_inc_sweep(St s);
 __global__ void k_inc_prop(St s, int qcap);
-__global__ void k_inc_mark(St s);
-__global__ void k_inc_csr_items(St s);
+__global__ void k_inc_mark(St s, long long* kmax, long long* kmin);
 __global__ void k_inc_mark_csr(St s);
-__global__ void k_inc_ext(St s);
 __global__ void k_inc_saddles(St s, const double* f, int2* pairs, int* pcnt, long long* kmax,
                               long long* kmin);
 __global__ void k_inc_pull(St s, long long* pkey, int* pid);
 __global__ void k_inc_clear(St s);
 __global__ void k_ctr_copy2(int* ctr, int a, int from_a, int b, int from_b);
-__global__ void k_ctr_fill(int* ctr, int a, int b, int v);
+__global__ void k_inc_end_ctr(int* ctr);
 __global__ void k_diff_count(const void* a, const void* b, long long nbytes, int* out);
 __global__ void k_act_copy(St s, int2* dst, int dctr);
 __global__ void k_wn_register(St s, int2* active, int actr);

@@ -268,6 +268,8 @@ class Run {
   // Blocks of a saddle launch (each takes a contiguous range of 256-side
   // chunks).
   static constexpr int kSaddleBlocks = 65536;
+  // Blocks of the incremental saddle launches (sizes known on the device only).
+  static constexpr int kIncSaddleBlocks = 1024;
   static int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     return (int)std::max<long long>(1, std::min<long long>(b, 2048));
@@ -304,6 +306,7 @@ class Run {
                   "hipMemcpyAsync");
     ez::hip_check(hipStreamSynchronize(st()), "hipStreamSynchronize");
     c_->host_syncs++;
+    if (c_->h_ctr[C_IBAIL]) throw HipError("incremental CP pass overran a work list");
   }
   int h(int idx) const { return c_->h_ctr[idx]; }
 
@@ -314,8 +317,8 @@ class Run {
   void apply(long long bound = 1LL << 62, int also = -1);
   void label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir = nullptr);
   void c_loops();
-  void cp_dec(bool force);
-  bool cp_inc(long long nch);
+  void cp_dec(bool force, bool fresh = false);
+  void cp_inc(long long nch);
   void cp_end();
   void cp_full();
   void verify_inc();
@@ -551,10 +554,11 @@ void Run::c_loops() {
 // candidates of decp (main.cpp:290-304 and the comparisons of 313-323).
 // Skipped when decp has not changed since the last call: everything it
 // computes depends on decp only.
-void Run::cp_dec(bool force) {
+// With fresh, the counters read last are still current (no launch since).
+void Run::cp_dec(bool force, bool fresh) {
   long long nch = 0;
   if (!force) {
-    pull();
+    if (!fresh) pull();
     if (!h(C_ANY_CHANGE)) return;
     nch = h(C_LCHG);
   }
@@ -562,10 +566,10 @@ void Run::cp_dec(bool force) {
   // (the self-check takes every pass that fits the lists, to exercise them)
   const bool inc = inc_ok_ && !inc_off_ && 15 * nch <= s_.icap &&
                    (inc_verify_ || nch <= s_.g.N / kIncDiv);
-  if (!(inc && cp_inc(nch))) cp_full();
+  if (inc) cp_inc(nch);
+  else cp_full();
   cp_end();
   inc_ok_ = true;
-  set_ctr(C_ANY_CHANGE, 0);
 }
 
 void Run::cp_full() {
@@ -574,65 +578,49 @@ void Run::cp_full() {
             s_.dsmall);
 }
 
-// The incremental CP pass (the kernels' section "incremental CP pass"):
-// false when it gave up (the caller runs the full pass).  The saddle part
-// falls back to the full saddle pass when half of the sides are queued.
-bool Run::cp_inc(long long nch) {
+// The incremental CP pass (the kernels' section "incremental CP pass"), with
+// no host read: list sizes stay on the device.  A list overrun (C_IBAIL,
+// never expected: every list holds its worst case) fails the call at the
+// next counter read.
+void Run::cp_inc(long long nch) {
   counts_[EXTREMZ_STAT_INC_PASSES]++;
   epoch_++;
   s_.iepoch = epoch_;
   const int b1 = blocks_for(15 * nch);
   tbegin(EXTREMZ_K_INCREMENTAL);
+  long long* kmax = c_->kmax.p;
+  long long* kmin = c_->kmin.p;
   k_inc_ptrs<<<b1, 256, 0, st()>>>(s_, epoch_);
-  k_inc_ext<<<blocks_for(nch), 256, 0, st()>>>(s_);
   k_inc_taint<<<b1, 256, 0, st()>>>(s_);
-  k_inc_walk<<<b1, 256, 0, st()>>>(s_);
+  k_inc_walk<<<std::min(b1, 512), 256, 0, st()>>>(s_);
   // the first levels of the root propagation over the whole grid, the rest in
   // one block
-  k_ctr_copy2<<<1, 64, 0, st()>>>(s_.ctr, C_QHU, C_QU, C_QHD, C_QD);
   for (int l = 0; l < kSweeps; l++) k_inc_sweep<<<kSweepBlocks, 256, 0, st()>>>(s_);
   k_inc_prop<<<1, 1024, 0, st()>>>(s_, s_.g.N);
-  k_inc_mark<<<1024, 256, 0, st()>>>(s_);
-  k_inc_csr_items<<<blocks_for(nch), 256, 0, st()>>>(s_);
+  k_inc_mark<<<1024, 256, 0, st()>>>(s_, kmax, kmin);
   k_inc_mark_csr<<<512, 256, 0, st()>>>(s_);
+  // the queued sides, their slots, ties, and the decp global maximum / minimum
+  // when the current one changed value; sizes stay on the device
+  k_inc_saddles<<<kIncSaddleBlocks, 256, 0, st()>>>(s_, s_.decp, c_->pairs.p, c_->pcnt.p, kmax,
+                                                     kmin);
+  k_inc_pull<<<256, 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
+  k_slot_merge<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(
+      s_, kmax, kmin, s_.dlarg, s_.dsmall, c_->pkey.p, c_->pid.p);
+  k_slot_ids<<<kIncSaddleBlocks, 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p, s_.decp, kmax,
+                                                 kmin, s_.dlarg, s_.dsmall);
+  k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, s_.decp, c_->bmax2.p,
+                                      c_->bmin2.p, s_.ctr + C_EXTD);
+  k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, 256, c_->bmin2.p, 256, s_.decp,
+                                    s_.ctr + C_DGMAX, s_.ctr + C_DGMIN, s_.ctr + C_EXTD);
   tend();
-  pull();
-  if (h(C_IBAIL)) return false;
-  const long long nd = (long long)h(C_DSL2) + h(C_DSL1);
-  if (2 * nd > (long long)s_.n2 + s_.n1) {
-    slot_pass(1, s_.decp, s_.dDS, s_.dAS, s_.dlow, s_.df2, s_.dl2, s_.df1, s_.dl1, s_.dlarg,
-              s_.dsmall);
-  } else {
-    long long* kmax = c_->kmax.p;
-    long long* kmin = c_->kmin.p;
-    tbegin(EXTREMZ_K_INCREMENTAL);
-    k_slot_init<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(s_, kmax, s_.dlarg,
-                                                                            kmin, s_.dsmall);
-    const int nbi = (h(C_DSL2) + 255) / 256 + (h(C_DSL1) + 255) / 256;
-    k_inc_saddles<<<std::max(1, std::min(nbi, kSaddleBlocks)), 256, 0, st()>>>(
-        s_, s_.decp, c_->pairs.p, c_->pcnt.p, kmax, kmin);
-    k_inc_pull<<<256, 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
-    k_slot_merge<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(
-        s_, kmax, kmin, s_.dlarg, s_.dsmall, c_->pkey.p, c_->pid.p);
-    k_slot_ids<<<blocks_for(64LL * nbi), 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p, s_.decp,
-                                                         kmax, kmin, s_.dlarg, s_.dsmall);
-    // decp global maximum / minimum, when the current one changed value
-    k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, s_.decp, c_->bmax2.p,
-                                        c_->bmin2.p, s_.ctr + C_EXTD);
-    k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, 256, c_->bmin2.p, 256, s_.decp,
-                                      s_.ctr + C_DGMAX, s_.ctr + C_DGMIN, s_.ctr + C_EXTD);
-    tend();
-  }
   if (inc_verify_) verify_inc();
-  return true;
 }
 
 // End of every CP pass of decp: the pass's flags are cleared, the changed
 // sides become the last pass's, and the change list starts empty.
 void Run::cp_end() {
   k_inc_clear<<<blocks_for(15 * (long long)s_.icap), 256, 0, st()>>>(s_);
-  k_ctr_fill<<<1, 64, 0, st()>>>(s_.ctr, C_LCHG, C_IBAIL + 1, 0);
-  k_ctr_copy2<<<1, 64, 0, st()>>>(s_.ctr, C_CL2, C_CLN2, C_CL1, C_CLN1);
+  k_inc_end_ctr<<<1, 64, 0, st()>>>(s_.ctr);
   std::swap(s_.cl2, s_.cln2);
   std::swap(s_.cl1, s_.cln1);
 }
@@ -1162,14 +1150,15 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   cp_dec(true);  // main.cpp:172-191
   wrong_neighbors();  // main.cpp:210-226
   int nfc = h(C_NFC), nfc1 = h(C_NFC1);
+  // (wrong_neighbors and c_loops end with a counter read: cp_dec can use it)
   if (nfc == 0) {  // main.cpp:230-254
-    cp_dec(false);
+    cp_dec(false, true);
     s_loops(2, 1);
     pull();
     if (h(C_WMC) == 0) saddle_loops(2, 1);
   }
   if (nfc1 == 0) {  // main.cpp:255-274
-    cp_dec(false);
+    cp_dec(false, nfc != 0);
     s_loops(1, 1);
     pull();
     if (h(C_WMINC) == 0) saddle_loops(1, 1);
@@ -1184,7 +1173,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
     r_loops();
     for (int rep = 0; rep < 3; rep++) {
       c_loops();
-      cp_dec(false);
+      cp_dec(false, true);
       wrong_neighbors();
       nfc = h(C_NFC);
       nfc1 = h(C_NFC1);

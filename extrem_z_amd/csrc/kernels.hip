@@ -1395,7 +1395,21 @@ __global__ void __launch_bounds__(256) k_inc_ptrs(St s, int epoch) {
 }
 
 // The old roots of the changed pointers (kept in ou / od, flagged in tu / td).
+// Also: a listed extremum other than the current global one changed value,
+// so the global maximum / minimum of decp is the better of it and the current
+// one (when the current one changed, k_inc_ptrs set C_EXTD and k_ext_reduce
+// recomputes both).
 __global__ void k_inc_taint(St s) {
+  if (!s.ctr[C_EXTD]) {
+    const int n = s.ctr[C_LCHG];
+    GRID_STRIDE(w, n) {
+      const int u = s.lchg[w];
+      const int e = s.ext_id[u];
+      if (e < 0) continue;
+      if ((e & 1) == 0) cas_best(&s.ctr[C_DGMAX], u, s.decp, true);
+      else cas_best(&s.ctr[C_DGMIN], u, s.decp, false);
+    }
+  }
   const int nu = s.ctr[C_CU], nd = s.ctr[C_CD];
   GRID_STRIDE(k, nu) {
     const int r = s.dDS[s.cu[k]];
@@ -1448,6 +1462,16 @@ __global__ void __launch_bounds__(256) k_inc_walk(St s) {
   }
   su.flush();
   sd.flush();
+  // the last block to finish opens the first propagation level
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&s.ctr[C_SWEEP], 1) == (int)gridDim.x - 1) {
+      __threadfence();
+      s.ctr[C_QHU] = __hip_atomic_load(&s.ctr[C_QU], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s.ctr[C_QHD] = __hip_atomic_load(&s.ctr[C_QD], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s.ctr[C_SWEEP] = 0;
+    }
+  }
 }
 
 // Wavefront-collective append of cnt entries per lane to the list Q whose
@@ -1602,8 +1626,10 @@ __device__ __forceinline__ void mark_side(const St& s, int q2, bool two, L& l2, 
 
 // Sides of saddles whose star or value changed (al, lchg), of saddles next to
 // a vertex whose root changed (qu: 2-saddles read up roots, qd: 1-saddles
-// down roots), and the changed sides of the last pass (cl2, cl1).
-__global__ void __launch_bounds__(256) k_inc_mark(St s) {
+// down roots), and the changed sides of the last pass (cl2, cl1).  Also the
+// chunks of k_inc_mark_csr, and empty slot keys and slots for this pass's
+// pushes (as k_slot_init; no kernel before the pushes reads them).
+__global__ void __launch_bounds__(256) k_inc_mark(St s, long long* kmax, long long* kmin) {
   if (s.ctr[C_IBAIL]) return;
   STAGED_LIST(l2, int, 1024, s.dsl2, &s.ctr[C_DSL2]);
   STAGED_LIST(l1, int, 1024, s.dsl1, &s.ctr[C_DSL1]);
@@ -1642,23 +1668,14 @@ __global__ void __launch_bounds__(256) k_inc_mark(St s) {
   }
   l2.flush();
   l1.flush();
-}
-
-// The saddles adjacent in the original to a changed vertex that is a listed
-// extremum and still a root of decp compare its value, so their sides are
-// queued; the other saddles with this root differ from the original and are
-// queued as last pass's changed sides or by a root change next to them.
-// k_inc_csr_items cuts those extrema's index ranges into chunks of 1024,
-// k_inc_mark_csr queues one chunk per wavefront.
-__global__ void __launch_bounds__(256) k_inc_csr_items(St s) {
-  if (s.ctr[C_IBAIL]) return;
-  const int n = s.ctr[C_LCHG];
+  // the extremum->saddle index ranges of the changed extrema that are still
+  // roots, in chunks of 1024 (uniform rounds: the chunk counts are summed per
+  // wavefront)
   const int stride = gridDim.x * blockDim.x;
-  // uniform rounds: the chunk counts are summed per wavefront
-  for (int w0 = blockIdx.x * blockDim.x; w0 < n; w0 += stride) {
+  for (int w0 = blockIdx.x * blockDim.x; w0 < nch; w0 += stride) {
     const int w = w0 + threadIdx.x;
     int k0 = 0, k1 = 0, mx = 0;
-    if (w < n) {
+    if (w < nch) {
       const int r = s.lchg[w];
       const int e = s.ext_id[r];
       if (e >= 0) {
@@ -1670,8 +1687,8 @@ __global__ void __launch_bounds__(256) k_inc_csr_items(St s) {
         }
       }
     }
-    const int nch = k1 > k0 ? (k1 - k0 + 1023) >> 10 : 0;
-    int pre = nch;
+    const int nck = k1 > k0 ? (k1 - k0 + 1023) >> 10 : 0;
+    int pre = nck;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const int y = __shfl_up(pre, o);
@@ -1681,12 +1698,32 @@ __global__ void __launch_bounds__(256) k_inc_csr_items(St s) {
     if (total == 0) continue;
     int b = 0;
     if (__lane_id() == 63) b = atomicAdd(&s.ctr[C_NITEM], total);
-    b = __shfl(b, 63) + pre - nch;
-    for (int c = 0; c < nch; c++)
+    b = __shfl(b, 63) + pre - nck;
+    for (int c = 0; c < nck; c++)
       s.citems[b + c] = make_int4(k0 + (c << 10), min(k1, k0 + ((c + 1) << 10)), mx, 0);
+  }
+  GRID_STRIDE(i, s.nmax) {
+    kmax[i] = (long long)0x8000000000000000ULL;
+    s.dlarg[i] = kNone;
+  }
+  GRID_STRIDE(i, s.nmin) {
+    kmin[i] = 0x7fffffffffffffffLL;
+    s.dsmall[i] = 0x7fffffff;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s.ctr[C_ACT0] = 0;
+    s.ctr[C_CLN2] = 0;
+    s.ctr[C_CLN1] = 0;
+    s.ctr[C_DSLOT] = 0;
   }
 }
 
+// The saddles adjacent in the original to a changed vertex that is a listed
+// extremum and still a root of decp compare its value, so their sides are
+// queued; the other saddles with this root differ from the original and are
+// queued as last pass's changed sides or by a root change next to them.
+// k_inc_mark cuts those extrema's index ranges into chunks of 1024,
+// k_inc_mark_csr queues one chunk per wavefront.
 __global__ void __launch_bounds__(256) k_inc_mark_csr(St s) {
   if (s.ctr[C_IBAIL]) return;
   STAGED_LIST(l2, int, 1024, s.dsl2, &s.ctr[C_DSL2]);
@@ -1700,22 +1737,6 @@ __global__ void __launch_bounds__(256) k_inc_mark_csr(St s) {
   }
   l2.flush();
   l1.flush();
-}
-
-// A listed extremum other than the current global one changed value: the
-// global maximum / minimum of decp is the better of it and the current one.
-// (When the current one changed, k_inc_ptrs set C_EXTD and k_ext_reduce
-// recomputes both.)
-__global__ void k_inc_ext(St s) {
-  if (s.ctr[C_EXTD]) return;
-  const int n = s.ctr[C_LCHG];
-  GRID_STRIDE(w, n) {
-    const int u = s.lchg[w];
-    const int e = s.ext_id[u];
-    if (e < 0) continue;
-    if ((e & 1) == 0) cas_best(&s.ctr[C_DGMAX], u, s.decp, true);
-    else cas_best(&s.ctr[C_DGMIN], u, s.decp, false);
-  }
 }
 
 // The queued sides through saddle_entry (kModeDec, with slot re-gather
@@ -1801,8 +1822,15 @@ __global__ void k_ctr_copy2(int* ctr, int a, int from_a, int b, int from_b) {
   }
 }
 
-__global__ void k_ctr_fill(int* ctr, int a, int b, int v) {
-  for (int i = a + (int)threadIdx.x; i < b; i += blockDim.x) ctr[i] = v;
+// End of a CP pass of decp: the pass counters start at zero, the changed
+// sides become the last pass's, decp counts as unchanged.
+__global__ void k_inc_end_ctr(int* ctr) {
+  for (int i = C_LCHG + (int)threadIdx.x; i <= C_SWEEP; i += blockDim.x) ctr[i] = 0;
+  if (threadIdx.x == 0) {
+    ctr[C_CL2] = ctr[C_CLN2];
+    ctr[C_CL1] = ctr[C_CLN1];
+    ctr[C_ANY_CHANGE] = 0;
+  }
 }
 
 // Number of differing bytes of two buffers (out[0], accumulated).
