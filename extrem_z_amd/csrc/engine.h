@@ -76,6 +76,8 @@ enum Ctr {
   C_CLN1,
   C_WL2,          // sides whose decp triplet ends may differ from the original ones
   C_WL1,
+  C_TOUCHED2,     // second touched / dirty list lengths: a list's consumer clears the
+  C_DIRTY2,       // other one, which the next producer then fills (St::tctr / dctr)
   C_COUNT_
 };
 
@@ -104,6 +106,7 @@ struct St {
   uint8_t* dflag;
   int *touched, *dirty;
   int* ctr;
+  int tctr, dctr;       // the touched / dirty list lengths in use (C_TOUCHED or C_TOUCHED2, ...)
   // CP lists (original types; equal to the decp types wherever used)
   int *S2, *S1, *MX, *MN;
   int n2, n1, nmax, nmin;
@@ -212,10 +215,10 @@ __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* 
                            uint16_t* lower);
 __global__ void k_check_full(St s, int* front, int fctr, int ntiles);
 __global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int* front,
-                             int fctr, int epoch);
+                             int fctr, int epoch, int clear_dirty);
 __global__ void k_clear_dirty(St s);
 __global__ void k_propose_c(St s, const int* front, int fctr);
-__global__ void k_apply(St s);
+__global__ void k_apply(St s, int zero_idx);
 __global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t* lower,
                               uint8_t* pdir);
 __global__ void k_chase(int n, int* ds, int* as);

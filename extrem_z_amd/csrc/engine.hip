@@ -315,6 +315,7 @@ class Run {
   void slot_pass(int dec, const double* f, const int* L2, const int* L1, const uint16_t* flow,
                  int* F2, int* La2, int* F1, int* La1, int* smax, int* smin);
   void apply(long long bound = 1LL << 62, int also = -1);
+  void dirty_consumed();
   void label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir = nullptr);
   void c_loops();
   void cp_dec(bool force, bool fresh = false);
@@ -476,11 +477,11 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
 // is emptied (with the counter `also`, when given).
 void Run::apply(long long bound, int also) {
   tbegin(EXTREMZ_K_APPLY);
-  k_apply<<<blocks_for(std::min<long long>(bound, s_.g.N)), 256, 0, st()>>>(s_);
+  k_apply<<<blocks_for(std::min<long long>(bound, s_.g.N)), 256, 0, st()>>>(s_, also);
   tend();
-  if (also >= 0) set_ctr2(C_TOUCHED, also, 0);
-  else set_ctr(C_TOUCHED, 0);
+  s_.tctr = s_.tctr == C_TOUCHED ? C_TOUCHED2 : C_TOUCHED;  // k_apply cleared it
 }
+void Run::dirty_consumed() { s_.dctr = s_.dctr == C_DIRTY ? C_DIRTY2 : C_DIRTY; }
 
 // ComputeDescendingManifold + ComputeAscendingManifold on field f (D7): tile
 // resolution, then a chase of whatever is left (a separate pointer-jumping
@@ -511,14 +512,14 @@ void Run::c_loops() {
     k_check_full<<<std::min(row_blocks(), 8192), dim3(64, 4, 1), 0, st()>>>(
         s_, front_buf_, front_, row_blocks());
     tend();
+    k_clear_dirty<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
   } else {
     tbegin(EXTREMZ_K_CHECK_INCR);
-    k_check_list<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, s_.dirty, C_DIRTY, 1, front_buf_,
-                                                        front_, epoch_);
+    k_check_list<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, s_.dirty, s_.dctr, 1, front_buf_,
+                                                        front_, epoch_, 1);
     tend();
   }
-  k_clear_dirty<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
-  set_ctr(C_DIRTY, 0);
+  dirty_consumed();
   while (true) {
     pull();
     int nf = h(front_);
@@ -536,14 +537,13 @@ void Run::c_loops() {
     epoch_++;
     tbegin(EXTREMZ_K_CHECK_INCR);
     k_check_list<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_, 0, back_buf_, back,
-                                                    epoch_);
+                                                    epoch_, 0);
     tend();
     tbegin(EXTREMZ_K_CHECK_INCR);
-    k_check_list<<<blocks_for(15 * nt), 256, 0, st()>>>(s_, s_.dirty, C_DIRTY, 1, back_buf_,
-                                                         back, epoch_);
+    k_check_list<<<blocks_for(15 * nt), 256, 0, st()>>>(s_, s_.dirty, s_.dctr, 1, back_buf_,
+                                                         back, epoch_, 1);
     tend();
-    k_clear_dirty<<<blocks_for(nt), 256, 0, st()>>>(s_);
-    set_ctr(C_DIRTY, 0);
+    dirty_consumed();
     std::swap(front_buf_, back_buf_);
     front_ = back;
   }
@@ -734,10 +734,11 @@ void Run::r_loops() {
 // get_wrong_index_max / _min of main.cpp:434,441.
 void Run::s_loops(int side, int do_fix) {
   int n = side == 2 ? s_.n2 : s_.n1;
-  if (side == 2) set_ctr2(C_WMC, C_WMC2, 0);
-  else set_ctr2(C_WMINC, C_WMINC2, 0);
   if (do_fix) counts_[EXTREMZ_STAT_S_LOOP_CALLS]++;
-  if (n) {
+  if (!n) {  // (else the first wrong-index launch clears them)
+    if (side == 2) set_ctr2(C_WMC, C_WMC2, 0);
+    else set_ctr2(C_WMINC, C_WMINC2, 0);
+  } else {
     const int b = std::min(blocks_for(n), 512);  // over the listed sides only
     k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 0, 0);
     tbegin(EXTREMZ_K_PROPOSE);
@@ -1009,6 +1010,8 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   s_.touched = c->touched.p;
   s_.dirty = c->dirty.p;
   s_.ctr = c->ctr.p;
+  s_.tctr = C_TOUCHED;
+  s_.dctr = C_DIRTY;
   s_.S2 = c->S2.p;
   s_.S1 = c->S1.p;
   s_.MX = c->MX.p;

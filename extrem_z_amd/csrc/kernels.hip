@@ -341,11 +341,15 @@ __global__ void __launch_bounds__(256) k_check_full(St s, int* front, int fctr, 
 // is evaluated once per epoch; mismatches go to the frontier.  Statuses of
 // vertices outside the edited neighbourhoods cannot change, so old frontier
 // entries plus edited neighbourhoods give the full get_false_criticle_points
-// result.
+// result.  With clear_dirty the list is the dirty list (length ctr[lctr]): its
+// flags are cleared and the other dirty length is set to zero for the next
+// producer.
 __global__ void __launch_bounds__(256) k_check_list(St s, const int* list, int lctr,
                                                     int with_nbrs, int* front, int fctr,
-                                                    int epoch) {
+                                                    int epoch, int clear_dirty) {
   STAGED_LIST(fl, int, kStage, front, &s.ctr[fctr]);
+  if (clear_dirty && blockIdx.x == 0 && threadIdx.x == 0)
+    s.ctr[lctr == C_DIRTY ? C_DIRTY2 : C_DIRTY] = 0;
   // one lane per (entry, candidate): candidate 0 is the entry itself,
   // candidate 1 + d its neighbour in direction d
   const int per = with_nbrs ? kNbr + 1 : 1;
@@ -354,6 +358,7 @@ __global__ void __launch_bounds__(256) k_check_list(St s, const int* list, int l
        k += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(k / per), j = (int)(k - (long long)i * per);
     int u = list[i];
+    if (clear_dirty && j == 0) s.dflag[u] = 0;
     if (j > 0) {
       int x, y, z;
       coords(s.g, u, x, y, z);
@@ -366,9 +371,11 @@ __global__ void __launch_bounds__(256) k_check_list(St s, const int* list, int l
   fl.flush();
 }
 
-// Clears the dirty flags of the listed vertices and the list counter.
+// Clears the dirty flags of the listed vertices; the other dirty length is set
+// to zero for the next producer.
 __global__ void k_clear_dirty(St s) {
-  int n = s.ctr[C_DIRTY];
+  if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr[s.dctr == C_DIRTY ? C_DIRTY2 : C_DIRTY] = 0;
+  int n = s.ctr[s.dctr];
   GRID_STRIDE(i, n) s.dflag[s.dirty[i]] = 0;
 }
 
@@ -376,7 +383,7 @@ __global__ void k_clear_dirty(St s) {
 // c_loop (src/ExtreM.cpp:1152-1236) for false maxima / minima and
 // get_false_stars (src/ExtreM.cpp:1412-1445) for saddle-list entries.
 __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int fctr) {
-  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[C_TOUCHED]);
+  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
   int n = s.ctr[fctr];
   GRID_STRIDE(i, n) {
     int v = front[i];
@@ -446,10 +453,16 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
 // the edit_type 0 branch of applyDeltaBuffer, over the touched list only;
 // resets the delta slot and records changed vertices as dirty (for the next
 // false-critical-point check) and as changed since the last CP pass of decp.
-__global__ void __launch_bounds__(256) k_apply(St s) {
-  STAGED_LIST(dl, int, kStage, s.dirty, &s.ctr[C_DIRTY]);
+// The other touched length (and zero_idx, when given) is set to zero for the
+// next producer.
+__global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
+  STAGED_LIST(dl, int, kStage, s.dirty, &s.ctr[s.dctr]);
   STAGED_LIST(cl, int, kStage, s.lchg, &s.ctr[C_LCHG]);
-  int n = s.ctr[C_TOUCHED];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s.ctr[s.tctr == C_TOUCHED ? C_TOUCHED2 : C_TOUCHED] = 0;
+    if (zero_idx >= 0) s.ctr[zero_idx] = 0;
+  }
+  int n = s.ctr[s.tctr];
   GRID_STRIDE(i, n) {
     int t = s.touched[i];
     double dd = okey_inv(s.delta[t]);
@@ -2036,7 +2049,7 @@ __global__ void k_wn_reset(St s) {
 // fixpath, split (src/ExtreM.cpp:2452-2575): walk the original steepest path
 // from a wrong neighbour until decp's steepest choice differs.
 __global__ void __launch_bounds__(256) k_fixpath(St s) {
-  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[C_TOUCHED]);
+  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
   int n2 = s.ctr[C_NFC], n1 = s.ctr[C_NFC1];
   GRID_STRIDE(e, n2 + n1) {
     int dir = e < n2 ? 0 : 1;
@@ -2089,9 +2102,13 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
   const int* ol = side == 2 ? s.ol2 : s.ol1;
   const int* df = side == 2 ? s.df2 : s.df1;
   const int* dl = side == 2 ? s.dl2 : s.dl1;
-  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[C_TOUCHED]);
+  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
   STAGED_COUNT(cf, &s.ctr[side == 2 ? C_WMC : C_WMINC2]);
   STAGED_COUNT(cl, &s.ctr[side == 2 ? C_WMC2 : C_WMINC]);
+  if (phase == 0 && blockIdx.x == 0 && threadIdx.x == 0) {  // counted by phase 1
+    s.ctr[side == 2 ? C_WMC : C_WMINC2] = 0;
+    s.ctr[side == 2 ? C_WMC2 : C_WMINC] = 0;
+  }
   GRID_STRIDE(k, n) {
     const int i = WL[k];
     int sv = S[i];
@@ -2139,7 +2156,7 @@ __global__ void __launch_bounds__(256) k_wrong_saddle(St s, int side, int do_fix
   const int* ds = side == 2 ? s.dlarg : s.dsmall;
   int og = s.ctr[side == 2 ? C_GMAX : C_GMIN];
   int dg = s.ctr[side == 2 ? C_DGMAX : C_DGMIN];
-  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[C_TOUCHED]);
+  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
   STAGED_COUNT(cnt, &s.ctr[side == 2 ? C_WSC : C_WSCJ]);
   GRID_STRIDE(i, n) {
     int m = E[i];
