@@ -215,7 +215,7 @@ __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* 
                            uint16_t* lower);
 __global__ void k_check_full(St s, int* front, int fctr, int ntiles);
 __global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int* front,
-                             int fctr, int epoch, int clear_dirty);
+                             int fctr, int epoch, int clear_dirty, int zero_idx);
 __global__ void k_clear_dirty(St s);
 __global__ void k_propose_c(St s, const int* front, int fctr);
 __global__ void k_apply(St s, int zero_idx);
@@ -242,7 +242,7 @@ __global__ void k_scan_single(int* a, int n);
 __global__ void k_slot_init(St s, long long* kmax, int* smax, long long* kmin, int* smin);
 __global__ void k_list_fill64(const int* list, int n, long long* arr, long long val);
 __global__ void k_ext_reduce(const int* inmax, int nmax, const int* inmin, int nmin,
-                             const double* f, int* outmax, int* outmin, const int* gate);
+                             const double* f, int* outmax, int* outmin);
 __global__ void k_spos(St s);
 __global__ void k_inc_ptrs(St s, int epoch);
 __global__ void k_inc_taint(St s);

@@ -466,9 +466,9 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
   if (dec) {
     // decp global maximum / minimum among the listed extrema
     k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, f, c_->bmax2.p,
-                                        c_->bmin2.p, nullptr);
+                                        c_->bmin2.p);
     k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, 256, c_->bmin2.p, 256, f, s_.ctr + C_DGMAX,
-                                      s_.ctr + C_DGMIN, nullptr);
+                                      s_.ctr + C_DGMIN);
   }
   tend();
 }
@@ -502,10 +502,11 @@ void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t
 // calls start from the vertices edited since the last clean state.
 void Run::c_loops() {
   counts_[EXTREMZ_STAT_C_LOOPS_CALLS]++;
+  // both frontier lengths are zero here: the correction starts with zeroed
+  // counters and every c_loops ends with both at zero (below)
   front_ = C_FRONT_A;
   front_buf_ = c_->frontA.p;
   back_buf_ = c_->frontB.p;
-  set_ctr2(C_FRONT_A, C_FRONT_B, 0);
   epoch_++;
   if (!clean_) {
     tbegin(EXTREMZ_K_CHECK_FULL);
@@ -516,7 +517,7 @@ void Run::c_loops() {
   } else {
     tbegin(EXTREMZ_K_CHECK_INCR);
     k_check_list<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, s_.dirty, s_.dctr, 1, front_buf_,
-                                                        front_, epoch_, 1);
+                                                        front_, epoch_, 1, -1);
     tend();
   }
   dirty_consumed();
@@ -537,11 +538,13 @@ void Run::c_loops() {
     epoch_++;
     tbegin(EXTREMZ_K_CHECK_INCR);
     k_check_list<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_, 0, back_buf_, back,
-                                                    epoch_, 0);
+                                                    epoch_, 0, -1);
     tend();
+    // (the consumed front length is cleared here, so the loop ends with both
+    // lengths at zero)
     tbegin(EXTREMZ_K_CHECK_INCR);
     k_check_list<<<blocks_for(15 * nt), 256, 0, st()>>>(s_, s_.dirty, s_.dctr, 1, back_buf_,
-                                                         back, epoch_, 1);
+                                                         back, epoch_, 1, front_);
     tend();
     dirty_consumed();
     std::swap(front_buf_, back_buf_);
@@ -599,8 +602,7 @@ void Run::cp_inc(long long nch) {
   k_inc_prop<<<1, 1024, 0, st()>>>(s_, s_.g.N);
   k_inc_mark<<<1024, 256, 0, st()>>>(s_, kmax, kmin);
   k_inc_mark_csr<<<512, 256, 0, st()>>>(s_);
-  // the queued sides, their slots, ties, and the decp global maximum / minimum
-  // when the current one changed value; sizes stay on the device
+  // the queued sides, their slots and ties; sizes stay on the device
   k_inc_saddles<<<kIncSaddleBlocks, 256, 0, st()>>>(s_, s_.decp, c_->pairs.p, c_->pcnt.p, kmax,
                                                      kmin);
   k_inc_pull<<<256, 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
@@ -608,10 +610,6 @@ void Run::cp_inc(long long nch) {
       s_, kmax, kmin, s_.dlarg, s_.dsmall, c_->pkey.p, c_->pid.p);
   k_slot_ids<<<kIncSaddleBlocks, 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p, s_.decp, kmax,
                                                  kmin, s_.dlarg, s_.dsmall);
-  k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, s_.decp, c_->bmax2.p,
-                                      c_->bmin2.p, s_.ctr + C_EXTD);
-  k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, 256, c_->bmin2.p, 256, s_.decp,
-                                    s_.ctr + C_DGMAX, s_.ctr + C_DGMIN, s_.ctr + C_EXTD);
   tend();
   if (inc_verify_) verify_inc();
 }

@@ -346,10 +346,12 @@ __global__ void __launch_bounds__(256) k_check_full(St s, int* front, int fctr, 
 // producer.
 __global__ void __launch_bounds__(256) k_check_list(St s, const int* list, int lctr,
                                                     int with_nbrs, int* front, int fctr,
-                                                    int epoch, int clear_dirty) {
+                                                    int epoch, int clear_dirty, int zero_idx) {
   STAGED_LIST(fl, int, kStage, front, &s.ctr[fctr]);
-  if (clear_dirty && blockIdx.x == 0 && threadIdx.x == 0)
-    s.ctr[lctr == C_DIRTY ? C_DIRTY2 : C_DIRTY] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (clear_dirty) s.ctr[lctr == C_DIRTY ? C_DIRTY2 : C_DIRTY] = 0;
+    if (zero_idx >= 0) s.ctr[zero_idx] = 0;  // a length this launch does not read
+  }
   // one lane per (entry, candidate): candidate 0 is the entry itself,
   // candidate 1 + d its neighbour in direction d
   const int per = with_nbrs ? kNbr + 1 : 1;
@@ -729,9 +731,10 @@ __device__ __forceinline__ int wave_best(int cand, double& fc, bool want_max) {
 __device__ __forceinline__ long long okey_c(double d) { return okey(d + 0.0); }
 
 // Block-wide best (value, id) of one candidate per thread; returns it on
-// thread 0.  All threads of the 256-thread block must call.
-__device__ __forceinline__ int block_best(int cand, double fc, const double* f, bool want_max) {
-  __shared__ int s_best[4];
+// thread 0.  All threads of the 256-thread block must call; s_best is four
+// ints of the calling kernel's LDS.
+__device__ __forceinline__ int block_best(int cand, double fc, const double* f, bool want_max,
+                                          int* s_best) {
   cand = wave_best(cand, fc, want_max);
   int tid = threadIdx.x + blockDim.x * threadIdx.y;
   if ((tid & 63) == 0) s_best[tid >> 6] = cand;
@@ -1226,12 +1229,10 @@ __global__ void __launch_bounds__(1024) k_scan_single(int* a, int n) {
 
 // Best (value, id) of two vertex lists under f: out[block] = the highest of
 // inmax's grid-stride share and the lowest of inmin's (kNone entries skipped).
-// Run over MX / MN, then over the per-block results with one block.  With a
-// gate, nothing is done while *gate is 0 (no listed extremum changed value).
+// Run over MX / MN, then over the per-block results with one block.
 __global__ void __launch_bounds__(256) k_ext_reduce(const int* inmax, int nmax, const int* inmin,
                                                     int nmin, const double* f, int* outmax,
-                                                    int* outmin, const int* gate) {
-  if (gate && *gate == 0) return;
+                                                    int* outmin) {
   int cmax = kNone, cmin = kNone;
   double fmax = 0, fmin = 0;
   GRID_STRIDE(i, nmax) {
@@ -1248,8 +1249,9 @@ __global__ void __launch_bounds__(256) k_ext_reduce(const int* inmax, int nmax, 
       if (cmin == kNone || smaller(fb, b, fmin, cmin)) { cmin = b; fmin = fb; }
     }
   }
-  int gm = block_best(cmax, fmax, f, true);
-  int gn = block_best(cmin, fmin, f, false);
+  __shared__ int s_best[4];
+  int gm = block_best(cmax, fmax, f, true, s_best);
+  int gn = block_best(cmin, fmin, f, false, s_best);
   if (threadIdx.x == 0) {
     outmax[blockIdx.x] = gm;
     outmin[blockIdx.x] = gn;
@@ -1410,10 +1412,15 @@ __global__ void __launch_bounds__(256) k_inc_ptrs(St s, int epoch) {
 // The old roots of the changed pointers (kept in ou / od, flagged in tu / td).
 // Also: a listed extremum other than the current global one changed value,
 // so the global maximum / minimum of decp is the better of it and the current
-// one (when the current one changed, k_inc_ptrs set C_EXTD and k_ext_reduce
-// recomputes both).
+// one (when the current one changed, k_inc_ptrs set C_EXTD, both are cleared
+// here and k_inc_mark recomputes them).
 __global__ void k_inc_taint(St s) {
-  if (!s.ctr[C_EXTD]) {
+  if (s.ctr[C_EXTD]) {  // k_inc_mark recomputes them from scratch
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      s.ctr[C_DGMAX] = kNone;
+      s.ctr[C_DGMIN] = kNone;
+    }
+  } else {
     const int n = s.ctr[C_LCHG];
     GRID_STRIDE(w, n) {
       const int u = s.lchg[w];
@@ -1640,10 +1647,38 @@ __device__ __forceinline__ void mark_side(const St& s, int q2, bool two, L& l2, 
 // Sides of saddles whose star or value changed (al, lchg), of saddles next to
 // a vertex whose root changed (qu: 2-saddles read up roots, qd: 1-saddles
 // down roots), and the changed sides of the last pass (cl2, cl1).  Also the
-// chunks of k_inc_mark_csr, and empty slot keys and slots for this pass's
-// pushes (as k_slot_init; no kernel before the pushes reads them).
+// chunks of k_inc_mark_csr, empty slot keys and slots for this pass's pushes
+// (as k_slot_init; no kernel before the pushes reads them), and the decp
+// global extrema when k_inc_taint cleared them.
 __global__ void __launch_bounds__(256) k_inc_mark(St s, long long* kmax, long long* kmin) {
   if (s.ctr[C_IBAIL]) return;
+  // the global maximum / minimum of decp over the lists, when the current one
+  // changed value (block bests into the cleared counters; the block barriers
+  // are reached by every thread either way, over empty ranges otherwise)
+  {
+    const bool ext = s.ctr[C_EXTD] != 0;
+    const int nmx = ext ? s.nmax : 0, nmn = ext ? s.nmin : 0;
+    int cmax = kNone, cmin = kNone;
+    double fmax = 0.0, fmin = 0.0;
+    GRID_STRIDE(i, nmx) {
+      const int a = s.MX[i];
+      const double fa = s.decp[a];
+      if (cmax == kNone || larger(fa, a, fmax, cmax)) { cmax = a; fmax = fa; }
+    }
+    GRID_STRIDE(i, nmn) {
+      const int b = s.MN[i];
+      const double fb = s.decp[b];
+      if (cmin == kNone || smaller(fb, b, fmin, cmin)) { cmin = b; fmin = fb; }
+    }
+    __shared__ int s_best[4];
+    const int gm = block_best(cmax, fmax, s.decp, true, s_best);
+    const int gn = block_best(cmin, fmin, s.decp, false, s_best);
+    if (threadIdx.x == 0) {
+      if (gm != kNone) cas_best(&s.ctr[C_DGMAX], gm, s.decp, true);
+      if (gn != kNone) cas_best(&s.ctr[C_DGMIN], gn, s.decp, false);
+    }
+  }
+
   STAGED_LIST(l2, int, 1024, s.dsl2, &s.ctr[C_DSL2]);
   STAGED_LIST(l1, int, 1024, s.dsl1, &s.ctr[C_DSL1]);
   const long long nal = s.ctr[C_AL], nch = s.ctr[C_LCHG], nqu = s.ctr[C_QU],
