@@ -145,12 +145,12 @@ struct St {
   int iepoch;                     // stamp of this pass
 };
 
-// Modes of the saddle pass.  kModeOrig: original field, counts each
-// saddle's (extremum, saddle) adjacencies; kModeFill: original field, the
-// adjacencies are written to the per-extremum index and every saddle leaves
-// its key for the gather (k_slot_pull); kModeDec: decp, only saddles whose
-// root set differs from the original one push (the others are gathered).
-constexpr int kModeOrig = 0, kModeFill = 1, kModeDec = 2;
+// Modes of the saddle pass.  kModeOrig: original field, counts and records
+// each saddle's (extremum, saddle) adjacencies (k_csr_fill then builds the
+// per-extremum index) and leaves every saddle's key for the gather
+// (k_slot_pull); kModeDec: decp, only saddles whose root set differs from the
+// original one push (the others are gathered).
+constexpr int kModeOrig = 0, kModeDec = 2;
 
 struct HipError : std::runtime_error {
   explicit HipError(const std::string& m) : std::runtime_error(m) {}
@@ -232,6 +232,7 @@ __global__ void k_slot_ids(St s, const int2* pairs, const int* pcnt, const doubl
                            int* slot_min);
 __global__ void k_ext_index(St s);
 __global__ void k_slot_pull(St s, long long* pkey, int* pid);
+__global__ void k_csr_fill(St s, const int2* pairs, const int* pcnt, int nb);
 __global__ void k_slot_pull_long(St s, long long* ckey, int* cpos);
 __global__ void k_slot_pull_join(St s, const long long* ckey, const int* cpos, long long* pkey,
                                  int* pid);

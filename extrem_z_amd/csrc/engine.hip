@@ -394,7 +394,7 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
   const int ns = s_.nmax + s_.nmin;
   auto saddle_pass = [&](int mode) {
     if (!(nb2 + nb1)) {
-      if (mode != kModeFill) set_ctr(C_PCHUNK, 0);
+      set_ctr(C_PCHUNK, 0);
       return;
     }
     tbegin(EXTREMZ_K_SADDLE_PASS);
@@ -441,8 +441,9 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
     ez::hip_check(hipMemcpyAsync(s_.csr_cur, s_.csr_off, sizeof(int) * (ns + 1),
                                  hipMemcpyDeviceToDevice, st()),
                   "hipMemcpyAsync");
-    saddle_pass(kModeFill);
     tbegin(EXTREMZ_K_SLOT_GATHER);
+    k_csr_fill<<<blocks_for(64LL * (nb2 + nb1)), 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p,
+                                                                 nb2 + nb1);
     k_slot_pull<<<std::max(1, (ns + 31) / 32), 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
     k_slot_pull_long<<<512, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p);
     k_slot_pull_join<<<64, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p, c_->pkey.p, c_->pid.p);

@@ -73,6 +73,22 @@ __device__ __forceinline__ void wave_append(T* list, int* counter, bool pred, T 
   if (pred) list[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
 }
 
+// Wavefront-collective append of cnt entries per lane to the list Q whose
+// length is the LDS counter qn (one LDS atomic per wavefront); returns the
+// lane's first index.  All lanes of the wave must call.
+__device__ __forceinline__ int wave_reserve(int* qn, int cnt) {
+  int pre = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(pre, o);
+    if (__lane_id() >= o) pre += y;
+  }
+  const int total = __shfl(pre, 63);
+  int base = 0;
+  if (__lane_id() == 63 && total) base = atomicAdd(qn, total);
+  return __shfl(base, 63) + pre - cnt;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs, each with its own L2.
 // Returns a logical block index such that every XCD walks one contiguous
 // range of tiles (so z +- 1 neighbour rows are in its own L2).
@@ -884,8 +900,8 @@ __device__ __forceinline__ int slot_index(const St& s, int e) {
 }
 
 // Queues the slots whose original adjacent saddles include the saddle v
-// (its original roots of this side, the extremum->saddle index of the
-// kModeFill pass) for a new gather: v's gather key changed.
+// (its original roots of this side: the extremum->saddle index of the
+// original pass) for a new gather: v's gather key changed.
 template <int SIDE>
 __device__ __forceinline__ void regather_slots(const St& s, int v) {
   int x, y, z;
@@ -933,7 +949,7 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
   int E[kNbr];
   unsigned cand = 0;
   saddle_side<SIDE>(s, dec, f, L, SIDE == 2 ? s.oDS : s.oAS, v, star, ostar, s_off, F, La,
-                    (act && mode != kModeFill) ? i : -1, cand, E);
+                    act ? i : -1, cand, E);
   // list positions of the distinct roots (maxima even, minima odd; roots
   // outside the lists have none) where this saddle pushes or is indexed
   const bool changed = dec && (star != ostar || cand != 0);
@@ -948,18 +964,23 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
     for (int d = 0; d < kNbr; d++) E[d] = -1;
   }
   if (mode == kModeOrig) {
+    // count the (extremum, saddle) adjacencies and record them in the chunk's
+    // region for k_csr_fill; every original saddle is gathered by its extrema
+    int cnt = 0;
 #pragma unroll
-    for (int d = 0; d < kNbr; d++) {
-      if (E[d] >= 0) atomicAdd(&s.csr_off[slot_index(s, E[d])], 1);
-      E[d] = -1;
-    }
-  } else if (mode == kModeFill) {
+    for (int d = 0; d < kNbr; d++)
+      if (E[d] >= 0) {
+        E[d] = slot_index(s, E[d]);
+        atomicAdd(&s.csr_off[E[d]], 1);
+        cnt++;
+      }
+    int b = wave_reserve(s_np, cnt);
 #pragma unroll
-    for (int d = 0; d < kNbr; d++) {
-      if (E[d] >= 0) s.csr[atomicAdd(&s.csr_cur[slot_index(s, E[d])], 1)] = i;
-      E[d] = -1;
-    }
-    // every original saddle is gathered by its extrema
+    for (int d = 0; d < kNbr; d++)
+      if (E[d] >= 0) {
+        cpairs[b++] = make_int2(E[d], i);
+        E[d] = -1;
+      }
     if (act) (SIDE == 2 ? s.pv2 : s.pv1)[i] = okey_c(fs);
   } else {
     // same star and the same root at every star neighbour: the root set is
@@ -995,9 +1016,8 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
 // order is remapped so each XCD walks one contiguous range (list order is
 // spatial order, so the neighbour rows of a range stay in that XCD's L2).
 // Triplet ends F2 / La2 / F1 / La1 are written at the list position, the
-// slot pairs of chunk c to its own region (count pcnt[c]), so no two blocks
-// share a counter for them; kModeFill records no pairs and leaves pcnt as
-// the kModeOrig pass wrote it.
+// slot pairs of chunk c (in kModeOrig its (slot, position) adjacencies) to
+// its own region (count pcnt[c]), so no two blocks share a counter for them.
 __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const double* f,
                                                      const int* L2, const int* L1,
                                                      const uint16_t* flow, int* F2, int* La2,
@@ -1013,7 +1033,7 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const doubl
   STAGED_LIST(sw1, int, 1024, s.wl1, &s.ctr[C_WL1]);
   const int tid = threadIdx.x;
   if (tid < kNbr) s_off[tid] = s.g.off[tid];
-  if (blockIdx.x == 0 && tid == 0 && mode != kModeFill) s.ctr[C_PCHUNK] = nb;
+  if (blockIdx.x == 0 && tid == 0) s.ctr[C_PCHUNK] = nb;
   const int per = (nb + gridDim.x - 1) / gridDim.x;
   const int c0 = xcd_remap(blockIdx.x, gridDim.x) * per, c1 = min(nb, c0 + per);
   for (int c = c0; c < c1; c++) {
@@ -1030,13 +1050,30 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const doubl
                       sc1, sw1);
     }
     __syncthreads();
-    if (tid == 0 && mode != kModeFill) pcnt[c] = s_np;
+    if (tid == 0) pcnt[c] = s_np;
   }
   sa.flush();
   sc2.flush();
   sc1.flush();
   sw2.flush();
   sw1.flush();
+}
+
+// The original extremum->saddle index from the adjacencies the kModeOrig
+// pass recorded per chunk (one wavefront per chunk; the order within a slot's
+// range is free: every reader takes a best over it).
+__global__ void __launch_bounds__(256) k_csr_fill(St s, const int2* pairs, const int* pcnt,
+                                                  int nb) {
+  const int lane = threadIdx.x & 63;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < nb; c += nw) {
+    const int n = pcnt[c];
+    const int2* P = pairs + (long long)c * kChunkPairs;
+    for (int j = lane; j < n; j += 64) {
+      const int2 pr = P[j];
+      s.csr[atomicAdd(&s.csr_cur[pr.x], 1)] = pr.y;
+    }
+  }
 }
 
 // Slot gather for the unchanged saddles of a decp pass: for slot index j
@@ -1492,22 +1529,6 @@ __global__ void __launch_bounds__(256) k_inc_walk(St s) {
       s.ctr[C_SWEEP] = 0;
     }
   }
-}
-
-// Wavefront-collective append of cnt entries per lane to the list Q whose
-// length is the LDS counter qn (one LDS atomic per wavefront); returns the
-// lane's first index.  All lanes of the wave must call.
-__device__ __forceinline__ int wave_reserve(int* qn, int cnt) {
-  int pre = cnt;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(pre, o);
-    if (__lane_id() >= o) pre += y;
-  }
-  const int total = __shfl(pre, 63);
-  int base = 0;
-  if (__lane_id() == 63 && total) base = atomicAdd(qn, total);
-  return __shfl(base, 63) + pre - cnt;
 }
 
 // The children of u (neighbours whose pointer of this side names u) that are
