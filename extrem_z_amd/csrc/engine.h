@@ -216,7 +216,7 @@ __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* 
 __global__ void k_check_full(St s, int* front, int fctr, int ntiles);
 __global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int* front,
                              int fctr, int epoch, int clear_dirty, int zero_idx);
-__global__ void k_clear_dirty(St s);
+__global__ void k_clear_dirty(St s, int zero_idx);
 __global__ void k_propose_c(St s, const int* front, int fctr);
 __global__ void k_apply(St s, int zero_idx);
 __global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t* lower,

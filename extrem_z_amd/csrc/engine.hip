@@ -514,7 +514,7 @@ void Run::c_loops() {
     k_check_full<<<std::min(row_blocks(), 8192), dim3(64, 4, 1), 0, st()>>>(
         s_, front_buf_, front_, row_blocks());
     tend();
-    k_clear_dirty<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
+    k_clear_dirty<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, -1);
   } else {
     tbegin(EXTREMZ_K_CHECK_INCR);
     k_check_list<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, s_.dirty, s_.dctr, 1, front_buf_,
@@ -537,16 +537,26 @@ void Run::c_loops() {
     int back = front_ == C_FRONT_A ? C_FRONT_B : C_FRONT_A;
     apply(nt, back);
     epoch_++;
-    tbegin(EXTREMZ_K_CHECK_INCR);
-    k_check_list<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_, 0, back_buf_, back,
-                                                    epoch_, 0, -1);
-    tend();
-    // (the consumed front length is cleared here, so the loop ends with both
-    // lengths at zero)
-    tbegin(EXTREMZ_K_CHECK_INCR);
-    k_check_list<<<blocks_for(15 * nt), 256, 0, st()>>>(s_, s_.dirty, s_.dctr, 1, back_buf_,
-                                                         back, epoch_, 1, front_);
-    tend();
+    if (nt >= s_.g.N / 8) {
+      // many edits: one full scan gives the same set as the re-checks below
+      // (old entries that stay false plus the edited neighbourhoods), faster
+      tbegin(EXTREMZ_K_CHECK_FULL);
+      k_check_full<<<std::min(row_blocks(), 8192), dim3(64, 4, 1), 0, st()>>>(
+          s_, back_buf_, back, row_blocks());
+      tend();
+      k_clear_dirty<<<blocks_for(nt), 256, 0, st()>>>(s_, front_);
+    } else {
+      tbegin(EXTREMZ_K_CHECK_INCR);
+      k_check_list<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_, 0, back_buf_, back,
+                                                      epoch_, 0, -1);
+      tend();
+      // (the consumed front length is cleared here, so the loop ends with both
+      // lengths at zero)
+      tbegin(EXTREMZ_K_CHECK_INCR);
+      k_check_list<<<blocks_for(15 * nt), 256, 0, st()>>>(s_, s_.dirty, s_.dctr, 1, back_buf_,
+                                                           back, epoch_, 1, front_);
+      tend();
+    }
     dirty_consumed();
     std::swap(front_buf_, back_buf_);
     front_ = back;

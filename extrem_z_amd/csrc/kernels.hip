@@ -389,10 +389,13 @@ __global__ void __launch_bounds__(256) k_check_list(St s, const int* list, int l
   fl.flush();
 }
 
-// Clears the dirty flags of the listed vertices; the other dirty length is set
-// to zero for the next producer.
-__global__ void k_clear_dirty(St s) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr[s.dctr == C_DIRTY ? C_DIRTY2 : C_DIRTY] = 0;
+// Clears the dirty flags of the listed vertices; the other dirty length (and
+// zero_idx, when given) is set to zero for the next producer.
+__global__ void k_clear_dirty(St s, int zero_idx) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s.ctr[s.dctr == C_DIRTY ? C_DIRTY2 : C_DIRTY] = 0;
+    if (zero_idx >= 0) s.ctr[zero_idx] = 0;
+  }
   int n = s.ctr[s.dctr];
   GRID_STRIDE(i, n) s.dflag[s.dirty[i]] = 0;
 }
@@ -455,12 +458,30 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
     bool sad = (dt != ot) ? (is_saddle_type(dt) || is_saddle_type(ot))
                           : (is_saddle_type(ot) && dl != ol);
     if (sad && dl != ol) {  // get_false_stars
-      unsigned a = ol & ~dl;
+      // propose() for every star neighbour, with all loads and atomics of the
+      // star issued before any result is used
+      const unsigned a = ol & ~dl;
+      double po[kNbr], pd[kNbr];
+#pragma unroll
+      for (int d = 0; d < kNbr; d++) {
+        const bool on = (a >> d) & 1;
+        po[d] = on ? s.orig[v + s.g.off[d]] : 0.0;
+        pd[d] = on ? s.decp[v + s.g.off[d]] : 0.0;
+      }
+      long long old[kNbr];
+      unsigned sent = 0;  // atomics issued
+#pragma unroll
+      for (int d = 0; d < kNbr; d++) {
+        old[d] = 0;
+        if (!((a >> d) & 1)) continue;
+        const long long k = okey(s.step_q ? s.step : (po[d] - s.xi) - pd[d]);
+        if (k <= s.sent) continue;
+        old[d] = atomicMax(&s.delta[v + s.g.off[d]], k);
+        sent |= 1u << d;
+      }
+#pragma unroll
       for (int d = 0; d < kNbr; d++)
-        if ((a >> d) & 1) {
-          int r = v + s.g.off[d];
-          propose(s, r, (s.orig[r] - s.xi) - s.decp[r], tl);
-        }
+        if (((sent >> d) & 1) && old[d] == s.sent) tl.push(v + s.g.off[d]);
       if (dl & ~ol) propose(s, v, (s.orig[v] - s.xi) - s.decp[v], tl);
     }
   }
