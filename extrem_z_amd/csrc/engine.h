@@ -126,6 +126,7 @@ struct St {
   // Incremental CP passes of decp: what changed since the last one, and the
   // per-pass work lists.
   uint8_t* pdir;                  // steepest (up + 1) | (down + 1) << 4 per voxel; 0: itself
+  uint8_t* opdir;                 // the same for the original field
   uint8_t* lflag;                 // decp changed since the last CP pass of decp
   int* lchg;                      // those vertices (C_LCHG)
   uint8_t* cflag;                 // 1: up pointer changed in this pass, 2: down pointer
@@ -265,7 +266,7 @@ __global__ void k_wn_commit(St s, int2* active, int actr);
 __global__ void k_wn_next(St s, const int2* active, int actr, int2* next, int nctr);
 __global__ void k_wn_reset(St s);
 __global__ void k_wn_solve(St s, int2* bufA, int2* bufB, int limit);
-__global__ void k_fixpath(St s);
+__global__ void k_fixpath(St s, int fast);
 __global__ void k_wrong_index(St s, int side, int phase, int do_fix);
 __global__ void k_wrong_saddle(St s, int side, int do_fix);
 __global__ void k_lists_count(St s, int* cnt, int nb, int tiled);

@@ -160,7 +160,7 @@ struct extremz_ctx {
   ez::DevBuf<double> orig_tmp, decp_tmp;
   ez::DevBuf<uint8_t> counts_tmp;
   // incremental CP passes of decp
-  ez::DevBuf<uint8_t> pdir, lflag, cflag, tu, td, wflag;
+  ez::DevBuf<uint8_t> pdir, opdir, lflag, cflag, tu, td, wflag;
   ez::DevBuf<int> lchg, spos2, spos1, ilist, qu, qd, sstamp, slists, slstamp, dslot, pcnt;
   ez::DevBuf<uint8_t> vsnap;  // snapshots of the self-check (EXTREMZ_VERIFY_INC)
   int* h_ctr = nullptr;
@@ -182,7 +182,7 @@ struct extremz_ctx {
       f(*b);
     for (ez::DevBuf<long long>* b : {&delta, &kmax, &kmin, &pkey, &ckey, &pv2, &pv1}) f(*b);
     for (ez::DevBuf<uint8_t>* b :
-         {&cnt, &otype, &dflag, &wn2, &wn1, &counts_tmp, &pdir, &lflag, &cflag, &tu, &td, &wflag,
+         {&cnt, &otype, &dflag, &wn2, &wn1, &counts_tmp, &pdir, &opdir, &lflag, &cflag, &tu, &td, &wflag,
           &vsnap})
       f(*b);
     for (ez::DevBuf<uint16_t>* b : {&olow, &dlow}) f(*b);
@@ -368,7 +368,8 @@ void Run::allocate(long long N) {
                          &c->win_a, &c->win_b, &c->of2, &c->ol2, &c->of1, &c->ol1, &c->df2,
                          &c->dl2, &c->df1, &c->dl1, &c->wnl2, &c->wnl1})
     b->ensure(N);
-  for (DevBuf<uint8_t>* b : {&c->pdir, &c->lflag, &c->cflag, &c->tu, &c->td}) b->ensure(N);
+  for (DevBuf<uint8_t>* b : {&c->pdir, &c->opdir, &c->lflag, &c->cflag, &c->tu, &c->td})
+    b->ensure(N);
   for (DevBuf<int>* b : {&c->lchg, &c->spos2, &c->spos1, &c->qu, &c->qd}) b->ensure(N);
   c->ilist.ensure(5 * inc_cap(N));
   c->bmax2.ensure(256);
@@ -731,10 +732,13 @@ void Run::wrong_neighbors() {
 }
 
 // r_loops, src/ExtreM.cpp:2577-2593.
+// Called right after a counter read (the outer loop's test), which says
+// whether decp changed since its last CP pass.
 void Run::r_loops() {
   counts_[EXTREMZ_STAT_R_LOOP_CALLS]++;
+  const int fast = inc_ok_ && !h(C_ANY_CHANGE);
   tbegin(EXTREMZ_K_PROPOSE);
-  k_fixpath<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_);
+  k_fixpath<<<blocks_for(std::max(1, h(C_NFC) + h(C_NFC1))), 256, 0, st()>>>(s_, fast);
   tend();
   apply();
 }
@@ -1048,6 +1052,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   s_.win_a = c->win_a.p;
   s_.win_b = c->win_b.p;
   s_.pdir = c->pdir.p;
+  s_.opdir = c->opdir.p;
   s_.lflag = c->lflag.p;
   s_.lchg = c->lchg.p;
   s_.cflag = c->cflag.p;
@@ -1093,7 +1098,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   tbegin(EXTREMZ_K_CLASSIFY);
   k_classify<<<row_blocks(), dim3(64, 4, 1), 0, q>>>(s_.g, s_.lut, d_orig, s_.otype, s_.olow);
   tend();
-  label_pass(d_orig, s_.oDS, s_.oAS, nullptr);
+  label_pass(d_orig, s_.oDS, s_.oAS, nullptr, s_.opdir);
   // main.cpp:100-119: CP lists, triplets, largest/smallest saddle maps of orig
   list_compaction();
   pull();

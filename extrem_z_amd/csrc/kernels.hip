@@ -2125,13 +2125,30 @@ __global__ void k_wn_reset(St s) {
 // ------------------------------------------------------------ fixpath ----
 // fixpath, split (src/ExtreM.cpp:2452-2575): walk the original steepest path
 // from a wrong neighbour until decp's steepest choice differs.
-__global__ void __launch_bounds__(256) k_fixpath(St s) {
+// With fast, decp has not changed since its last CP pass, so the steepest
+// directions of both fields are the stored ones (opdir, pdir) and each step
+// is two byte loads instead of the 28 values of the stencil.
+__global__ void __launch_bounds__(256) k_fixpath(St s, int fast) {
   STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
   int n2 = s.ctr[C_NFC], n1 = s.ctr[C_NFC1];
   GRID_STRIDE(e, n2 + n1) {
     int dir = e < n2 ? 0 : 1;
     int cur = dir == 0 ? s.wnl2[e] : s.wnl1[e - n2];
     int tru = kNone, fal = kNone;
+    if (fast) {
+      const int sh = dir == 0 ? 0 : 4;
+      for (int step = 0; step <= s.g.N; step++) {
+        const int a = (s.opdir[cur] >> sh) & 15, b = (s.pdir[cur] >> sh) & 15;
+        const int nx = a ? cur + s.g.off[a - 1] : cur, dn = b ? cur + s.g.off[b - 1] : cur;
+        if (nx != dn) {
+          tru = nx;
+          fal = dn;
+          break;
+        }
+        if (nx == cur) break;
+        cur = nx;
+      }
+    } else
     for (int step = 0; step <= s.g.N; step++) {
       int x, y, z;
       coords(s.g, cur, x, y, z);
