@@ -559,6 +559,7 @@ __global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
 // so every (value, id) comparison of src/ExtreM.cpp:153 / 511 is one compare:
 // scanning in id order, ">=" keeps the highest (value, id), "<" the lowest.
 constexpr int kLTX = 64, kLTY = 4, kLTZ = 4, kLTN = kLTX * kLTY * kLTZ;
+constexpr int kFinal = (int)0x80000000u;  // label bit: a root found inside the tile
 constexpr int kPX = kLTX + 2, kPY = kLTY + 2, kPN = kPX * kPY;
 constexpr int kPPer = (kPN + 255) / 256;
 
@@ -700,10 +701,12 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
     int x = x0 + li % kLTX, y = y0 + (li / kLTX) % kLTY, z = z0 + li / (kLTX * kLTY);
     if (x >= g.W || y >= g.H || z >= g.D) continue;
     int v = x + g.W * (y + g.H * z);
+    // a root inside the tile is final: written with the top bit set, which
+    // k_chase clears without a gather
     int p = sup[li];
-    ds[v] = p >= 0 ? (x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY))) : ~p;
+    ds[v] = p >= 0 ? kFinal | ((x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY)))) : ~p;
     p = sdn[li];
-    as[v] = p >= 0 ? (x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY))) : ~p;
+    as[v] = p >= 0 ? kFinal | ((x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY)))) : ~p;
   }
 }
 
@@ -711,11 +714,22 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
 // vertex is compressed).  Other lanes' writes only ever replace a pointer by
 // one further along the same path, so any value read is an ancestor; the
 // walk is bounded by N.
+// A value with the top bit set (kFinal, from k_labels_tile) is a root:
+// its own lane clears the bit, and a walker that reads it stops there.
 __device__ __forceinline__ void chase_one(int* S, int v, int n) {
   int r = S[v];
+  if (r < 0) {  // resolved inside its tile
+    S[v] = r & ~kFinal;
+    return;
+  }
   int rr = __hip_atomic_load(&S[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (rr == r) return;
   for (int step = 0; step < n && rr != r; step++) {
+    if (rr < 0) {
+      rr &= ~kFinal;
+      r = rr;
+      break;
+    }
     r = rr;
     rr = __hip_atomic_load(&S[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
