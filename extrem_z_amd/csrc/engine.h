@@ -214,7 +214,9 @@ __global__ void k_f32_to_f64(const float* in, double* out, long long n);
 __global__ void k_check_finite(const double* a, long long n, int* ctr);
 __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* type,
                            uint16_t* lower);
-__global__ void k_check_full(St s, int* front, int fctr, int ntiles);
+template <int MODE>
+__global__ void k_mask_cols(St s, const double* f, uint8_t* type, uint16_t* lower, int* front,
+                            int fctr, int nzc, int zlen);
 __global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int* front,
                              int fctr, int epoch, int clear_dirty, int zero_idx);
 __global__ void k_clear_dirty(St s, int zero_idx);

@@ -251,8 +251,28 @@ class Run {
     s_.g.N = W * H * D;
     for (int d = 0; d < kNbr; d++) s_.g.off[d] = kDx[d] + W * (kDy[d] + H * kDz[d]);
   }
-  // 64x4 row blocks (k_classify, k_check_full)
+  // 64x4 row blocks (k_classify)
   int row_blocks() const { return ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4) * s_.g.D; }
+  // 64x4 columns cut into nzc chunks of zlen planes, about kColBlocks blocks
+  // (k_mask_cols)
+  struct ColGrid {
+    int blocks, nzc, zlen;
+  };
+  ColGrid col_grid() const {
+    const int ncol = ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4);
+    int nzc = std::max(1, std::min(s_.g.D, (kColBlocks + ncol - 1) / ncol));
+    const int zlen = (s_.g.D + nzc - 1) / nzc;
+    nzc = (s_.g.D + zlen - 1) / zlen;
+    return {ncol * nzc, nzc, zlen};
+  }
+  // one full false-critical-point scan of decp into the frontier list fbuf
+  void check_full(int* fbuf, int fctr) {
+    const ColGrid cg = col_grid();
+    tbegin(EXTREMZ_K_CHECK_FULL);
+    k_mask_cols<1><<<cg.blocks, dim3(64, 4, 1), 0, st()>>>(s_, s_.decp, nullptr, nullptr, fbuf,
+                                                            fctr, cg.nzc, cg.zlen);
+    tend();
+  }
   // 64x4x4 label tiles
   int tile_blocks() const {
     return ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4) * ((s_.g.D + 3) / 4);
@@ -268,6 +288,8 @@ class Run {
   // Blocks of a saddle launch (each takes a contiguous range of 256-side
   // chunks).
   static constexpr int kSaddleBlocks = 65536;
+  // Blocks of a column sweep (one frontier append each).
+  static constexpr int kColBlocks = 4096;
   // Blocks of the incremental saddle launches (sizes known on the device only).
   static constexpr int kIncSaddleBlocks = 1024;
   static int blocks_for(long long n) {
@@ -511,10 +533,7 @@ void Run::c_loops() {
   back_buf_ = c_->frontB.p;
   epoch_++;
   if (!clean_) {
-    tbegin(EXTREMZ_K_CHECK_FULL);
-    k_check_full<<<std::min(row_blocks(), 8192), dim3(64, 4, 1), 0, st()>>>(
-        s_, front_buf_, front_, row_blocks());
-    tend();
+    check_full(front_buf_, front_);
     k_clear_dirty<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, -1);
   } else {
     tbegin(EXTREMZ_K_CHECK_INCR);
@@ -541,10 +560,7 @@ void Run::c_loops() {
     if (nt >= s_.g.N / 8) {
       // many edits: one full scan gives the same set as the re-checks below
       // (old entries that stay false plus the edited neighbourhoods), faster
-      tbegin(EXTREMZ_K_CHECK_FULL);
-      k_check_full<<<std::min(row_blocks(), 8192), dim3(64, 4, 1), 0, st()>>>(
-          s_, back_buf_, back, row_blocks());
-      tend();
+      check_full(back_buf_, back);
       k_clear_dirty<<<blocks_for(nt), 256, 0, st()>>>(s_, front_);
     } else {
       tbegin(EXTREMZ_K_CHECK_INCR);
@@ -1096,7 +1112,11 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   k_argext<<<bN, 256, 0, q>>>(d_orig, N, s_.ctr + C_GMIN, s_.ctr + C_GMAX);
   // main.cpp:85-89
   tbegin(EXTREMZ_K_CLASSIFY);
-  k_classify<<<row_blocks(), dim3(64, 4, 1), 0, q>>>(s_.g, s_.lut, d_orig, s_.otype, s_.olow);
+  {
+    const ColGrid cg = col_grid();
+    k_mask_cols<0><<<cg.blocks, dim3(64, 4, 1), 0, q>>>(s_, d_orig, s_.otype, s_.olow, nullptr,
+                                                         0, cg.nzc, cg.zlen);
+  }
   tend();
   label_pass(d_orig, s_.oDS, s_.oAS, nullptr, s_.opdir);
   // main.cpp:100-119: CP lists, triplets, largest/smallest saddle maps of orig

@@ -328,31 +328,6 @@ __global__ void __launch_bounds__(256) k_classify(Geo g, const uint8_t* lut, con
   lower[v] = (uint16_t)lm;
 }
 
-// Full-grid false-critical-point scan (the first get_false_criticle_points
-// of c_loops, src/ExtreM.cpp:1513-1519); appends to the frontier.
-// Each block takes a contiguous range of the 64x4 row tiles (one frontier
-// append per block, not per tile: a shared counter taken by every tile's
-// block serialises).
-__global__ void __launch_bounds__(256) k_check_full(St s, int* front, int fctr, int ntiles) {
-  STAGED_LIST(fl, int, 2048, front, &s.ctr[fctr]);
-  const int per = (ntiles + gridDim.x - 1) / gridDim.x;
-  const int t0 = xcd_remap(blockIdx.x, gridDim.x) * per, t1 = min(ntiles, t0 + per);
-  for (int t = t0; t < t1; t++) {
-    const RowPos p = row_pos_at(s.g, t);
-    if (!p.in) continue;
-    const int v = p.x + s.g.W * (p.y + s.g.H * p.z);
-    const int cls = class_of(s.g, p.x, p.y, p.z);
-    const unsigned dl = lower_mask_at(s.g, s.decp, v, cls);
-    const unsigned ol = s.olow[v];
-    if (dl != ol) {
-      const int ot = s.otype[v];
-      const int dt = type_of_mask(s.lut, cls, dl);
-      if (dt != ot || is_saddle_type(ot)) fl.push(v);
-    }
-  }
-  fl.flush();
-}
-
 // Re-check after edits: every list entry (and, with nbrs, its 14 neighbours)
 // is evaluated once per epoch; mismatches go to the frontier.  Statuses of
 // vertices outside the edited neighbourhoods cannot change, so old frontier
@@ -709,6 +684,96 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
     as[v] = p >= 0 ? kFinal | ((x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY)))) : ~p;
   }
 }
+
+// Full-grid lower masks through the same LDS ring as k_labels_tile: a block
+// marches one 64x4 column through zlen planes (z chunk zc of the column), so
+// each decp value is read once per plane and the halo planes are shared.  A
+// present direction d names a neighbour with a larger id for even d and a
+// smaller one for odd d, so a tie counts as lower exactly for odd d
+// (src/ExtreM.cpp:577-583).  MODE 0: vertex type and mask (classifyVertex,
+// src/ExtreM_cuda.cu:3796-3853) into type / lower.  MODE 1: the full
+// false-critical-point scan (the first get_false_criticle_points of c_loops,
+// src/ExtreM.cpp:1513-1519) appending to the frontier, one append per block.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_t* type,
+                                                   uint16_t* lower, int* front, int fctr,
+                                                   int nzc, int zlen) {
+  __shared__ double ring[3][kPN];
+  STAGED_LIST(fl, int, 2048, front, &s.ctr[fctr]);
+  const Geo& g = s.g;
+  const int ntx = (g.W + kLTX - 1) / kLTX;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int col = L / nzc;
+  const int x0 = (col % ntx) * kLTX, y0 = (col / ntx) * kLTY;
+  const int z0 = (L % nzc) * zlen, z1 = min(g.D, z0 + zlen);
+  const int lx = threadIdx.x, ly = threadIdx.y, tid = lx + 64 * ly;
+  double pre[kPPer];
+  auto fetch = [&](int z) {
+#pragma unroll
+    for (int j = 0; j < kPPer; j++) {
+      int i = tid + 256 * j;
+      double val = 0.0;
+      if (i < kPN) {
+        int px = i % kPX, py = i / kPX, gx = x0 + px - 1, gy = y0 + py - 1;
+        if (z >= 0 && z < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
+          val = f[gx + g.W * (gy + g.H * z)];
+      }
+      pre[j] = val;
+    }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int j = 0; j < kPPer; j++) {
+      int i = tid + 256 * j;
+      if (i < kPN) ring[slot][i] = pre[j];
+    }
+  };
+  if (z0 < z1) {
+    fetch(z0 - 1);
+    store(0);
+    fetch(z0);
+    store(1);
+    fetch(z0 + 1);
+  }
+  const int x = x0 + lx, y = y0 + ly;
+  const int c0 = (lx + 1) + kPX * (ly + 1);
+  for (int z = z0; z < z1; z++) {
+    const int lz = z - z0;
+    store((lz + 2) % 3);
+    __syncthreads();
+    if (z + 2 < z1 + 1) fetch(z + 2);
+    const double* P0 = ring[lz % 3];        // z - 1
+    const double* P1 = ring[(lz + 1) % 3];  // z
+    const double* P2 = ring[(lz + 2) % 3];  // z + 1
+    if (x < g.W && y < g.H) {
+      const int v = x + g.W * (y + g.H * z);
+      const int cls = class_of(g, x, y, z);
+      const unsigned key = c_cls_key[cls];
+      const double c = P1[c0];
+      unsigned lm = 0;
+#pragma unroll
+      for (int d = 0; d < kNbr; d++) {
+        const double* P = kSZ[d] < 0 ? P0 : (kSZ[d] > 0 ? P2 : P1);
+        const double fr = P[c0 + kSX[d] + kPX * kSY[d]];
+        const bool lo = fr < c || ((d & 1) && fr == c);
+        lm |= (unsigned)(((key >> d) & 1) && lo) << d;
+      }
+      if (MODE == 0) {
+        type[v] = (uint8_t)type_of_mask(s.lut, cls, lm);
+        lower[v] = (uint16_t)lm;
+      } else if (lm != s.olow[v]) {
+        const int ot = s.otype[v];
+        if (type_of_mask(s.lut, cls, lm) != ot || is_saddle_type(ot)) fl.push(v);
+      }
+    }
+    __syncthreads();
+  }
+  fl.flush();
+}
+template __global__ void k_mask_cols<0>(St, const double*, uint8_t*, uint16_t*, int*, int, int,
+                                        int);
+template __global__ void k_mask_cols<1>(St, const double*, uint8_t*, uint16_t*, int*, int, int,
+                                        int);
 
 // Follows every pointer that is not yet at a root to its root (D7: every
 // vertex is compressed).  Other lanes' writes only ever replace a pointer by
