@@ -226,7 +226,8 @@ __global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t
                               uint8_t* pdir);
 __global__ void k_chase(int n, int* ds, int* as);
 __global__ void k_list_fill(const int* list, int n, int* arr, int val);
-__global__ void k_saddle_list(St s, int mode, const double* f, const int* L2, const int* L1,
+template <int MODE>
+__global__ void k_saddle_list(St s, const double* f, const int* L2, const int* L1,
                               const uint16_t* flow, int* F2, int* La2, int* F1, int* La1,
                               int2* pairs, int* pcnt, long long* kmax, long long* kmin, int nb2,
                               int nb);

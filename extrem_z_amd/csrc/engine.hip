@@ -421,8 +421,9 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
       return;
     }
     tbegin(EXTREMZ_K_SADDLE_PASS);
-    k_saddle_list<<<std::min(nb2 + nb1, kSaddleBlocks), 256, 0, st()>>>(
-        s_, mode, f, L2, L1, flow, F2, La2, F1, La1, c_->pairs.p, c_->pcnt.p, kmax, kmin, nb2,
+    auto* kern = mode == kModeOrig ? k_saddle_list<kModeOrig> : k_saddle_list<kModeDec>;
+    kern<<<std::min(nb2 + nb1, kSaddleBlocks), 256, 0, st()>>>(
+        s_, f, L2, L1, flow, F2, La2, F1, La1, c_->pairs.p, c_->pcnt.p, kmax, kmin, nb2,
         nb2 + nb1);
     tend();
   };

@@ -1118,12 +1118,12 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
 // Triplet ends F2 / La2 / F1 / La1 are written at the list position, the
 // slot pairs of chunk c (in kModeOrig its (slot, position) adjacencies) to
 // its own region (count pcnt[c]), so no two blocks share a counter for them.
-__global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const double* f,
-                                                     const int* L2, const int* L1,
-                                                     const uint16_t* flow, int* F2, int* La2,
-                                                     int* F1, int* La1, int2* pairs, int* pcnt,
-                                                     long long* kmax, long long* kmin, int nb2,
-                                                     int nb) {
+template <int MODE>
+__global__ void __launch_bounds__(256, 4) k_saddle_list(St s, const double* f, const int* L2,
+                                                     const int* L1, const uint16_t* flow,
+                                                     int* F2, int* La2, int* F1, int* La1,
+                                                     int2* pairs, int* pcnt, long long* kmax,
+                                                     long long* kmin, int nb2, int nb) {
   __shared__ int s_off[kNbr];
   __shared__ int s_np;
   STAGED_LIST(sa, int2, 1024, s.act0, &s.ctr[C_ACT0]);
@@ -1142,11 +1142,11 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const doubl
     int2* cp = pairs + (long long)c * kChunkPairs;
     if (c < nb2) {
       const int i = c * 256 + tid;
-      saddle_entry<2>(s, mode, f, L2, flow, F2, La2, i, i < s.n2, s_off, cp, &s_np, kmax, 0, sa,
+      saddle_entry<2>(s, MODE, f, L2, flow, F2, La2, i, i < s.n2, s_off, cp, &s_np, kmax, 0, sa,
                       sc2, sw2);
     } else {
       const int i = (c - nb2) * 256 + tid;
-      saddle_entry<1>(s, mode, f, L1, flow, F1, La1, i, i < s.n1, s_off, cp, &s_np, kmin, 0, sa,
+      saddle_entry<1>(s, MODE, f, L1, flow, F1, La1, i, i < s.n1, s_off, cp, &s_np, kmin, 0, sa,
                       sc1, sw1);
     }
     __syncthreads();
@@ -1158,6 +1158,12 @@ __global__ void __launch_bounds__(256) k_saddle_list(St s, int mode, const doubl
   sw2.flush();
   sw1.flush();
 }
+template __global__ void k_saddle_list<kModeOrig>(St, const double*, const int*, const int*,
+                                                  const uint16_t*, int*, int*, int*, int*, int2*,
+                                                  int*, long long*, long long*, int, int);
+template __global__ void k_saddle_list<kModeDec>(St, const double*, const int*, const int*,
+                                                 const uint16_t*, int*, int*, int*, int*, int2*,
+                                                 int*, long long*, long long*, int, int);
 
 // The original extremum->saddle index from the adjacencies the kModeOrig
 // pass recorded per chunk (one wavefront per chunk; the order within a slot's
@@ -1912,7 +1918,7 @@ __global__ void __launch_bounds__(256) k_inc_mark_csr(St s) {
 // marking), in chunks of 256 queued sides (S2 sides first, each part rounded
 // up to whole chunks), a contiguous range of chunks per block, slot pairs in
 // the chunks' own regions as in k_saddle_list.
-__global__ void __launch_bounds__(256) k_inc_saddles(St s, const double* f, int2* pairs, int* pcnt,
+__global__ void __launch_bounds__(256, 4) k_inc_saddles(St s, const double* f, int2* pairs, int* pcnt,
                                                      long long* kmax, long long* kmin) {
   __shared__ int s_off[kNbr];
   __shared__ int s_np;
