@@ -217,6 +217,7 @@ __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* 
 template <int MODE>
 __global__ void k_mask_cols(St s, const double* f, uint8_t* type, uint16_t* lower, int* front,
                             int fctr, int nzc, int zlen);
+__global__ void k_scan_propose(St s, int* front, int fctr, int nzc, int zlen);
 __global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int* front,
                              int fctr, int epoch, int clear_dirty, int zero_idx);
 __global__ void k_clear_dirty(St s, int zero_idx);

@@ -273,6 +273,14 @@ class Run {
                                                             fctr, cg.nzc, cg.zlen);
     tend();
   }
+  // the same scan with the frontier's proposals made (delta and the touched
+  // list)
+  void scan_propose(int* fbuf, int fctr) {
+    const ColGrid cg = col_grid();
+    tbegin(EXTREMZ_K_CHECK_FULL);
+    k_scan_propose<<<cg.blocks, dim3(64, 4, 1), 0, st()>>>(s_, fbuf, fctr, cg.nzc, cg.zlen);
+    tend();
+  }
   // 64x4x4 label tiles
   int tile_blocks() const {
     return ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4) * ((s_.g.D + 3) / 4);
@@ -533,8 +541,15 @@ void Run::c_loops() {
   front_buf_ = c_->frontA.p;
   back_buf_ = c_->frontB.p;
   epoch_++;
+  // whether the frontier's proposals are made already.  The first scan of a
+  // correction meets the compressor's false critical points, a dense
+  // frontier (5-30% of the grid) whose proposals cost more as scattered
+  // atomics than the scan itself: that scan proposes on the way.  Rescans
+  // after many edits find few false points, and the plain scan is cheaper.
+  bool proposed = false;
   if (!clean_) {
-    check_full(front_buf_, front_);
+    scan_propose(front_buf_, front_);
+    proposed = true;
     k_clear_dirty<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, -1);
   } else {
     tbegin(EXTREMZ_K_CHECK_INCR);
@@ -549,9 +564,12 @@ void Run::c_loops() {
     if (nf == 0) break;
     counts_[EXTREMZ_STAT_FRONTIER_ITEMS] += nf;
     if (++counts_[EXTREMZ_STAT_C_LOOP_ITERS] > max_c_) throw std::range_error("c_loops");
-    tbegin(EXTREMZ_K_PROPOSE);
-    k_propose_c<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_);
-    tend();
+    if (!proposed) {
+      tbegin(EXTREMZ_K_PROPOSE);
+      k_propose_c<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_);
+      tend();
+    }
+    proposed = false;
     // a frontier vertex proposes to itself or at most its 14 neighbours, so
     // at most 15 nf vertices are touched and changed
     const long long nt = 15LL * nf;

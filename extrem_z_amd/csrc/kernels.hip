@@ -127,15 +127,21 @@ __device__ __forceinline__ int classify_at(const Geo& g, const uint8_t* lut,
 
 // Type of a known lower mask (LUT lookup only).
 __device__ __forceinline__ int type_of_mask(const uint8_t* lut, int cls, unsigned lm) {
-  int key = c_cls_key[cls];
+  const unsigned key = c_cls_key[cls];
   unsigned cfg = 0;
   int k = 0, lc = 0;
+  if (key == (1u << kNbr) - 1) {  // interior: the upper config is the complement
+    cfg = ~lm & key;
+    k = kNbr;
+    lc = __popc(lm & key);
+  } else {
 #pragma unroll
-  for (int d = 0; d < kNbr; d++) {
-    if (!((key >> d) & 1)) continue;
-    if ((lm >> d) & 1) lc++;
-    else cfg |= 1u << k;
-    k++;
+    for (int d = 0; d < kNbr; d++) {
+      if (!((key >> d) & 1)) continue;
+      if ((lm >> d) & 1) lc++;
+      else cfg |= 1u << k;
+      k++;
+    }
   }
   if (lc == k) return 4;
   if (lc == 0) return 0;
@@ -211,6 +217,22 @@ struct StagedList {
     if (tid == 0) *base = cnt ? atomicAdd(gcount, cnt) : 0;
     __syncthreads();
     for (int i = tid; i < cnt; i += nt) glist[*base + i] = buf[i];
+  }
+  // Moves the buffer to the global list and empties it when fewer than room
+  // entries are free, so the pushes that follow stay in LDS.  Every thread of
+  // the block calls it right after a barrier, with no push in flight (so all
+  // read the same count).
+  __device__ __forceinline__ void drain(int room) {
+    const int cnt = *n < K ? *n : K;
+    if (cnt + room <= K) return;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nt = blockDim.x * blockDim.y;
+    if (tid == 0) *base = atomicAdd(gcount, cnt);
+    __syncthreads();
+    for (int i = tid; i < cnt; i += nt) glist[*base + i] = buf[i];
+    __syncthreads();
+    if (tid == 0) *n = 0;
+    __syncthreads();
   }
 };
 #define STAGED_LIST(name, T, K, glist, gcount)                         \
@@ -774,6 +796,256 @@ template __global__ void k_mask_cols<0>(St, const double*, uint8_t*, uint16_t*, 
                                         int);
 template __global__ void k_mask_cols<1>(St, const double*, uint8_t*, uint16_t*, int*, int, int,
                                         int);
+
+// The full false-critical-point scan of decp (k_mask_cols<1>) fused with the
+// c_loops proposals of the frontier it finds (k_propose_c over that list).
+// Every proposal's value is a function of its target alone: (orig - xi) -
+// decp for a star neighbour or the vertex itself (get_false_stars,
+// src/ExtreM.cpp:1412-1445), the lower midpoint for the c_loop cases
+// (src/ExtreM.cpp:1152-1236).  So each voxel gathers what its neighbours
+// send to it, and no atomics are needed.
+// A block owns a 64x4 column over zlen planes. Each plane is processed as follows:
+//   - decp, with a halo of two, is held in an LDS ring.
+//   - The outgoing proposals of every voxel of the tile and its halo of one
+//     are stored in a second ring (the word below).
+//   - Each owned voxel ORs what its 14 neighbours and itself send to it and
+//     writes its delta key. A plain store suffices: the voxel is the only
+//     writer, and delta holds the empty key between edit rounds.
+//   - The voxel then joins the touched list, and frontier entries are pushed
+//     as k_mask_cols<1> pushes them.
+// The outgoing word:
+//   bits 0-13  (orig - xi) - decp to neighbour d
+//   bit 14     the same to the voxel itself
+//   bits 15-28 the lower midpoint to neighbour d (the largest decp or the
+//              smallest original neighbour)
+//   bit 29     the lower midpoint to the voxel itself
+//   bit 30     the voxel is a frontier entry
+constexpr int kQX = kLTX + 4, kQY = kLTY + 4, kQN = kQX * kQY;
+constexpr int kQPer = (kQN + 255) / 256;
+
+// Outgoing proposals of one voxel from LDS planes of decp around it (Q0..Q2,
+// centre qc) and its original lower mask / type.
+__device__ __forceinline__ unsigned out_proposals(const St& s, const double* Q0, const double* Q1,
+                                                  const double* Q2, int qc, int x, int y, int z,
+                                                  unsigned ol, int ot) {
+  const Geo& g = s.g;
+  const int v = x + g.W * (y + g.H * z);
+  const int cls = class_of(g, x, y, z);
+  const unsigned key = c_cls_key[cls];
+  const double c = Q1[qc];
+  double fr[kNbr];
+  unsigned dl = 0;
+#pragma unroll
+  for (int d = 0; d < kNbr; d++) {
+    const double* Q = kSZ[d] < 0 ? Q0 : (kSZ[d] > 0 ? Q2 : Q1);
+    fr[d] = Q[qc + kSX[d] + kQX * kSY[d]];
+    const bool lo = fr[d] < c || ((d & 1) && fr[d] == c);
+    dl |= (unsigned)(((key >> d) & 1) && lo) << d;
+  }
+  if (dl == ol) return 0;
+  const int dt = type_of_mask(s.lut, cls, dl);
+  if (!(dt != ot || is_saddle_type(ot))) return 0;
+  unsigned w = 1u << 30;
+  if ((dt == 4) != (ot == 4)) {  // all_max
+    if (ot != 4) {
+      w |= 1u << 29;
+    } else {
+      int largest = v, ld = -1;
+      double fl = c;
+#pragma unroll
+      for (int d = 0; d < kNbr; d++) {
+        if (!((key >> d) & 1)) continue;
+        const int r = v + g.off[d];
+        if (larger(fr[d], r, fl, largest)) {
+          largest = r;
+          fl = fr[d];
+          ld = d;
+        }
+      }
+      if (!(c > fl || (c == fl && v > largest))) w |= ld < 0 ? 1u << 29 : 1u << (15 + ld);
+    }
+  }
+  if ((dt == 0) != (ot == 0)) {  // all_min
+    if (ot != 0) {
+      // the smallest original neighbour: every load issued before the scan
+      double on[kNbr];
+#pragma unroll
+      for (int d = 0; d < kNbr; d++) on[d] = ((key >> d) & 1) ? s.orig[v + g.off[d]] : 0.0;
+      int smallest = v;
+      double fo = s.orig[v], ds = c;
+      unsigned sd = 0;  // 1 + direction of the smallest, 0 for the voxel itself
+#pragma unroll
+      for (int d = 0; d < kNbr; d++) {
+        if (!((key >> d) & 1)) continue;
+        const int r = v + g.off[d];
+        if (smaller(on[d], r, fo, smallest)) {
+          smallest = r;
+          fo = on[d];
+          ds = fr[d];
+          sd = d + 1;
+        }
+      }
+      if (!(c > ds || (c == ds && v > smallest))) w |= sd ? 1u << (14 + sd) : 1u << 29;
+    } else {
+      w |= 1u << 29;
+    }
+  }
+  const bool sad = (dt != ot) ? (is_saddle_type(dt) || is_saddle_type(ot))
+                              : (is_saddle_type(ot) && dl != ol);
+  if (sad) {  // get_false_stars (dl != ol here)
+    w |= ol & ~dl;
+    if (dl & ~ol) w |= 1u << 14;
+  }
+  return w;
+}
+
+// Pipeline (one barrier per step): step k stores decp plane z0 + k, computes
+// the words of plane p = z0 - 1 + k and gathers plane q = z0 - 3 + k, whose
+// three word planes were finished in earlier steps.  decp plane j sits in
+// ring slot (j - z0 + 3) % 5, the words of plane j in sring slot
+// (j - z0 + 1) % 4; each slot is rewritten only a barrier after its last
+// reader.
+__global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr, int nzc,
+                                                      int zlen) {
+  __shared__ double ring[5][kQN];
+  __shared__ unsigned sring[4][kPN];
+  STAGED_LIST(fl, int, 1024, front, &s.ctr[fctr]);
+  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
+  const Geo& g = s.g;
+  const int ntx = (g.W + kLTX - 1) / kLTX;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int col = L / nzc;
+  const int x0 = (col % ntx) * kLTX, y0 = (col / ntx) * kLTY;
+  const int z0 = (L % nzc) * zlen, z1 = min(g.D, z0 + zlen);
+  const int lx = threadIdx.x, ly = threadIdx.y, tid = lx + 64 * ly;
+  const int x = x0 + lx, y = y0 + ly;
+  const bool own_in = x < g.W && y < g.H;
+  // prefetched a step ahead: a decp plane (halo 2), the original lower masks
+  // and types of a word plane (halo 1), orig of the owned voxel of a gather plane
+  double pd[kQPer];
+  unsigned pol[2];
+  int pot[2];
+  double pog = 0.0;
+  auto fetch = [&](int z) {
+#pragma unroll
+    for (int j = 0; j < kQPer; j++) {
+      const int i = tid + 256 * j;
+      double a = 0.0;
+      if (i < kQN) {
+        const int gx = x0 + i % kQX - 2, gy = y0 + i / kQX - 2;
+        if (z >= 0 && z < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
+          a = s.decp[gx + g.W * (gy + g.H * z)];
+      }
+      pd[j] = a;
+    }
+  };
+  auto store = [&](int slot) {
+#pragma unroll
+    for (int j = 0; j < kQPer; j++) {
+      const int i = tid + 256 * j;
+      if (i < kQN) ring[slot][i] = pd[j];
+    }
+  };
+  auto fetch_words = [&](int z) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int i = tid + 256 * j;
+      unsigned a = 0;
+      int b = 0;
+      if (i < kPN) {
+        const int gx = x0 + i % kPX - 1, gy = y0 + i / kPX - 1;
+        if (z >= 0 && z < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H) {
+          const int v = gx + g.W * (gy + g.H * z);
+          a = s.olow[v];
+          b = s.otype[v];
+        }
+      }
+      pol[j] = a;
+      pot[j] = b;
+    }
+  };
+  auto fetch_orig = [&](int z) {
+    pog = (own_in && z >= z0 && z < z1) ? s.orig[x + g.W * (y + g.H * z)] : 0.0;
+  };
+  const int nk = z0 < z1 ? z1 - z0 + 3 : 0;
+  if (nk) {
+    fetch(z0 - 2);
+    store(1);
+    fetch(z0 - 1);
+    store(2);
+    fetch(z0);
+    fetch_words(z0 - 1);
+  }
+  for (int k = 0; k < nk; k++) {
+    const int p = z0 - 1 + k, q = z0 - 3 + k;
+    const bool do_a = p <= z1, do_b = k >= 3;
+    if (do_a) store((k + 3) % 5);
+    const unsigned ol0 = pol[0], ol1 = pol[1];
+    const int ot0 = pot[0], ot1 = pot[1];
+    const double og = pog;
+    __syncthreads();
+    // a step pushes at most one entry per owned voxel to each list
+    fl.drain(256);
+    tl.drain(256);
+    if (p + 1 <= z1) {
+      fetch(p + 2);
+      fetch_words(p + 1);
+    }
+    fetch_orig(q + 1);
+    if (do_a) {
+      const double* Q0 = ring[(k + 1) % 5];
+      const double* Q1 = ring[(k + 2) % 5];
+      const double* Q2 = ring[(k + 3) % 5];
+#pragma unroll 1
+      for (int j = 0; j < 2; j++) {
+        const int i = tid + 256 * j;
+        if (i < kPN) {
+          const int sx = i % kPX, sy = i / kPX, gx = x0 + sx - 1, gy = y0 + sy - 1;
+          unsigned w = 0;
+          if (p >= 0 && p < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
+            w = out_proposals(s, Q0, Q1, Q2, (sx + 1) + kQX * (sy + 1), gx, gy, p,
+                              j ? ol1 : ol0, j ? ot1 : ot0);
+          sring[k % 4][i] = w;
+        }
+      }
+    }
+    if (do_b && own_in) {
+      const unsigned* S0 = sring[(k + 1) % 4];  // plane q - 1
+      const unsigned* S1 = sring[(k + 2) % 4];  // plane q
+      const unsigned* S2 = sring[(k + 3) % 4];  // plane q + 1
+      const int sc = (lx + 1) + kPX * (ly + 1);
+      const unsigned own = S1[sc];
+      // bit 0: (orig - xi) - decp, bit 15: the lower midpoint; a neighbour in
+      // direction d sends to this voxel along direction d ^ 1
+      unsigned kinds = (own >> 14) & 0x8001u;
+#pragma unroll
+      for (int d = 0; d < kNbr; d++) {
+        const unsigned* S = kSZ[d] < 0 ? S0 : (kSZ[d] > 0 ? S2 : S1);
+        kinds |= (S[sc + kSX[d] + kPX * kSY[d]] >> (d ^ 1)) & 0x8001u;
+      }
+      const int t = x + g.W * (y + g.H * q);
+      if (own >> 30) fl.push(t);
+      if (kinds) {
+        const double dv = ring[k % 5][(lx + 2) + kQX * (ly + 2)];
+        long long key = s.sent;
+        if (kinds & 1u) {
+          const long long ka = okey(s.step_q ? s.step : (og - s.xi) - dv);
+          key = ka > key ? ka : key;
+        }
+        if (kinds & 0x8000u) {
+          const long long kb = okey(s.step_q ? s.step : ((og - s.xi) + dv) / 2.0 - dv);
+          key = kb > key ? kb : key;
+        }
+        if (key > s.sent) {
+          s.delta[t] = key;
+          tl.push(t);
+        }
+      }
+    }
+  }
+  fl.flush();
+  tl.flush();
+}
 
 // Follows every pointer that is not yet at a root to its root (D7: every
 // vertex is compressed).  Other lanes' writes only ever replace a pointer by
