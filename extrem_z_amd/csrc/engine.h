@@ -70,6 +70,8 @@ enum Ctr {
                   // (C_LCHG..C_SWEEP are cleared at the end of every CP pass of decp)
   C_IBAIL,        // an incremental pass overran a list (never expected; fails the call)
   C_DSLOT,        // slots to gather again
+  C_PPREV,        // slots the last CP pass of decp pushed to (list pprev)
+  C_PNEXT,        // slots this pass pushes to (list pnext)
   C_CL2,          // changed sides of the last CP pass
   C_CL1,
   C_CLN2,         // changed sides of this CP pass
@@ -140,6 +142,8 @@ struct St {
   int *cl2, *cl1, *cln2, *cln1;   // changed sides of the last / this pass (positions)
   int* slstamp;                   // per slot index
   int* dslot;                     // slots to gather again
+  int* pstamp;                    // per slot index: pushed in the pass of this stamp
+  int *pprev, *pnext;             // slots pushed by the last / this pass
   int4* citems;                   // {k0, k1, maxima?, -}: index chunks whose saddles are queued
   uint8_t* wflag;                 // per side: listed in wl2 / wl1
   int *wl2, *wl1;                 // sides whose triplet ends differed when last computed
@@ -255,7 +259,9 @@ __global__ void k_inc_taint(St s);
 __global__ void k_inc_walk(St s);
 __global__ void k_inc_sweep(St s);
 __global__ void k_inc_prop(St s, int qcap);
-__global__ void k_inc_mark(St s, long long* kmax, long long* kmin);
+__global__ void k_inc_mark(St s, long long* kmax, long long* kmin, int all_slots);
+__global__ void k_inc_merge(St s, long long* kmax, long long* kmin, const long long* pkey,
+                            const int* pid, const int2* pairs, const int* pcnt, int all_slots);
 __global__ void k_inc_mark_csr(St s);
 __global__ void k_inc_saddles(St s, const double* f, int2* pairs, int* pcnt, long long* kmax,
                               long long* kmin);

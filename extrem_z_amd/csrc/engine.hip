@@ -161,7 +161,8 @@ struct extremz_ctx {
   ez::DevBuf<uint8_t> counts_tmp;
   // incremental CP passes of decp
   ez::DevBuf<uint8_t> pdir, opdir, lflag, cflag, tu, td, wflag;
-  ez::DevBuf<int> lchg, spos2, spos1, ilist, qu, qd, sstamp, slists, slstamp, dslot, pcnt;
+  ez::DevBuf<int> lchg, spos2, spos1, ilist, qu, qd, sstamp, slists, slstamp, dslot, pcnt,
+      pstamp, plists;
   ez::DevBuf<uint8_t> vsnap;  // snapshots of the self-check (EXTREMZ_VERIFY_INC)
   int* h_ctr = nullptr;
   std::vector<ez::EventPair> ev_pool;
@@ -178,7 +179,7 @@ struct extremz_ctx {
           &olarg, &osmall, &dlarg, &dsmall, &minsad2, &minsad1, &win_a, &win_b, &lcnt, &csr,
           &csr_off, &csr_cur, &pid, &cpos, &ext_id, &of2, &ol2, &of1, &ol1, &df2, &dl2, &df1,
           &dl1, &wnl2, &wnl1, &bmax2, &bmin2, &ctr, &lchg, &spos2, &spos1, &ilist, &qu, &qd,
-          &sstamp, &slists, &slstamp, &dslot, &pcnt})
+          &sstamp, &slists, &slstamp, &dslot, &pcnt, &pstamp, &plists})
       f(*b);
     for (ez::DevBuf<long long>* b : {&delta, &kmax, &kmin, &pkey, &ckey, &pv2, &pv1}) f(*b);
     for (ez::DevBuf<uint8_t>* b :
@@ -234,6 +235,9 @@ class Run {
   int step_q_ = 0;
   int epoch_ = 0;
   bool clean_ = false;
+  // the last CP pass of decp was a full one: the next incremental pass resets
+  // and merges every slot (it has no list of the full pass's pushes)
+  bool slots_all_ = true;
   int front_ = C_FRONT_A;  // counter index of the live frontier
   int* front_buf_ = nullptr;
   int* back_buf_ = nullptr;
@@ -417,6 +421,7 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
                     int* F2, int* La2, int* F1, int* La1, int* smax, int* smin) {
   long long* kmax = c_->kmax.p;
   long long* kmin = c_->kmin.p;
+  if (dec) slots_all_ = true;
   tbegin(EXTREMZ_K_SLOT_GATHER);
   k_slot_init<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(s_, kmax, smax, kmin,
                                                                           smin);
@@ -647,14 +652,16 @@ void Run::cp_inc(long long nch) {
   // one block
   for (int l = 0; l < kSweeps; l++) k_inc_sweep<<<kSweepBlocks, 256, 0, st()>>>(s_);
   k_inc_prop<<<1, 1024, 0, st()>>>(s_, s_.g.N);
-  k_inc_mark<<<1024, 256, 0, st()>>>(s_, kmax, kmin);
+  k_inc_mark<<<1024, 256, 0, st()>>>(s_, kmax, kmin, slots_all_);
   k_inc_mark_csr<<<512, 256, 0, st()>>>(s_);
   // the queued sides, their slots and ties; sizes stay on the device
   k_inc_saddles<<<kIncSaddleBlocks, 256, 0, st()>>>(s_, s_.decp, c_->pairs.p, c_->pcnt.p, kmax,
                                                      kmin);
   k_inc_pull<<<256, 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
-  k_slot_merge<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(
-      s_, kmax, kmin, s_.dlarg, s_.dsmall, c_->pkey.p, c_->pid.p);
+  k_inc_merge<<<slots_all_ ? blocks_for((long long)s_.nmax + s_.nmin) : kIncSaddleBlocks, 256, 0,
+                st()>>>(s_, kmax, kmin, c_->pkey.p, c_->pid.p, c_->pairs.p, c_->pcnt.p,
+                        slots_all_);
+  slots_all_ = false;
   k_slot_ids<<<kIncSaddleBlocks, 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p, s_.decp, kmax,
                                                  kmin, s_.dlarg, s_.dsmall);
   tend();
@@ -668,6 +675,7 @@ void Run::cp_end() {
   k_inc_end_ctr<<<1, 64, 0, st()>>>(s_.ctr);
   std::swap(s_.cl2, s_.cln2);
   std::swap(s_.cl1, s_.cln1);
+  std::swap(s_.pprev, s_.pnext);
 }
 
 // Self-check of an incremental pass: its state against the full pass's on
@@ -685,7 +693,8 @@ void Run::verify_inc() {
                       {"df1", s_.df1, 4LL * s_.n1},   {"dl1", s_.dl1, 4LL * s_.n1},
                       {"pv2", s_.pv2, 8LL * s_.n2},   {"pv1", s_.pv1, 8LL * s_.n1},
                       {"dlarg", s_.dlarg, 4LL * s_.nmax}, {"dsmall", s_.dsmall, 4LL * s_.nmin},
-                      {"pkey", c_->pkey.p, 8 * ns},   {"pid", c_->pid.p, 4 * ns}};
+                      {"pkey", c_->pkey.p, 8 * ns},   {"pid", c_->pid.p, 4 * ns},
+                      {"kmax", c_->kmax.p, 8LL * s_.nmax}, {"kmin", c_->kmin.p, 8LL * s_.nmin}};
   const int na = (int)(sizeof(arrs) / sizeof(arrs[0]));
   long long total = 0;
   for (const Arr& a : arrs) total += a.bytes;
@@ -1162,6 +1171,11 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
     s_.wflag = c->wflag.p;
     c->slstamp.ensure(ns + 1);
     c->dslot.ensure(ns + 1);
+    c->pstamp.ensure(ns + 1);
+    c->plists.ensure(2 * (ns + 1));
+    s_.pstamp = c->pstamp.p;
+    s_.pprev = c->plists.p;
+    s_.pnext = c->plists.p + ns + 1;
     s_.sstamp = c->sstamp.p;
     s_.slstamp = c->slstamp.p;
     s_.dslot = c->dslot.p;
@@ -1176,6 +1190,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
     s_.wl1 = s_.wl2 + s_.n2;
     ez::hip_check(hipMemsetAsync(s_.sstamp, 0, sizeof(int) * (n21 + 1), q), "memset");
     ez::hip_check(hipMemsetAsync(s_.slstamp, 0, sizeof(int) * (ns + 1), q), "memset");
+    ez::hip_check(hipMemsetAsync(s_.pstamp, 0, sizeof(int) * (ns + 1), q), "memset");
     k_fill_i32<<<bN, 256, 0, q>>>(s_.spos2, -1, N);
     k_fill_i32<<<bN, 256, 0, q>>>(s_.spos1, -1, N);
     k_spos<<<blocks_for(std::max(s_.n2, s_.n1)), 256, 0, q>>>(s_);
@@ -1200,7 +1215,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   slot_pass(0, d_orig, s_.oDS, s_.oAS, s_.olow, s_.of2, s_.ol2, s_.of1, s_.ol1, s_.olarg,
             s_.osmall);
   // main.cpp:74-75, 122-131
-  k_presnap<<<blocks_for(s_.n2 + s_.n1 + 2), 256, 0, q>>>(s_);
+  k_presnap<<<(int)((N + 255) / 256), 256, 0, q>>>(s_);
 
   c_loops();     // main.cpp:139
   cp_dec(true);  // main.cpp:172-191

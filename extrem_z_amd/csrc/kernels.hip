@@ -1601,19 +1601,28 @@ __global__ void k_long_slots(St s) {
 // Final slot keys: the better of the gathered and the pushed key; the slot
 // starts with the gathered saddle when it holds the final key (k_slot_ids
 // then adds the pushed saddles holding it).
-__global__ void k_slot_merge(St s, long long* kmax, long long* kmin, int* slot_max,
-                             int* slot_min, const long long* pkey, const int* pid) {
-  GRID_STRIDE(j, s.nmax) {
+// Slot index j (maxima, then minima): the pushed key merged with the
+// gathered one; the slot is the gathered saddle when its key stands (ties
+// among pushed saddles follow in k_slot_ids).  Idempotent, so a slot may be
+// merged more than once.
+__device__ __forceinline__ void merge_slot(const St& s, int j, long long* kmax, long long* kmin,
+                                           int* slot_max, int* slot_min, const long long* pkey,
+                                           const int* pid) {
+  if (j < s.nmax) {
     const long long k = kmax[j] > pkey[j] ? kmax[j] : pkey[j];
     kmax[j] = k;
     slot_max[j] = (pid[j] != kNone && pkey[j] == k) ? pid[j] : kNone;
+  } else {
+    const int i = j - s.nmax;
+    const long long k = kmin[i] < pkey[j] ? kmin[i] : pkey[j];
+    kmin[i] = k;
+    slot_min[i] = (pid[j] != kNone && pkey[j] == k) ? pid[j] : 0x7fffffff;
   }
-  GRID_STRIDE(j, s.nmin) {
-    const int q = s.nmax + j;
-    const long long k = kmin[j] < pkey[q] ? kmin[j] : pkey[q];
-    kmin[j] = k;
-    slot_min[j] = (pid[q] != kNone && pkey[q] == k) ? pid[q] : 0x7fffffff;
-  }
+}
+
+__global__ void k_slot_merge(St s, long long* kmax, long long* kmin, int* slot_max,
+                             int* slot_min, const long long* pkey, const int* pid) {
+  GRID_STRIDE(j, s.nmax + s.nmin) merge_slot(s, j, kmax, kmin, slot_max, slot_min, pkey, pid);
 }
 
 // In-place exclusive scan of a[0..n) by one block of 1024 threads; a[n]
@@ -2049,7 +2058,8 @@ __device__ __forceinline__ void mark_side(const St& s, int q2, bool two, L& l2, 
 // chunks of k_inc_mark_csr, empty slot keys and slots for this pass's pushes
 // (as k_slot_init; no kernel before the pushes reads them), and the decp
 // global extrema when k_inc_taint cleared them.
-__global__ void __launch_bounds__(256) k_inc_mark(St s, long long* kmax, long long* kmin) {
+__global__ void __launch_bounds__(256) k_inc_mark(St s, long long* kmax, long long* kmin,
+                                                  int all_slots) {
   if (s.ctr[C_IBAIL]) return;
   // the global maximum / minimum of decp over the lists, when the current one
   // changed value (block bests into the cleared counters; the block barriers
@@ -2149,13 +2159,24 @@ __global__ void __launch_bounds__(256) k_inc_mark(St s, long long* kmax, long lo
     for (int c = 0; c < nck; c++)
       s.citems[b + c] = make_int4(k0 + (c << 10), min(k1, k0 + ((c + 1) << 10)), mx, 0);
   }
-  GRID_STRIDE(i, s.nmax) {
-    kmax[i] = (long long)0x8000000000000000ULL;
-    s.dlarg[i] = kNone;
-  }
-  GRID_STRIDE(i, s.nmin) {
-    kmin[i] = 0x7fffffffffffffffLL;
-    s.dsmall[i] = 0x7fffffff;
+  if (all_slots) {
+    GRID_STRIDE(i, s.nmax) {
+      kmax[i] = (long long)0x8000000000000000ULL;
+      s.dlarg[i] = kNone;
+    }
+    GRID_STRIDE(i, s.nmin) {
+      kmin[i] = 0x7fffffffffffffffLL;
+      s.dsmall[i] = 0x7fffffff;
+    }
+  } else {
+    // every other slot key is the merged one of the last pass with no push
+    // in it, i.e. the gathered key, which a push of this pass may raise
+    const int np = s.ctr[C_PPREV];
+    GRID_STRIDE(i, np) {
+      const int j = s.pprev[i];
+      if (j < s.nmax) kmax[j] = (long long)0x8000000000000000ULL;
+      else kmin[j - s.nmax] = 0x7fffffffffffffffLL;
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     s.ctr[C_ACT0] = 0;
@@ -2163,6 +2184,40 @@ __global__ void __launch_bounds__(256) k_inc_mark(St s, long long* kmax, long lo
     s.ctr[C_CLN1] = 0;
     s.ctr[C_DSLOT] = 0;
   }
+}
+
+// Slot merge of an incremental pass: every slot (all_slots, the first pass
+// after a full one), else the slots pushed by the last pass (reset by
+// k_inc_mark) and the gathered-again ones; always the slots of this pass's
+// pair records, which are listed (deduplicated) for the next pass.  Other
+// slots keep last pass's keys and slots: nothing they merge changed.
+__global__ void __launch_bounds__(256) k_inc_merge(St s, long long* kmax, long long* kmin,
+                                                   const long long* pkey, const int* pid,
+                                                   const int2* pairs, const int* pcnt,
+                                                   int all_slots) {
+  STAGED_LIST(pl, int, 1024, s.pnext, &s.ctr[C_PNEXT]);
+  if (all_slots) {
+    GRID_STRIDE(j, s.nmax + s.nmin)
+        merge_slot(s, j, kmax, kmin, s.dlarg, s.dsmall, pkey, pid);
+  } else {
+    const int np = s.ctr[C_PPREV], nd = s.ctr[C_DSLOT];
+    GRID_STRIDE(i, np) merge_slot(s, s.pprev[i], kmax, kmin, s.dlarg, s.dsmall, pkey, pid);
+    GRID_STRIDE(i, nd) merge_slot(s, s.dslot[i], kmax, kmin, s.dlarg, s.dsmall, pkey, pid);
+  }
+  const int nc = s.ctr[C_PCHUNK];
+  const int lane = threadIdx.x & 63;
+  const int nw = (gridDim.x * blockDim.x) >> 6;
+  for (int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < nc; c += nw) {
+    const int n = pcnt[c];
+    const int2* P = pairs + (long long)c * kChunkPairs;
+    for (int k = lane; k < n; k += 64) {
+      const int2 pr = P[k];
+      const int j = (pr.x & 1) ? s.nmax + (pr.x >> 1) : (pr.x >> 1);
+      merge_slot(s, j, kmax, kmin, s.dlarg, s.dsmall, pkey, pid);
+      if (atomicExch(&s.pstamp[j], s.iepoch) != s.iepoch) pl.push(j);
+    }
+  }
+  pl.flush();
 }
 
 // The saddles adjacent in the original to a changed vertex that is a listed
@@ -2270,12 +2325,14 @@ __global__ void k_ctr_copy2(int* ctr, int a, int from_a, int b, int from_b) {
 }
 
 // End of a CP pass of decp: the pass counters start at zero, the changed
-// sides become the last pass's, decp counts as unchanged.
+// sides and pushed slots become the last pass's, decp counts as unchanged.
 __global__ void k_inc_end_ctr(int* ctr) {
   for (int i = C_LCHG + (int)threadIdx.x; i <= C_SWEEP; i += blockDim.x) ctr[i] = 0;
   if (threadIdx.x == 0) {
     ctr[C_CL2] = ctr[C_CLN2];
     ctr[C_CL1] = ctr[C_CLN1];
+    ctr[C_PPREV] = ctr[C_PNEXT];
+    ctr[C_PNEXT] = 0;
     ctr[C_ANY_CHANGE] = 0;
   }
 }
@@ -2714,15 +2771,13 @@ __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int n
 // main.cpp:74-75 and 122-131: decp := orig - xi at the global min / max and
 // at every original 1- and 2-saddle.
 __global__ void k_presnap(St s) {
-  int n = s.n2 + s.n1 + 2;
-  GRID_STRIDE(e, n) {
-    int v;
-    if (e == 0) v = s.ctr[C_GMIN];
-    else if (e == 1) v = s.ctr[C_GMAX];
-    else if (e - 2 < s.n2) v = s.S2[e - 2];
-    else v = s.S1[e - 2 - s.n2];
+  // one lane per voxel (coalesced; the saddles are a quarter of a rough
+  // field); the listed saddles are the vertices of types 1-3
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= s.g.N) return;
+  const int t = s.otype[v];
+  if ((t >= 1 && t <= 3) || v == s.ctr[C_GMIN] || v == s.ctr[C_GMAX])
     s.decp[v] = s.orig[v] - s.xi;
-  }
 }
 
 __global__ void k_argext(const double* f, int n, int* slot_min, int* slot_max) {
