@@ -271,7 +271,7 @@ __global__ void k_ctr_copy2(int* ctr, int a, int from_a, int b, int from_b);
 __global__ void k_inc_end_ctr(int* ctr);
 __global__ void k_diff_count(const void* a, const void* b, long long nbytes, int* out);
 __global__ void k_act_copy(St s, int2* dst, int dctr);
-__global__ void k_wn_register(St s, int2* active, int actr);
+__global__ void k_wn_register(St s, int2* active, int actr, int nctr);
 __global__ void k_wn_commit(St s, int2* active, int actr);
 __global__ void k_wn_next(St s, const int2* active, int actr, int2* next, int nctr);
 __global__ void k_wn_reset(St s);

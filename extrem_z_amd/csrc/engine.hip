@@ -298,12 +298,15 @@ class Run {
   // Whole-grid levels of the root propagation before the one-block remainder.
   static constexpr int kSweeps = 3, kSweepBlocks = 256;
   // Blocks of a saddle launch (each takes a contiguous range of 256-side
-  // chunks).
-  static constexpr int kSaddleBlocks = 65536;
+  // chunks, consecutive planes of one 32x32 column, so a block's plane
+  // neighbours stay cached): at least this many, else about 4 chunks each.
+  static constexpr int kSaddleBlocks = 4096;
   // Blocks of a column sweep (one frontier append each).
   static constexpr int kColBlocks = 4096;
   // Blocks of the incremental saddle launches (sizes known on the device only).
   static constexpr int kIncSaddleBlocks = 1024;
+  // Wrong-neighbour reservation rounds launched per read of the list length.
+  static constexpr int kWnRounds = 4;
   static int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     return (int)std::max<long long>(1, std::min<long long>(b, 2048));
@@ -435,7 +438,8 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
     }
     tbegin(EXTREMZ_K_SADDLE_PASS);
     auto* kern = mode == kModeOrig ? k_saddle_list<kModeOrig> : k_saddle_list<kModeDec>;
-    kern<<<std::min(nb2 + nb1, kSaddleBlocks), 256, 0, st()>>>(
+    const int nb = nb2 + nb1;
+    kern<<<std::min(nb, std::max(kSaddleBlocks, nb / 4)), 256, 0, st()>>>(
         s_, f, L2, L1, flow, F2, La2, F1, La1, c_->pairs.p, c_->pcnt.p, kmax, kmin, nb2,
         nb2 + nb1);
     tend();
@@ -757,21 +761,25 @@ void Run::wrong_neighbors() {
   int actr = C_ACT_A, nctr = C_ACT_B;
   k_act_copy<<<blocks_for(2LL * s_.g.N), 256, 0, st()>>>(s_, act, actr);
   long long guard = (long long)s_.n2 + s_.n1 + 2;
-  for (long long round = 0;; round++) {
+  // the list length is read every kWnRounds rounds; it never grows, so the
+  // grid of the last read covers the rounds until the next one (rounds on an
+  // empty list do nothing)
+  for (long long round = 0;;) {
     pull();
     int na = h(actr);
     if (na == 0) break;
     // each round commits at least the lowest active saddle
     if (round > guard) throw HipError("wrong-neighbour reservations did not finish");
-    set_ctr(nctr, 0);
     int b = blocks_for(na);
     tbegin(EXTREMZ_K_RESERVE);
-    k_wn_register<<<b, 256, 0, st()>>>(s_, act, actr);
-    k_wn_commit<<<b, 256, 0, st()>>>(s_, act, actr);
-    k_wn_next<<<b, 256, 0, st()>>>(s_, act, actr, nxt, nctr);
+    for (int r = 0; r < kWnRounds; r++, round++) {
+      k_wn_register<<<b, 256, 0, st()>>>(s_, act, actr, nctr);
+      k_wn_commit<<<b, 256, 0, st()>>>(s_, act, actr);
+      k_wn_next<<<b, 256, 0, st()>>>(s_, act, actr, nxt, nctr);
+      std::swap(act, nxt);
+      std::swap(actr, nctr);
+    }
     tend();
-    std::swap(act, nxt);
-    std::swap(actr, nctr);
   }
 }
 

@@ -1406,8 +1406,9 @@ __global__ void __launch_bounds__(256, 4) k_saddle_list(St s, const double* f, c
   const int tid = threadIdx.x;
   if (tid < kNbr) s_off[tid] = s.g.off[tid];
   if (blockIdx.x == 0 && tid == 0) s.ctr[C_PCHUNK] = nb;
-  const int per = (nb + gridDim.x - 1) / gridDim.x;
-  const int c0 = xcd_remap(blockIdx.x, gridDim.x) * per, c1 = min(nb, c0 + per);
+  // balanced ranges: every block (so every XCD) gets its share
+  const long long L = xcd_remap(blockIdx.x, gridDim.x);
+  const int c0 = (int)(L * nb / gridDim.x), c1 = (int)((L + 1) * nb / gridDim.x);
   for (int c = c0; c < c1; c++) {
     if (tid == 0) s_np = 0;
     __syncthreads();
@@ -2259,8 +2260,8 @@ __global__ void __launch_bounds__(256, 4) k_inc_saddles(St s, const double* f, i
   const int n2 = s.ctr[C_DSL2], n1 = s.ctr[C_DSL1];
   const int nb2 = (n2 + 255) >> 8, nb = nb2 + ((n1 + 255) >> 8);
   if (blockIdx.x == 0 && tid == 0) s.ctr[C_PCHUNK] = nb;
-  const int per = (nb + gridDim.x - 1) / gridDim.x;
-  const int c0 = blockIdx.x * per, c1 = min(nb, c0 + per);
+  const int c0 = (int)((long long)blockIdx.x * nb / gridDim.x),
+            c1 = (int)((long long)(blockIdx.x + 1) * nb / gridDim.x);
   for (int c = c0; c < c1; c++) {
     if (tid == 0) s_np = 0;
     __syncthreads();
@@ -2361,7 +2362,8 @@ __global__ void k_act_copy(St s, int2* dst, int dctr) {
 
 // Drop flagged candidates; every remaining candidate records the smallest
 // saddle id that may still take it.
-__global__ void __launch_bounds__(256) k_wn_register(St s, int2* active, int actr) {
+__global__ void __launch_bounds__(256) k_wn_register(St s, int2* active, int actr, int nctr) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr[nctr] = 0;  // the list k_wn_next fills
   int n = s.ctr[actr];
   GRID_STRIDE(a, n) {
     int2 e = active[a];
