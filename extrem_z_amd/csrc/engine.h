@@ -70,6 +70,7 @@ enum Ctr {
                   // (C_LCHG..C_SWEEP are cleared at the end of every CP pass of decp)
   C_IBAIL,        // an incremental pass overran a list (never expected; fails the call)
   C_DSLOT,        // slots to gather again
+  C_SLC,          // saddle_loops calls with edits (run behind a device-side gate)
   C_PPREV,        // slots the last CP pass of decp pushed to (list pprev)
   C_PNEXT,        // slots this pass pushes to (list pnext)
   C_CL2,          // changed sides of the last CP pass
@@ -278,7 +279,8 @@ __global__ void k_wn_reset(St s);
 __global__ void k_wn_solve(St s, int2* bufA, int2* bufB, int limit);
 __global__ void k_fixpath(St s, int fast);
 __global__ void k_wrong_index(St s, int side, int phase, int do_fix);
-__global__ void k_wrong_saddle(St s, int side, int do_fix);
+__global__ void k_wrong_saddle(St s, int side, int do_fix, int gate);
+__global__ void k_saddle_gate(int* ctr, int gate, int wsc);
 __global__ void k_lists_count(St s, int* cnt, int nb, int tiled);
 __global__ void k_lists_totals(St s, const int* cnt, int nb);
 __global__ void k_lists_write(St s, const int* cnt, int nb, int tiled);

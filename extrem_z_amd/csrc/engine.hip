@@ -364,6 +364,7 @@ class Run {
   void r_loops();
   void s_loops(int side, int do_fix);
   void saddle_loops(int side, int do_fix);
+  void saddle_loops_if(int side);
 };
 
 // The CP lists S2 / S1 / MX / MN of the types in s_.otype, in the column
@@ -822,10 +823,24 @@ void Run::saddle_loops(int side, int do_fix) {
   if (do_fix) counts_[EXTREMZ_STAT_SADDLE_LOOP_CALLS]++;
   if (n) {
     tbegin(EXTREMZ_K_PROPOSE);
-    k_wrong_saddle<<<blocks_for(n), 256, 0, st()>>>(s_, side, do_fix);
+    k_wrong_saddle<<<blocks_for(n), 256, 0, st()>>>(s_, side, do_fix, -1);
     tend();
   }
   if (do_fix) apply();
+}
+
+// saddle_loops(side, 1) when the side's wrong-index count is zero, decided on
+// the device (no counter read); the calls made are counted there (C_SLC).
+void Run::saddle_loops_if(int side) {
+  const int n = side == 2 ? s_.nmax : s_.nmin;
+  const int gate = side == 2 ? C_WMC : C_WMINC;
+  k_saddle_gate<<<1, 64, 0, st()>>>(s_.ctr, gate, side == 2 ? C_WSC : C_WSCJ);
+  if (n) {
+    tbegin(EXTREMZ_K_PROPOSE);
+    k_wrong_saddle<<<blocks_for(n), 256, 0, st()>>>(s_, side, 1, gate);
+    tend();
+  }
+  apply();
 }
 
 void Run::classify(const double* d_f, int W, int H, int D, uint8_t* d_t, uint16_t* d_l) {
@@ -1233,14 +1248,12 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   if (nfc == 0) {  // main.cpp:230-254
     cp_dec(false, true);
     s_loops(2, 1);
-    pull();
-    if (h(C_WMC) == 0) saddle_loops(2, 1);
+    saddle_loops_if(2);
   }
   if (nfc1 == 0) {  // main.cpp:255-274
     cp_dec(false, nfc != 0);
     s_loops(1, 1);
-    pull();
-    if (h(C_WMINC) == 0) saddle_loops(1, 1);
+    saddle_loops_if(1);
   }
   auto any = [&]() {
     pull();
@@ -1259,13 +1272,11 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
       if (rep < 2) {
         if (nfc == 0) {
           s_loops(2, 1);
-          pull();
-          if (h(C_WMC) == 0) saddle_loops(2, 1);
+          saddle_loops_if(2);
         }
         if (nfc1 == 0) {
           s_loops(1, 1);
-          pull();
-          if (h(C_WMINC) == 0) saddle_loops(1, 1);
+          saddle_loops_if(1);
         }
       } else {
         if (nfc == 0) {
@@ -1283,6 +1294,7 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   k_count_edited<<<bN, 256, 0, q>>>(s_.cnt, N, s_.ctr);
   pull();
   counts_[EXTREMZ_STAT_EDITED] = h(C_EDITED);
+  counts_[EXTREMZ_STAT_SADDLE_LOOP_CALLS] += h(C_SLC);
   return 0;
 }
 

@@ -2659,7 +2659,18 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
 // (its triplet entries are skipped, src/ExtreM.cpp:2330); a side without an
 // adjacent saddle (the reference reads one past its saddle vector there) is
 // no mismatch.
-__global__ void __launch_bounds__(256) k_wrong_saddle(St s, int side, int do_fix) {
+// saddle_loops' condition (main.cpp:250-251, 270-271, 328-336, 378-387): the side's
+// wrong-index count is zero.  Then its wrong-saddle count starts at zero and
+// the call is counted; else k_wrong_saddle with this gate does nothing.
+__global__ void k_saddle_gate(int* ctr, int gate, int wsc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && ctr[gate] == 0) {
+    ctr[wsc] = 0;
+    ctr[C_SLC]++;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_wrong_saddle(St s, int side, int do_fix, int gate) {
+  if (gate >= 0 && s.ctr[gate] != 0) return;  // the whole grid reads the same count
   int n = side == 2 ? s.nmax : s.nmin;
   const int* E = side == 2 ? s.MX : s.MN;
   const int* os = side == 2 ? s.olarg : s.osmall;
