@@ -2428,6 +2428,142 @@ __global__ void __launch_bounds__(256) k_wn_next(St s, const int2* active, int a
   nl.flush();
 }
 
+// The rounds of k_wn_solve with every reservation, flag and active entry in
+// LDS, for candidate lists of at most kWnLds entries whose distinct
+// candidate vertices fill at most half of a kWnHash-slot table (open
+// addressing on (vertex, side)).  Returns false, having written nothing to
+// global memory, when the list does not fit; the caller then runs the global
+// rounds.  The committed wrong neighbours go to wnl2 / wnl1 (the vertex flags
+// wn2 / wn1 are not needed: k_wn_solve and k_wn_reset clear them through
+// those lists).  Called by the whole block of 1024 threads.
+constexpr int kWnLds = 4096, kWnHash = 8192;
+constexpr unsigned kWnEmpty = 0xffffffffu;
+
+__device__ __forceinline__ unsigned wn_slot0(unsigned key) {
+  return (key * 2654435761u) >> (32 - 13);  // kWnHash = 2^13
+}
+// The slot of a key known to be in the table.
+__device__ __forceinline__ int wn_find(const unsigned* keys, unsigned key) {
+  unsigned h = wn_slot0(key);
+  while (keys[h] != key) h = (h + 1) & (kWnHash - 1);
+  return (int)h;
+}
+
+__device__ bool wn_solve_lds(const St& s) {
+  __shared__ unsigned hkey[kWnHash];
+  __shared__ int hmin[kWnHash];
+  __shared__ uint8_t hflag[kWnHash];
+  __shared__ int2 la[kWnLds], lb[kWnLds];
+  __shared__ int n_cur, n_next, n_fc, n_fc1, nkeys, ok;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    n_cur = s.ctr[C_ACT0];
+    n_next = 0;
+    n_fc = 0;
+    n_fc1 = 0;
+    nkeys = 0;
+    ok = n_cur <= kWnLds;
+  }
+  __syncthreads();
+  if (!ok) return false;
+  const int n0 = n_cur;
+  for (int i = t; i < kWnHash; i += 1024) {
+    hkey[i] = kWnEmpty;
+    hmin[i] = 0x7fffffff;
+    hflag[i] = 0;
+  }
+  for (int i = t; i < n0; i += 1024) la[i] = s.act0[i];
+  __syncthreads();
+  // every candidate (vertex, side) once in the table
+  for (int a = t; a < n0; a += 1024) {
+    const int2 e = la[a];
+    const int sv = e.x >> 1, side = e.x & 1;
+    unsigned mask = (unsigned)e.y;
+    while (mask) {
+      const int d = __ffs(mask) - 1;
+      mask &= mask - 1;
+      const unsigned key = ((unsigned)(sv + s.g.off[d]) << 1) | (unsigned)side;
+      unsigned h = wn_slot0(key);
+      for (int p = 0;; p++) {
+        const unsigned old = atomicCAS(&hkey[h], kWnEmpty, key);
+        if (old == kWnEmpty) {
+          if (atomicAdd(&nkeys, 1) >= kWnHash / 2) ok = 0;
+          break;
+        }
+        if (old == key) break;
+        if (p == kWnHash / 2) {  // (cannot happen while the table is at most half full)
+          ok = 0;
+          break;
+        }
+        h = (h + 1) & (kWnHash - 1);
+      }
+    }
+  }
+  __syncthreads();
+  if (!ok) return false;
+  int2* act = la;
+  int2* nxt = lb;
+  for (int round = 0; n_cur > 0 && round <= n0; round++) {
+    const int n = n_cur;
+    for (int a = t; a < n; a += 1024) {  // register
+      const int2 e = act[a];
+      const int sv = e.x >> 1, side = e.x & 1;
+      unsigned mask = (unsigned)e.y, m = mask;
+      while (m) {
+        const int d = __ffs(m) - 1;
+        m &= m - 1;
+        const int h = wn_find(hkey, ((unsigned)(sv + s.g.off[d]) << 1) | (unsigned)side);
+        if (hflag[h]) mask &= ~(1u << d);
+        else atomicMin(&hmin[h], sv);
+      }
+      act[a].y = (int)mask;
+    }
+    __syncthreads();
+    for (int a = t; a < n; a += 1024) {  // commit
+      const int2 e = act[a];
+      const unsigned mask = (unsigned)e.y;
+      if (!mask) continue;
+      const int sv = e.x >> 1, side = e.x & 1;
+      const int r = sv + s.g.off[__ffs(mask) - 1];
+      const int h = wn_find(hkey, ((unsigned)r << 1) | (unsigned)side);
+      if (hmin[h] == sv) {
+        hflag[h] = 1;
+        if (!side) s.wnl2[atomicAdd(&n_fc, 1)] = r;
+        else s.wnl1[atomicAdd(&n_fc1, 1)] = r;
+        act[a].y = (int)(mask | 0x8000u);  // done
+      }
+    }
+    __syncthreads();
+    for (int a = t; a < n; a += 1024) {  // release and compaction
+      const int2 e = act[a];
+      const unsigned mask = (unsigned)e.y;
+      const int sv = e.x >> 1, side = e.x & 1;
+      unsigned m = mask & ((1u << kNbr) - 1);
+      while (m) {
+        const int d = __ffs(m) - 1;
+        m &= m - 1;
+        hmin[wn_find(hkey, ((unsigned)(sv + s.g.off[d]) << 1) | (unsigned)side)] = 0x7fffffff;
+      }
+      if (mask && !(mask & 0x8000u)) nxt[atomicAdd(&n_next, 1)] = e;
+    }
+    __syncthreads();
+    if (t == 0) {
+      n_cur = n_next;
+      n_next = 0;
+    }
+    int2* tmp = act;
+    act = nxt;
+    nxt = tmp;
+    __syncthreads();
+  }
+  if (t == 0) {
+    s.ctr[C_NFC] = n_fc;
+    s.ctr[C_NFC1] = n_fc1;
+    if (n_cur > 0) s.ctr[C_WN_BAIL] = 2;  // every round commits one saddle: unreachable
+  }
+  return true;
+}
+
 // All rounds of the reservations above in one block of 1024 threads, for
 // candidate lists of at most `limit` entries (else C_WN_BAIL is set and
 // nothing is done): the flag reset of k_wn_reset, the copy of k_act_copy and
@@ -2456,6 +2592,7 @@ __global__ void __launch_bounds__(1024) k_wn_solve(St s, int2* bufA, int2* bufB,
     for (int i = t; i < n; i += 1024) wn2[s.wnl2[i]] = 0;
     for (int i = t; i < n1; i += 1024) wn1[s.wnl1[i]] = 0;
   }
+  if (wn_solve_lds(s)) return;
   for (int i = t; i < n_cur; i += 1024) bufA[i] = s.act0[i];
   __syncthreads();
   int2* act = bufA;
