@@ -662,7 +662,7 @@ void Run::cp_inc(long long nch) {
   // the queued sides, their slots and ties; sizes stay on the device
   k_inc_saddles<<<kIncSaddleBlocks, 256, 0, st()>>>(s_, s_.decp, c_->pairs.p, c_->pcnt.p, kmax,
                                                      kmin);
-  k_inc_pull<<<256, 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
+  k_inc_pull<<<1024, 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
   k_inc_merge<<<slots_all_ ? blocks_for((long long)s_.nmax + s_.nmin) : kIncSaddleBlocks, 256, 0,
                 st()>>>(s_, kmax, kmin, c_->pkey.p, c_->pid.p, c_->pairs.p, c_->pcnt.p,
                         slots_all_);

@@ -2287,19 +2287,47 @@ __global__ void __launch_bounds__(256, 4) k_inc_saddles(St s, const double* f, i
   sw1.flush();
 }
 
-// The queued slots' gathers (one wavefront per slot).
+// The queued slots' gathers: kPullGroup lanes per slot, a wavefront for a
+// slot of more than kPullShort adjacencies (taken after its block's short
+// ones; the loop's trip count is the same for every thread of a block).
 __global__ void __launch_bounds__(256) k_inc_pull(St s, long long* pkey, int* pid) {
+  constexpr int kPer = 256 / kPullGroup;  // slots per block and step
+  __shared__ int lng[kPer];
+  __shared__ int nl;
+  if (threadIdx.x == 0) nl = 0;
+  __syncthreads();
   const int n = s.ctr[C_DSLOT];
-  const int lane = threadIdx.x & 63;
-  const int nw = (gridDim.x * blockDim.x) >> 6;
-  for (int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n; w += nw) {
-    const int j = s.dslot[w];
-    const bool mx = j < s.nmax;
+  const int gl = threadIdx.x & (kPullGroup - 1), lane = threadIdx.x & 63;
+  for (int w0 = blockIdx.x * kPer; w0 < n; w0 += gridDim.x * kPer) {
+    const int w = w0 + threadIdx.x / kPullGroup;
+    int j = 0, k0 = 0, k1 = 0;
+    if (w < n) {
+      j = s.dslot[w];
+      k0 = s.csr_off[j];
+      k1 = s.csr_off[j + 1];
+    }
+    const bool mine = w < n && k1 - k0 <= kPullShort;
+    if (w < n && !mine && gl == 0) lng[atomicAdd(&nl, 1)] = j;
+    bool mx = j < s.nmax;
     long long bk = 0;
     int bp = -1;
-    pull_range(s, mx, s.csr_off[j] + lane, s.csr_off[j + 1], 64, bk, bp);
-    pull_group(mx, 64, bk, bp);
-    if (lane == 0) pull_store(s, j, mx, bk, bp, pkey, pid);
+    if (mine) pull_range(s, mx, k0 + gl, k1, kPullGroup, bk, bp);
+    pull_group(mx, kPullGroup, bk, bp);
+    if (mine && gl == 0) pull_store(s, j, mx, bk, bp, pkey, pid);
+    __syncthreads();
+    const int nlong = nl;
+    for (int q = threadIdx.x >> 6; q < nlong; q += 4) {  // a wavefront per long slot
+      const int jj = lng[q];
+      mx = jj < s.nmax;
+      bk = 0;
+      bp = -1;
+      pull_range(s, mx, s.csr_off[jj] + lane, s.csr_off[jj + 1], 64, bk, bp);
+      pull_group(mx, 64, bk, bp);
+      if (lane == 0) pull_store(s, jj, mx, bk, bp, pkey, pid);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) nl = 0;
+    __syncthreads();
   }
 }
 
