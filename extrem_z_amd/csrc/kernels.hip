@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
 // so every (value, id) comparison of src/ExtreM.cpp:153 / 511 is one compare:
 // scanning in id order, ">=" keeps the highest (value, id), "<" the lowest.
 constexpr int kLTX = 64, kLTY = 4, kLTZ = 4, kLTN = kLTX * kLTY * kLTZ;
-constexpr int kFinal = (int)0x80000000u;  // label bit: a root found inside the tile
+constexpr int kExit = (int)0x80000000u;  // label bit: a pointer out of the tile, not a root
 constexpr int kPX = kLTX + 2, kPY = kLTY + 2, kPN = kPX * kPY;
 constexpr int kPPer = (kPN + 255) / 256;
 
@@ -698,12 +698,12 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
     int x = x0 + li % kLTX, y = y0 + (li / kLTX) % kLTY, z = z0 + li / (kLTX * kLTY);
     if (x >= g.W || y >= g.H || z >= g.D) continue;
     int v = x + g.W * (y + g.H * z);
-    // a root inside the tile is final: written with the top bit set, which
-    // k_chase clears without a gather
+    // a root inside the tile is final as written; a pointer out of the tile
+    // carries the top bit, and only those are left to k_chase
     int p = sup[li];
-    ds[v] = p >= 0 ? kFinal | ((x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY)))) : ~p;
+    ds[v] = p >= 0 ? (x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY))) : kExit | ~p;
     p = sdn[li];
-    as[v] = p >= 0 ? kFinal | ((x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY)))) : ~p;
+    as[v] = p >= 0 ? (x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY))) : kExit | ~p;
   }
 }
 
@@ -1047,28 +1047,23 @@ __global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr
   tl.flush();
 }
 
-// Follows every pointer that is not yet at a root to its root (D7: every
-// vertex is compressed).  Other lanes' writes only ever replace a pointer by
-// one further along the same path, so any value read is an ancestor; the
-// walk is bounded by N.
-// A value with the top bit set (kFinal, from k_labels_tile) is a root:
-// its own lane clears the bit, and a walker that reads it stops there.
+// Follows every pointer out of a tile to its root (D7: every vertex is
+// compressed).  A label without the top bit (kExit) is a root: found inside
+// the tile by k_labels_tile or written here, so voxels resolved in their tile
+// are only read.  A label with it names the next vertex on the steepest path;
+// other lanes' writes only ever replace it by that path's root.  The walk is
+// bounded by N.
 __device__ __forceinline__ void chase_one(int* S, int v, int n) {
   int r = S[v];
-  if (r < 0) {  // resolved inside its tile
-    S[v] = r & ~kFinal;
-    return;
-  }
-  int rr = __hip_atomic_load(&S[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (rr == r) return;
-  for (int step = 0; step < n && rr != r; step++) {
-    if (rr < 0) {
-      rr &= ~kFinal;
+  if (r >= 0) return;  // resolved inside its tile
+  r &= ~kExit;
+  for (int step = 0; step < n; step++) {
+    const int rr = __hip_atomic_load(&S[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (rr >= 0) {
       r = rr;
       break;
     }
-    r = rr;
-    rr = __hip_atomic_load(&S[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    r = rr & ~kExit;
   }
   S[v] = r;
 }
