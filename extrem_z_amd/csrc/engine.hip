@@ -307,6 +307,8 @@ class Run {
   static constexpr int kIncSaddleBlocks = 1024;
   // Wrong-neighbour reservation rounds launched per read of the list length.
   static constexpr int kWnRounds = 4;
+  // Listed sides up to which the wrong-index phases run in one block.
+  static constexpr int kWrongOne = 4096;
   static int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     return (int)std::max<long long>(1, std::min<long long>(b, 2048));
@@ -362,7 +364,7 @@ class Run {
   void verify_inc();
   void wrong_neighbors();
   void r_loops();
-  void s_loops(int side, int do_fix);
+  void s_loops(int side, int do_fix, int known = -1);
   void saddle_loops(int side, int do_fix);
   void saddle_loops_if(int side);
 };
@@ -798,12 +800,18 @@ void Run::r_loops() {
 
 // s_loops / s_loops_join (src/ExtreM.cpp:2838-2886) with do_fix, or the bare
 // get_wrong_index_max / _min of main.cpp:434,441.
-void Run::s_loops(int side, int do_fix) {
+// known: the side's listed-side count when the last counter read holds it
+// (no CP pass since), else -1.
+void Run::s_loops(int side, int do_fix, int known) {
   int n = side == 2 ? s_.n2 : s_.n1;
   if (do_fix) counts_[EXTREMZ_STAT_S_LOOP_CALLS]++;
   if (!n) {  // (else the first wrong-index launch clears them)
     if (side == 2) set_ctr2(C_WMC, C_WMC2, 0);
     else set_ctr2(C_WMINC, C_WMINC2, 0);
+  } else if (known >= 0 && known <= kWrongOne) {  // few listed sides: one block
+    tbegin(EXTREMZ_K_PROPOSE);
+    k_wrong_index_one<<<1, 1024, 0, st()>>>(s_, side, do_fix);
+    tend();
   } else {
     const int b = std::min(blocks_for(n), 512);  // over the listed sides only
     k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 0, 0);
@@ -1269,22 +1277,24 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
       wrong_neighbors();
       nfc = h(C_NFC);
       nfc1 = h(C_NFC1);
+      // (the listed-side counts of that read stand: no CP pass follows it here)
+      const int wl2 = h(C_WL2), wl1 = h(C_WL1);
       if (rep < 2) {
         if (nfc == 0) {
-          s_loops(2, 1);
+          s_loops(2, 1, wl2);
           saddle_loops_if(2);
         }
         if (nfc1 == 0) {
-          s_loops(1, 1);
+          s_loops(1, 1, wl1);
           saddle_loops_if(1);
         }
       } else {
         if (nfc == 0) {
-          s_loops(2, 0);
+          s_loops(2, 0, wl2);
           saddle_loops(2, 0);
         }
         if (nfc1 == 0) {
-          s_loops(1, 0);
+          s_loops(1, 0, wl1);
           saddle_loops(1, 0);
         }
       }

@@ -2814,6 +2814,70 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
   cl.flush();
 }
 
+// The three phases of k_wrong_index in one block of 1024 threads, for listed
+// sides the host knows to be few: block barriers order the phases, and the
+// registrations are atomics, which the CU's L1 does not see, so they are read
+// and released with atomic loads and stores.
+__global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do_fix) {
+  const int n = s.ctr[side == 2 ? C_WL2 : C_WL1];
+  const int* WL = side == 2 ? s.wl2 : s.wl1;
+  const int* S = side == 2 ? s.S2 : s.S1;
+  const int* of = side == 2 ? s.of2 : s.of1;
+  const int* ol = side == 2 ? s.ol2 : s.ol1;
+  const int* df = side == 2 ? s.df2 : s.df1;
+  const int* dl = side == 2 ? s.dl2 : s.dl1;
+  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
+  STAGED_COUNT(cf, &s.ctr[side == 2 ? C_WMC : C_WMINC2]);
+  STAGED_COUNT(cl, &s.ctr[side == 2 ? C_WMC2 : C_WMINC]);
+  const int t = threadIdx.x;
+  if (t == 0) {  // counted below (the flushes come after, from this thread)
+    s.ctr[side == 2 ? C_WMC : C_WMINC2] = 0;
+    s.ctr[side == 2 ? C_WMC2 : C_WMINC] = 0;
+  }
+  for (int k = t; k < n; k += 1024) {  // register
+    const int i = WL[k];
+    const int sv = S[i], a = of[i], b = ol[i];
+    if (a != df[i]) atomicMin(&s.win_a[a], sv);
+    if (b != dl[i]) atomicMin(&s.win_b[b], sv);
+  }
+  __syncthreads();
+  for (int k = t; k < n; k += 1024) {  // count and fix
+    const int i = WL[k];
+    const int sv = S[i];
+    const int a = of[i], b = ol[i], c = df[i], d = dl[i];
+    if (a != c &&
+        __hip_atomic_load(&s.win_a[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == sv) {
+      cf.inc();
+      if (side == 2) {
+        if (do_fix) propose(s, a, lower_mid(s, a), tl);
+      } else {
+        if (do_fix && c != kNone) propose(s, c, lower_mid(s, c), tl);
+      }
+    }
+    if (b != d &&
+        __hip_atomic_load(&s.win_b[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == sv) {
+      cl.inc();
+      if (side == 2) {
+        if (do_fix && d != kNone) propose(s, d, lower_mid(s, d), tl);
+      } else {
+        if (do_fix) propose(s, b, lower_mid(s, b), tl);
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = t; k < n; k += 1024) {  // release
+    const int i = WL[k];
+    const int a = of[i], b = ol[i];
+    if (a != df[i])
+      __hip_atomic_store(&s.win_a[a], 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (b != dl[i])
+      __hip_atomic_store(&s.win_b[b], 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  tl.flush();
+  cf.flush();
+  cl.flush();
+}
+
 // get_wrong_index_saddles(_join) + fix_wrong_index_saddle
 // (src/ExtreM.cpp:2672-2715, 2811-2836).  The global extremum has no saddle
 // (its triplet entries are skipped, src/ExtreM.cpp:2330); a side without an
