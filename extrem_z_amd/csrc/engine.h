@@ -226,6 +226,8 @@ __global__ void k_scan_propose(St s, int* front, int fctr, int nzc, int zlen);
 __global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int* front,
                              int fctr, int epoch, int clear_dirty, int zero_idx);
 __global__ void k_clear_dirty(St s, int zero_idx);
+__global__ void k_check_two(St s, const int* flist, int nf, int* front, int fctr, int epoch,
+                            int zero_idx);
 __global__ void k_propose_c(St s, const int* front, int fctr);
 __global__ void k_apply(St s, int zero_idx);
 __global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t* lower,

@@ -594,15 +594,11 @@ void Run::c_loops() {
       check_full(back_buf_, back);
       k_clear_dirty<<<blocks_for(nt), 256, 0, st()>>>(s_, front_);
     } else {
+      // the old entries and the edited neighbourhoods; the consumed front
+      // length is cleared here, so the loop ends with both lengths at zero
       tbegin(EXTREMZ_K_CHECK_INCR);
-      k_check_list<<<blocks_for(nf), 256, 0, st()>>>(s_, front_buf_, front_, 0, back_buf_, back,
-                                                      epoch_, 0, -1);
-      tend();
-      // (the consumed front length is cleared here, so the loop ends with both
-      // lengths at zero)
-      tbegin(EXTREMZ_K_CHECK_INCR);
-      k_check_list<<<blocks_for(15 * nt), 256, 0, st()>>>(s_, s_.dirty, s_.dctr, 1, back_buf_,
-                                                           back, epoch_, 1, front_);
+      k_check_two<<<blocks_for(nf + 15 * nt), 256, 0, st()>>>(s_, front_buf_, nf, back_buf_, back,
+                                                              epoch_, front_);
       tend();
     }
     dirty_consumed();

@@ -386,6 +386,43 @@ __global__ void __launch_bounds__(256) k_check_list(St s, const int* list, int l
   fl.flush();
 }
 
+// The two re-checks of a c_loops round in one launch: lanes [0, nf) take the
+// last frontier's entries (nf passed by value, so its length can be cleared
+// here: zero_idx), the rest one (dirty vertex, candidate) each as
+// k_check_list with nbrs and clear_dirty.  One epoch stamp serves both, as
+// for the two launches it replaces.
+__global__ void __launch_bounds__(256) k_check_two(St s, const int* flist, int nf, int* front,
+                                                   int fctr, int epoch, int zero_idx) {
+  STAGED_LIST(fl, int, kStage, front, &s.ctr[fctr]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s.ctr[s.dctr == C_DIRTY ? C_DIRTY2 : C_DIRTY] = 0;
+    s.ctr[zero_idx] = 0;
+  }
+  const long long n = nf + (long long)s.ctr[s.dctr] * (kNbr + 1);
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    int u;
+    if (k < nf) {
+      u = flist[k];
+    } else {
+      const long long q = k - nf;
+      const int i = (int)(q / (kNbr + 1)), j = (int)(q - (long long)i * (kNbr + 1));
+      u = s.dirty[i];
+      if (j == 0) {
+        s.dflag[u] = 0;
+      } else {
+        int x, y, z;
+        coords(s.g, u, x, y, z);
+        if (!((c_cls_key[class_of(s.g, x, y, z)] >> (j - 1)) & 1)) continue;
+        u += s.g.off[j - 1];
+      }
+    }
+    if (atomicExch(&s.stamp[u], epoch) == epoch) continue;
+    if (mismatch_at(s, u)) fl.push(u);
+  }
+  fl.flush();
+}
+
 // Clears the dirty flags of the listed vertices; the other dirty length (and
 // zero_idx, when given) is set to zero for the next producer.
 __global__ void k_clear_dirty(St s, int zero_idx) {
