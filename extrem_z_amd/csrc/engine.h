@@ -281,7 +281,7 @@ __global__ void k_wn_reset(St s);
 __global__ void k_wn_solve(St s, int2* bufA, int2* bufB, int limit);
 __global__ void k_fixpath(St s, int fast);
 __global__ void k_wrong_index(St s, int side, int phase, int do_fix);
-__global__ void k_wrong_index_one(St s, int side, int do_fix);
+__global__ void k_wrong_index_one(St s, int side, int do_fix, int wsc);
 __global__ void k_wrong_saddle(St s, int side, int do_fix, int gate);
 __global__ void k_saddle_gate(int* ctr, int gate, int wsc);
 __global__ void k_lists_count(St s, int* cnt, int nb, int tiled);

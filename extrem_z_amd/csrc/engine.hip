@@ -364,9 +364,9 @@ class Run {
   void verify_inc();
   void wrong_neighbors();
   void r_loops();
-  void s_loops(int side, int do_fix, int known = -1);
+  bool s_loops(int side, int do_fix, int known = -1, bool gate = false);
   void saddle_loops(int side, int do_fix);
-  void saddle_loops_if(int side);
+  void saddle_loops_if(int side, bool gated = false);
 };
 
 // The CP lists S2 / S1 / MX / MN of the types in s_.otype, in the column
@@ -797,8 +797,10 @@ void Run::r_loops() {
 // s_loops / s_loops_join (src/ExtreM.cpp:2838-2886) with do_fix, or the bare
 // get_wrong_index_max / _min of main.cpp:434,441.
 // known: the side's listed-side count when the last counter read holds it
-// (no CP pass since), else -1.
-void Run::s_loops(int side, int do_fix, int known) {
+// (no CP pass since), else -1.  With gate, a one-block run also opens
+// saddle_loops' gate (returns true then).
+bool Run::s_loops(int side, int do_fix, int known, bool gate) {
+  bool gated = false;
   int n = side == 2 ? s_.n2 : s_.n1;
   if (do_fix) counts_[EXTREMZ_STAT_S_LOOP_CALLS]++;
   if (!n) {  // (else the first wrong-index launch clears them)
@@ -806,8 +808,10 @@ void Run::s_loops(int side, int do_fix, int known) {
     else set_ctr2(C_WMINC, C_WMINC2, 0);
   } else if (known >= 0 && known <= kWrongOne) {  // few listed sides: one block
     tbegin(EXTREMZ_K_PROPOSE);
-    k_wrong_index_one<<<1, 1024, 0, st()>>>(s_, side, do_fix);
+    k_wrong_index_one<<<1, 1024, 0, st()>>>(s_, side, do_fix,
+                                            gate ? (side == 2 ? C_WSC : C_WSCJ) : -1);
     tend();
+    gated = gate;
   } else {
     const int b = std::min(blocks_for(n), 512);  // over the listed sides only
     k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 0, 0);
@@ -817,6 +821,7 @@ void Run::s_loops(int side, int do_fix, int known) {
     k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 2, 0);
   }
   if (do_fix) apply();
+  return gated;
 }
 
 // saddle_loops / saddle_loops_join (src/ExtreM.cpp:2888-2916) with do_fix,
@@ -835,10 +840,11 @@ void Run::saddle_loops(int side, int do_fix) {
 
 // saddle_loops(side, 1) when the side's wrong-index count is zero, decided on
 // the device (no counter read); the calls made are counted there (C_SLC).
-void Run::saddle_loops_if(int side) {
+// gated: the gate is open or closed already (by the one-block wrong-index run).
+void Run::saddle_loops_if(int side, bool gated) {
   const int n = side == 2 ? s_.nmax : s_.nmin;
   const int gate = side == 2 ? C_WMC : C_WMINC;
-  k_saddle_gate<<<1, 64, 0, st()>>>(s_.ctr, gate, side == 2 ? C_WSC : C_WSCJ);
+  if (!gated) k_saddle_gate<<<1, 64, 0, st()>>>(s_.ctr, gate, side == 2 ? C_WSC : C_WSCJ);
   if (n) {
     tbegin(EXTREMZ_K_PROPOSE);
     k_wrong_saddle<<<blocks_for(n), 256, 0, st()>>>(s_, side, 1, gate);
@@ -1277,12 +1283,10 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
       const int wl2 = h(C_WL2), wl1 = h(C_WL1);
       if (rep < 2) {
         if (nfc == 0) {
-          s_loops(2, 1, wl2);
-          saddle_loops_if(2);
+          saddle_loops_if(2, s_loops(2, 1, wl2, true));
         }
         if (nfc1 == 0) {
-          s_loops(1, 1, wl1);
-          saddle_loops_if(1);
+          saddle_loops_if(1, s_loops(1, 1, wl1, true));
         }
       } else {
         if (nfc == 0) {

@@ -2854,8 +2854,9 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
 // The three phases of k_wrong_index in one block of 1024 threads, for listed
 // sides the host knows to be few: block barriers order the phases, and the
 // registrations are atomics, which the CU's L1 does not see, so they are read
-// and released with atomic loads and stores.
-__global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do_fix) {
+// and released with atomic loads and stores.  With wsc >= 0 the block, whose
+// counts are then final, also opens saddle_loops' gate as k_saddle_gate does.
+__global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do_fix, int wsc) {
   const int n = s.ctr[side == 2 ? C_WL2 : C_WL1];
   const int* WL = side == 2 ? s.wl2 : s.wl1;
   const int* S = side == 2 ? s.S2 : s.S1;
@@ -2913,6 +2914,13 @@ __global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do
   tl.flush();
   cf.flush();
   cl.flush();
+  if (wsc >= 0 && t == 0) {  // the flushes' adds were this thread's own
+    const int gate = side == 2 ? C_WMC : C_WMINC;
+    if (__hip_atomic_load(&s.ctr[gate], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+      s.ctr[wsc] = 0;
+      s.ctr[C_SLC]++;
+    }
+  }
 }
 
 // get_wrong_index_saddles(_join) + fix_wrong_index_saddle
