@@ -1201,7 +1201,8 @@ template <int SIDE>
 __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* f, const int* L,
                                             const int* Lo, int v, unsigned star, unsigned ostar,
                                             const int* s_off, int* F, int* La, int o,
-                                            unsigned& cand, int (&U)[kNbr]) {
+                                            unsigned& cand, int (&U)[kNbr], int& f_end,
+                                            int& l_end) {
   const Geo& g = s.g;
   int R[kNbr], O[kNbr];
 #pragma unroll
@@ -1232,9 +1233,11 @@ __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* 
     if (lo == kNone || smaller(V[d], R[d], flo, lo)) { lo = R[d]; flo = V[d]; }
     if (hi == kNone || larger(V[d], R[d], fhi, hi)) { hi = R[d]; fhi = V[d]; }
   }
+  f_end = SIDE == 2 ? lo : hi;
+  l_end = SIDE == 2 ? hi : lo;
   if (o >= 0) {  // o < 0: a lane past the end of its list
-    F[o] = SIDE == 2 ? lo : hi;
-    La[o] = SIDE == 2 ? hi : lo;
+    F[o] = f_end;
+    La[o] = l_end;
   }
   cand = 0;
   if (!dec) return;
@@ -1340,8 +1343,18 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
   int v = 0;
   unsigned star = 0, ostar = 0;
   double fs = 0.0;
+  // what a decp pass compares at the end, loaded with the list entry
+  long long pvi = 0;
+  int ofi = 0, oli = 0;
+  uint8_t wfi = 1;
   if (act) {
     v = (SIDE == 2 ? s.S2 : s.S1)[i];
+    if (dec) {
+      pvi = (SIDE == 2 ? s.pv2 : s.pv1)[i];
+      ofi = (SIDE == 2 ? s.of2 : s.of1)[i];
+      oli = (SIDE == 2 ? s.ol2 : s.ol1)[i];
+      wfi = s.wflag[SIDE == 2 ? i : s.n2 + i];
+    }
     int x, y, z;
     coords(g, v, x, y, z);
     const unsigned key = c_cls_key[class_of(g, x, y, z)];
@@ -1352,8 +1365,9 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
   }
   int E[kNbr];
   unsigned cand = 0;
+  int f_end, l_end;
   saddle_side<SIDE>(s, dec, f, L, SIDE == 2 ? s.oDS : s.oAS, v, star, ostar, s_off, F, La,
-                    act ? i : -1, cand, E);
+                    act ? i : -1, cand, E, f_end, l_end);
   // list positions of the distinct roots (maxima even, minima odd; roots
   // outside the lists have none) where this saddle pushes or is indexed
   const bool changed = dec && (star != ostar || cand != 0);
@@ -1395,7 +1409,7 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
       const long long nk =
           changed ? (SIDE == 2 ? (long long)0x8000000000000000ULL : 0x7fffffffffffffffLL)
                   : okey_c(fs);
-      if (inc && pv[i] != nk) regather_slots<SIDE>(s, v);
+      if (inc && pvi != nk) regather_slots<SIDE>(s, v);
       pv[i] = nk;
     }
   }
@@ -1404,10 +1418,9 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
     if (cand != 0) acl.push(make_int2(2 * v + (SIDE == 1), (int)cand));
     if (act && changed) cl.push(i);
     if (act) {  // the sides k_wrong_index has to look at
-      const bool wm = F[i] != (SIDE == 2 ? s.of2 : s.of1)[i] ||
-                      La[i] != (SIDE == 2 ? s.ol2 : s.ol1)[i];
+      const bool wm = f_end != ofi || l_end != oli;
       const int q = SIDE == 2 ? i : s.n2 + i;
-      if (wm && !s.wflag[q]) {
+      if (wm && !wfi) {
         s.wflag[q] = 1;
         wl.push(i);
       }
