@@ -356,7 +356,9 @@ class Run {
   void apply(long long bound = 1LL << 62, int also = -1);
   void dirty_consumed();
   void label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir = nullptr);
-  void c_loops();
+  void c_loops(bool entered = false);
+  void c_loops_entry();
+  bool proposed_ = false;  // the frontier's proposals are made already
   void cp_dec(bool force, bool fresh = false);
   void cp_inc(long long nch);
   void cp_end();
@@ -543,9 +545,8 @@ void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t
   tend();
 }
 
-// c_loops, src/ExtreM.cpp:1505-1583.  The first call scans the full grid; later
-// calls start from the vertices edited since the last clean state.
-void Run::c_loops() {
+// The start of c_loops: the first false-critical-point check.
+void Run::c_loops_entry() {
   counts_[EXTREMZ_STAT_C_LOOPS_CALLS]++;
   // both frontier lengths are zero here: the correction starts with zeroed
   // counters and every c_loops ends with both at zero (below)
@@ -558,10 +559,10 @@ void Run::c_loops() {
   // frontier (5-30% of the grid) whose proposals cost more as scattered
   // atomics than the scan itself: that scan proposes on the way.  Rescans
   // after many edits find few false points, and the plain scan is cheaper.
-  bool proposed = false;
+  proposed_ = false;
   if (!clean_) {
     scan_propose(front_buf_, front_);
-    proposed = true;
+    proposed_ = true;
     k_clear_dirty<<<blocks_for(s_.g.N), 256, 0, st()>>>(s_, -1);
   } else {
     tbegin(EXTREMZ_K_CHECK_INCR);
@@ -570,8 +571,16 @@ void Run::c_loops() {
     tend();
   }
   dirty_consumed();
-  while (true) {
-    pull();
+}
+
+// c_loops, src/ExtreM.cpp:1505-1583.  The first call scans the full grid; later
+// calls start from the vertices edited since the last clean state.  With
+// entered, c_loops_entry has run and the last counter read came after it.
+void Run::c_loops(bool entered) {
+  if (!entered) c_loops_entry();
+  bool proposed = proposed_;
+  for (bool read = !entered;; read = true) {
+    if (read) pull();
     int nf = h(front_);
     if (nf == 0) break;
     counts_[EXTREMZ_STAT_FRONTIER_ITEMS] += nf;
@@ -783,8 +792,10 @@ void Run::wrong_neighbors() {
 }
 
 // r_loops, src/ExtreM.cpp:2577-2593.
-// Called right after a counter read (the outer loop's test), which says
-// whether decp changed since its last CP pass.
+// Called after a counter read with no edit since (the outer loop's first
+// test, or the wrong-neighbour read of the loop's last repetition, which
+// makes no edits): it holds the wrong-neighbour counts and says whether decp
+// changed since its last CP pass.
 void Run::r_loops() {
   counts_[EXTREMZ_STAT_R_LOOP_CALLS]++;
   const int fast = inc_ok_ && !h(C_ANY_CHANGE);
@@ -1265,16 +1276,19 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
     s_loops(1, 1);
     saddle_loops_if(1);
   }
-  auto any = [&]() {
-    pull();
+  auto any = [&]() {  // on the last counter read
     return h(C_NFC) > 0 || h(C_NFC1) > 0 || h(C_WMC) > 0 || h(C_WMC2) > 0 || h(C_WSC) > 0 ||
            h(C_WMINC) > 0 || h(C_WMINC2) > 0 || h(C_WSCJ) > 0;
   };
+  pull();
+  // entered: the next iteration's r_loops and first c_loops entry have run,
+  // and the last read came after them
+  bool entered = false;
   while (any()) {  // main.cpp:277-446
     if (++counts_[EXTREMZ_STAT_OUTER_ITERS] > max_outer_) throw std::range_error("outer");
-    r_loops();
+    if (!entered) r_loops();
     for (int rep = 0; rep < 3; rep++) {
-      c_loops();
+      c_loops(entered && rep == 0);
       cp_dec(false, true);
       wrong_neighbors();
       nfc = h(C_NFC);
@@ -1299,6 +1313,19 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
         }
       }
     }
+    // The next iteration's r_loops and c_loops entry go before the loop test.
+    // When the loop ends every branch counter is zero, and no edit has
+    // happened since the last c_loops: fixpath has no wrong neighbour and the
+    // dirty list is empty, so both do nothing.  One read then serves the test
+    // and the first frontier length.
+    r_loops();
+    c_loops_entry();
+    pull();
+    entered = true;
+  }
+  if (entered) {  // the last such start began no iteration
+    counts_[EXTREMZ_STAT_R_LOOP_CALLS]--;
+    counts_[EXTREMZ_STAT_C_LOOPS_CALLS]--;
   }
   set_ctr(C_EDITED, 0);
   k_count_edited<<<bN, 256, 0, q>>>(s_.cnt, N, s_.ctr);
