@@ -1328,6 +1328,23 @@ __device__ __forceinline__ void regather_slots(const St& s, int v) {
   }
 }
 
+// A block's table of (slot, count) for the original pass's adjacency counts
+// (neighbouring saddles share their roots): a slot's count is added here,
+// or, when its probe finds no room, straight to the global count.
+constexpr int kSlotCountN = 1024;
+__device__ __forceinline__ void count_slot(int2* hc, int* gcount, int slot) {
+  unsigned h = ((unsigned)slot * 2654435761u) >> 22;  // kSlotCountN = 2^10
+  for (int p = 0; p < 16; p++) {
+    const int old = atomicCAS(&hc[h].x, -1, slot);
+    if (old == -1 || old == slot) {
+      atomicAdd(&hc[h].y, 1);
+      return;
+    }
+    h = (h + 1) & (kSlotCountN - 1);
+  }
+  atomicAdd(&gcount[slot], 1);
+}
+
 // One side for one saddle: the list entry i of S2 (SIDE 2) or S1 (SIDE 1).
 // In kModeDec the wrong-neighbour candidates go to acl (act0) and the changed
 // sides to cl (cln2 / cln1, for the next incremental pass); with inc, a side
@@ -1337,7 +1354,7 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
                                              const uint16_t* flow, int* F, int* La, int i,
                                              bool act, const int* s_off, int2* cpairs,
                                              int* s_np, long long* key_slot, int inc, LA& acl,
-                                             LC& cl, LC& wl) {
+                                             LC& cl, LC& wl, int2* hcount = nullptr) {
   const Geo& g = s.g;
   const int dec = mode == kModeDec;
   int v = 0;
@@ -1389,7 +1406,8 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
     for (int d = 0; d < kNbr; d++)
       if (E[d] >= 0) {
         E[d] = slot_index(s, E[d]);
-        atomicAdd(&s.csr_off[E[d]], 1);
+        if (hcount) count_slot(hcount, s.csr_off, E[d]);
+        else atomicAdd(&s.csr_off[E[d]], 1);
         cnt++;
       }
     int b = wave_reserve(s_np, cnt);
@@ -1443,6 +1461,8 @@ __global__ void __launch_bounds__(256, 4) k_saddle_list(St s, const double* f, c
                                                      long long* kmin, int nb2, int nb) {
   __shared__ int s_off[kNbr];
   __shared__ int s_np;
+  // the original pass counts its adjacencies per slot in the block first
+  __shared__ int2 hc[MODE == kModeOrig ? kSlotCountN : 1];
   STAGED_LIST(sa, int2, 1024, s.act0, &s.ctr[C_ACT0]);
   STAGED_LIST(sc2, int, 1024, s.cln2, &s.ctr[C_CLN2]);
   STAGED_LIST(sc1, int, 1024, s.cln1, &s.ctr[C_CLN1]);
@@ -1450,6 +1470,8 @@ __global__ void __launch_bounds__(256, 4) k_saddle_list(St s, const double* f, c
   STAGED_LIST(sw1, int, 1024, s.wl1, &s.ctr[C_WL1]);
   const int tid = threadIdx.x;
   if (tid < kNbr) s_off[tid] = s.g.off[tid];
+  if (MODE == kModeOrig)
+    for (int j = tid; j < kSlotCountN; j += 256) hc[j] = make_int2(-1, 0);
   if (blockIdx.x == 0 && tid == 0) s.ctr[C_PCHUNK] = nb;
   // balanced ranges: every block (so every XCD) gets its share
   const long long L = xcd_remap(blockIdx.x, gridDim.x);
@@ -1461,14 +1483,21 @@ __global__ void __launch_bounds__(256, 4) k_saddle_list(St s, const double* f, c
     if (c < nb2) {
       const int i = c * 256 + tid;
       saddle_entry<2>(s, MODE, f, L2, flow, F2, La2, i, i < s.n2, s_off, cp, &s_np, kmax, 0, sa,
-                      sc2, sw2);
+                      sc2, sw2, MODE == kModeOrig ? hc : nullptr);
     } else {
       const int i = (c - nb2) * 256 + tid;
       saddle_entry<1>(s, MODE, f, L1, flow, F1, La1, i, i < s.n1, s_off, cp, &s_np, kmin, 0, sa,
-                      sc1, sw1);
+                      sc1, sw1, MODE == kModeOrig ? hc : nullptr);
     }
     __syncthreads();
     if (tid == 0) pcnt[c] = s_np;
+  }
+  if (MODE == kModeOrig) {
+    __syncthreads();
+    for (int j = tid; j < kSlotCountN; j += 256) {
+      const int2 e = hc[j];
+      if (e.x >= 0) atomicAdd(&s.csr_off[e.x], e.y);
+    }
   }
   sa.flush();
   sc2.flush();
