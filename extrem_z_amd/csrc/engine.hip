@@ -488,8 +488,8 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
                                  hipMemcpyDeviceToDevice, st()),
                   "hipMemcpyAsync");
     tbegin(EXTREMZ_K_SLOT_GATHER);
-    k_csr_fill<<<blocks_for(64LL * (nb2 + nb1)), 256, 0, st()>>>(s_, c_->pairs.p, c_->pcnt.p,
-                                                                 nb2 + nb1);
+    k_csr_fill<<<std::max(1, std::min(nb2 + nb1, 8192)), 256, 0, st()>>>(
+        s_, c_->pairs.p, c_->pcnt.p, nb2 + nb1);
     k_slot_pull<<<std::max(1, (ns + 31) / 32), 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
     k_slot_pull_long<<<512, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p);
     k_slot_pull_join<<<64, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p, c_->pkey.p, c_->pid.p);

@@ -1515,17 +1515,56 @@ template __global__ void k_saddle_list<kModeDec>(St, const double*, const int*, 
 // The original extremum->saddle index from the adjacencies the kModeOrig
 // pass recorded per chunk (one wavefront per chunk; the order within a slot's
 // range is free: every reader takes a best over it).
+// A block per chunk: each pair takes its rank among the chunk's pairs of its
+// slot from an LDS table of (slot, count); then one global atomic per
+// distinct slot of the chunk claims the slot's entries, and each pair is
+// written at that base plus its rank.  A pair whose probe finds no room takes
+// a global atomic of its own.
+constexpr int kFillN = 1024;
 __global__ void __launch_bounds__(256) k_csr_fill(St s, const int2* pairs, const int* pcnt,
                                                   int nb) {
-  const int lane = threadIdx.x & 63;
-  const int nw = (gridDim.x * blockDim.x) >> 6;
-  for (int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < nb; c += nw) {
+  __shared__ int2 hc[kFillN];  // (slot, count), then (slot, base)
+  constexpr int kPer = kChunkPairs / 256;
+  const int tid = threadIdx.x;
+  for (int c = blockIdx.x; c < nb; c += gridDim.x) {  // block-uniform
     const int n = pcnt[c];
     const int2* P = pairs + (long long)c * kChunkPairs;
-    for (int j = lane; j < n; j += 64) {
-      const int2 pr = P[j];
-      s.csr[atomicAdd(&s.csr_cur[pr.x], 1)] = pr.y;
+    for (int j = tid; j < kFillN; j += 256) hc[j] = make_int2(-1, 0);
+    __syncthreads();
+    int hs[kPer], rk[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int j = tid + 256 * k;
+      hs[k] = -1;
+      rk[k] = 0;
+      if (j >= n) continue;
+      const int slot = P[j].x;
+      unsigned h = ((unsigned)slot * 2654435761u) >> 22;  // kFillN = 2^10
+      for (int p = 0; p < 32; p++) {
+        const int old = atomicCAS(&hc[h].x, -1, slot);
+        if (old == -1 || old == slot) {
+          hs[k] = (int)h;
+          rk[k] = atomicAdd(&hc[h].y, 1);
+          break;
+        }
+        h = (h + 1) & (kFillN - 1);
+      }
     }
+    __syncthreads();
+    for (int j = tid; j < kFillN; j += 256) {
+      const int2 e = hc[j];
+      if (e.x >= 0) hc[j].y = atomicAdd(&s.csr_cur[e.x], e.y);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int j = tid + 256 * k;
+      if (j >= n) continue;
+      const int2 pr = P[j];
+      const int at = hs[k] >= 0 ? hc[hs[k]].y + rk[k] : atomicAdd(&s.csr_cur[pr.x], 1);
+      s.csr[at] = pr.y;
+    }
+    __syncthreads();
   }
 }
 
