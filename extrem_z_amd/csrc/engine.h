@@ -270,7 +270,6 @@ __global__ void k_inc_saddles(St s, const double* f, int2* pairs, int* pcnt, lon
                               long long* kmin);
 __global__ void k_inc_pull(St s, long long* pkey, int* pid);
 __global__ void k_inc_clear(St s);
-__global__ void k_ctr_copy2(int* ctr, int a, int from_a, int b, int from_b);
 __global__ void k_inc_end_ctr(int* ctr);
 __global__ void k_diff_count(const void* a, const void* b, long long nbytes, int* out);
 __global__ void k_act_copy(St s, int2* dst, int dctr);

@@ -504,7 +504,7 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
     k_slot_pull_long<<<512, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p);
     k_slot_pull_join<<<64, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p, c_->pkey.p, c_->pid.p);
   }
-  k_slot_merge<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, st()>>>(
+  k_slot_merge<<<blocks_for((long long)s_.nmax + s_.nmin), 256, 0, st()>>>(
       s_, kmax, kmin, smax, smin, c_->pkey.p, c_->pid.p);
   // ties among the pushed saddles: one wavefront per chunk region
   if (dec)

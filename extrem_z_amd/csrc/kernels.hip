@@ -2459,13 +2459,6 @@ __global__ void k_inc_clear(St s) {
   GRID_STRIDE(k, nl) s.lflag[s.lchg[k]] = 0;
 }
 
-__global__ void k_ctr_copy2(int* ctr, int a, int from_a, int b, int from_b) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    ctr[a] = ctr[from_a];
-    ctr[b] = ctr[from_b];
-  }
-}
-
 // End of a CP pass of decp: the pass counters start at zero, the changed
 // sides and pushed slots become the last pass's, decp counts as unchanged.
 __global__ void k_inc_end_ctr(int* ctr) {
