@@ -581,7 +581,7 @@ __global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
 // (66x6 values each, halo included; the next plane is prefetched into
 // registers while the current one is processed).  Steepest ascent / descent
 // pointers (first loops of src/ExtreM.cpp:146-157 and 504-515) are computed
-// from LDS and resolved inside the tile by pointer doubling; a pointer then
+// from LDS and resolved inside the tile by walking them in LDS; a pointer then
 // names the root or the first vertex outside the tile on the steepest path.
 // With lower != nullptr the lower-star mask of every voxel is written too,
 // with pdir != nullptr its steepest directions ((up + 1) | (down + 1) << 4,
@@ -711,24 +711,27 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
     sdn[li] = pd;
     __syncthreads();
   }
-  for (int round = 0; round < 11; round++) {
-    int ch = 0;
+  // each voxel walks its in-tile path to a root or an exit (paths are short
+  // on rough fields; a value read on the way is the next step, or, once its
+  // own lane has finished, that path's end: either is an ancestor)
 #pragma unroll
-    for (int k = 0; k < kLTN / 256; k++) {
-      int li = tid + 256 * k;
-      int p = sup[li];
-      if (p >= 0 && p != li) {
-        int q = sup[p];
-        if (q != p) { sup[li] = q; ch = 1; }
-      }
-      p = sdn[li];
-      if (p >= 0 && p != li) {
-        int q = sdn[p];
-        if (q != p) { sdn[li] = q; ch = 1; }
-      }
+  for (int k = 0; k < kLTN / 256; k++) {
+    const int li = tid + 256 * k;
+    int c = li, p = sup[li];
+    for (int step = 0; step < kLTN && p >= 0 && p != c; step++) {
+      c = p;
+      p = sup[c];
     }
-    if (!__syncthreads_or(ch)) break;
+    sup[li] = p >= 0 ? c : p;
+    c = li;
+    p = sdn[li];
+    for (int step = 0; step < kLTN && p >= 0 && p != c; step++) {
+      c = p;
+      p = sdn[c];
+    }
+    sdn[li] = p >= 0 ? c : p;
   }
+  __syncthreads();
 #pragma unroll
   for (int k = 0; k < kLTN / 256; k++) {
     int li = tid + 256 * k;
