@@ -231,7 +231,7 @@ __global__ void k_check_two(St s, const int* flist, int nf, int* front, int fctr
 __global__ void k_propose_c(St s, const int* front, int fctr);
 __global__ void k_apply(St s, int zero_idx);
 __global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t* lower,
-                              uint8_t* pdir);
+                              uint8_t* pdir, const uint8_t* lut, uint8_t* type);
 __global__ void k_chase(int n, int* ds, int* as);
 __global__ void k_list_fill(const int* list, int n, int* arr, int val);
 template <int MODE>

@@ -355,7 +355,8 @@ class Run {
                  int* F2, int* La2, int* F1, int* La1, int* smax, int* smin);
   void apply(long long bound = 1LL << 62, int also = -1);
   void dirty_consumed();
-  void label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir = nullptr);
+  void label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir = nullptr,
+                  uint8_t* type = nullptr);
   void c_loops(bool entered = false);
   void c_loops_entry();
   bool proposed_ = false;  // the frontier's proposals are made already
@@ -534,10 +535,12 @@ void Run::dirty_consumed() { s_.dctr = s_.dctr == C_DIRTY ? C_DIRTY2 : C_DIRTY; 
 // resolution, then a chase of whatever is left (a separate pointer-jumping
 // round before the chase costs more than the chase saves).  No host round
 // trip.
-void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir) {
+void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir,
+                     uint8_t* type) {
   counts_[EXTREMZ_STAT_LABEL_PASSES]++;
   tbegin(EXTREMZ_K_LABEL_TILE);
-  k_labels_tile<<<tile_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as, lower, pdir);
+  k_labels_tile<<<tile_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as, lower, pdir,
+                                                             c_->lut.p, type);
   tend();
   int nb = (s_.g.N + 255) / 256;
   tbegin(EXTREMZ_K_LABEL_CHASE);
@@ -1182,15 +1185,9 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
 
   // main.cpp:67-75
   k_argext<<<bN, 256, 0, q>>>(d_orig, N, s_.ctr + C_GMIN, s_.ctr + C_GMAX);
-  // main.cpp:85-89
-  tbegin(EXTREMZ_K_CLASSIFY);
-  {
-    const ColGrid cg = col_grid();
-    k_mask_cols<0><<<cg.blocks, dim3(64, 4, 1), 0, q>>>(s_, d_orig, s_.otype, s_.olow, nullptr,
-                                                         0, cg.nzc, cg.zlen);
-  }
-  tend();
-  label_pass(d_orig, s_.oDS, s_.oAS, nullptr, s_.opdir);
+  // main.cpp:84-88: the original field's classification (lower masks and
+  // types) comes out of its own label pass
+  label_pass(d_orig, s_.oDS, s_.oAS, s_.olow, s_.opdir, s_.otype);
   // main.cpp:100-119: CP lists, triplets, largest/smallest saddle maps of orig
   list_compaction();
   pull();

@@ -583,7 +583,9 @@ __global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
 // pointers (first loops of src/ExtreM.cpp:146-157 and 504-515) are computed
 // from LDS and resolved inside the tile by walking them in LDS; a pointer then
 // names the root or the first vertex outside the tile on the steepest path.
-// With lower != nullptr the lower-star mask of every voxel is written too,
+// With lower != nullptr the lower-star mask of every voxel is written too
+// (with type != nullptr also its vertex type from the LUT, the classification
+// of src/ExtreM_cuda.cu:3796-3853),
 // with pdir != nullptr its steepest directions ((up + 1) | (down + 1) << 4,
 // 0 for the voxel itself), the state the incremental CP pass starts from.
 //
@@ -598,7 +600,8 @@ constexpr int kPX = kLTX + 2, kPY = kLTY + 2, kPN = kPX * kPY;
 constexpr int kPPer = (kPN + 255) / 256;
 
 __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int* ds, int* as,
-                                                     uint16_t* lower, uint8_t* pdir) {
+                                                     uint16_t* lower, uint8_t* pdir,
+                                                     const uint8_t* lut, uint8_t* type) {
   __shared__ double ring[3][kPN];
   __shared__ int sup[kLTN];
   __shared__ int sdn[kLTN];
@@ -648,7 +651,8 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
     int pu = li, pd = li;
     if (x < g.W && y < g.H && z < g.D) {
       const int v = x + g.W * (y + g.H * z);
-      const int key = c_cls_key[class_of(g, x, y, z)];
+      const int cls = class_of(g, x, y, z);
+      const int key = c_cls_key[cls];
       const int c0 = (lx + 1) + kPX * (ly + 1);
       const double c = P1[c0];
       int du = -1, dd = -1;
@@ -695,6 +699,7 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
         }
       }
       if (lower) lower[v] = (uint16_t)lm;
+      if (type) type[v] = (uint8_t)type_of_mask(lut, cls, lm);  // classifyVertex
       if (pdir) pdir[v] = (uint8_t)((du + 1) | ((dd + 1) << 4));
       if (du >= 0) {
         int ux = lx + c_dx[du], uy = ly + c_dy[du], uz = lz + c_dz[du];
@@ -752,10 +757,10 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
 // each decp value is read once per plane and the halo planes are shared.  A
 // present direction d names a neighbour with a larger id for even d and a
 // smaller one for odd d, so a tie counts as lower exactly for odd d
-// (src/ExtreM.cpp:577-583).  MODE 0: vertex type and mask (classifyVertex,
-// src/ExtreM_cuda.cu:3796-3853) into type / lower.  MODE 1: the full
-// false-critical-point scan (the first get_false_criticle_points of c_loops,
-// src/ExtreM.cpp:1513-1519) appending to the frontier, one append per block.
+// (src/ExtreM.cpp:577-583).  MODE 1 (the one instantiated): the full
+// false-critical-point scan of get_false_criticle_points (src/ExtreM.cpp:
+// 1513-1519), appending to the frontier, one append per block; the
+// original field's classification comes out of its label pass.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_t* type,
                                                    uint16_t* lower, int* front, int fctr,
@@ -820,10 +825,7 @@ __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_
         const bool lo = fr < c || ((d & 1) && fr == c);
         lm |= (unsigned)(((key >> d) & 1) && lo) << d;
       }
-      if (MODE == 0) {
-        type[v] = (uint8_t)type_of_mask(s.lut, cls, lm);
-        lower[v] = (uint16_t)lm;
-      } else if (lm != s.olow[v]) {
+      if (lm != s.olow[v]) {
         const int ot = s.otype[v];
         if (type_of_mask(s.lut, cls, lm) != ot || is_saddle_type(ot)) fl.push(v);
       }
@@ -832,8 +834,6 @@ __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_
   }
   fl.flush();
 }
-template __global__ void k_mask_cols<0>(St, const double*, uint8_t*, uint16_t*, int*, int, int,
-                                        int);
 template __global__ void k_mask_cols<1>(St, const double*, uint8_t*, uint16_t*, int*, int, int,
                                         int);
 
