@@ -1580,7 +1580,7 @@ __global__ void __launch_bounds__(256) k_csr_fill(St s, const int2* pairs, const
 // are cut into chunks of kPullChunk (k_long_slots), one block per chunk
 // (k_slot_pull_long), joined by k_slot_pull_join.
 constexpr int kPullShort = 256;
-constexpr int kPullGroup = 8;
+constexpr int kPullGroup = 4;
 constexpr int kPullChunk = 1024;
 
 __device__ __forceinline__ bool pull_better(bool mx, long long ka, int pa, long long kb, int pb) {
