@@ -304,7 +304,7 @@ class Run {
   // Blocks of a column sweep (one frontier append each).
   static constexpr int kColBlocks = 4096;
   // Blocks of the incremental saddle launches (sizes known on the device only).
-  static constexpr int kIncSaddleBlocks = 1024;
+  static constexpr int kIncSaddleBlocks = 2048;
   // Wrong-neighbour reservation rounds launched per read of the list length.
   static constexpr int kWnRounds = 4;
   // Listed sides up to which the wrong-index phases run in one block.
