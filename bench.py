@@ -1,21 +1,27 @@
 """Benchmark: corrected voxels/s of the ExtreM correction on one MI355X.
 
-Workload (default "256"): one float32 field 256x256x256, sum of 32 anisotropic
-Gaussians + 1e-3*range uniform noise (seed 2), xi = 1e-4 * range, decp the
-uniform quantiser 2*xi*rint(orig/(2*xi)).  A step is one full correction call
-(main.cpp:65-446 semantics) on device-resident inputs; each step gets its own
-pristine copy of decp, prepared before the timed region.  "512" is the
-512^3 Gaussian random field (k^-3, xi = 1e-3); "64" the 64^3 CPU-sized case.
+Workload (default "512"): one float32 field 512x512x512, a Gaussian random
+field with power spectrum k^-3 (seed 3) normalised to [0, 1], xi = 1e-3,
+decp the uniform quantiser 2*xi*rint(orig/(2*xi)).  A step is one full
+correction call (main.cpp:65-446 semantics) on device-resident inputs; each
+step gets its own pristine copy of decp, prepared before the timed region.
+"256" is the 256^3 sum of 32 Gaussians (er 1e-4); "64" the 64^3 CPU-sized
+case.  After the timed steps, the last step's corrected field and edit
+counts are hashed and compared with the oracle's digest of the same
+workload (tests/golden/*_digest.json) when one exists: "verified".
 
-Prints one JSON line.  The roofline object is for the dominant single kernel:
-the kernel family with the largest summed time in the warm-up call (every
-family timed there); the timed steps record HIP events around that family's
-launches only.  achieved = algorithmic bytes per launch / average launch time,
+The roofline object is for the dominant single kernel: among the families
+that time one kernel per event pair (every family except the sequences
+incremental, slot_gather, propose and reserve, whose kernels rocprof lists
+one by one, each below the dominant one), the family with the largest
+summed time in the warm-up call (every family timed there).  The timed
+steps record HIP events around that kernel's launches only, on the stream
+it runs on.  achieved = algorithmic bytes per launch / average launch time,
 with the per-unit bytes of ALG_BYTES.  traffic is the measured HBM bytes per
 launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a
-plain copy) when a summary for this kernel and workload exists under
-profiles/.  cpu_baseline is the CPU oracle on a bounded sample of the same
-workload recipe.
+plain copy) from profiles/<kernel>_<workload>_traffic.md when present.
+cpu_baseline is the CPU oracle on the 64^3 CPU-sized case, with the oracle's
+time on the full workload (from the digest) beside it.
 
 "batch" is the mixed-size batch: 64 float32 fields, shapes log-uniform
 per axis from 32^3 to 384x384x256, Gaussian sums, er from {1e-2, 1e-3, 1e-4};
@@ -25,7 +31,7 @@ of jobs (cubic edges log-uniform 16..512, Gaussian sums and k^-3 random
 fields, er from the same set) at offered loads below saturation; it reports
 p50 / p99 job latency and the sustained voxels/s per load.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 256|512|64|batch|serve]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 512|256|64|batch|serve]
     python bench.py --calibrate-copy      # plain 1 GiB copies, for the counter scale
 """
 from __future__ import annotations
@@ -45,19 +51,29 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "corrected voxels/s + % HBM roofline (byte-exact vs CPU); p50/p99 job latency"
 
-# Algorithmic bytes per unit for the single-kernel families (unit, bytes):
-#   label_tile  per voxel: read decp f64 (8) + write both pointer labels (4 + 4)
-#               + write the lower-star mask (2)
-#   saddle_pass per saddle side: one frontier item, 15 x (8 B value + 4 B label)
-#   classify / check_full per voxel: read f64 (8) + type (1) + mask (2)
-#   label_chase: the walk of the pointers left by the tiles carries no algorithmic bytes
+# Algorithmic bytes per unit of the single-kernel families (unit, bytes), the
+# compulsory bytes of each pass of the reference algorithm:
+#   label_tile   per voxel: read decp f64 (8) + write both pointer labels (4 + 4)
+#                + write the lower-star mask (2)
+#   saddle_pass, saddle_orig  per saddle side: one frontier item, 15 x (8 B value
+#                + 4 B label)
+#   scan_propose, check_full, classify  per voxel: read f64 (8) + type (1) + mask (2)
+#   label_chase  the walk of the pointers left by the tiles: no algorithmic bytes
+#                (extra pointer-jumping rounds are excluded from the model)
+# apply (30 B per touched voxel), check_incr and csr_fill are list-sized: their
+# unit counts stay on the device, so they carry no byte model here.
 ALG_BYTES = {
     "label_tile": ("voxel", 18),
     "saddle_pass": ("saddle side", 180),
-    "classify": ("voxel", 11),
+    "saddle_orig": ("saddle side", 180),
+    "scan_propose": ("voxel", 11),
     "check_full": ("voxel", 11),
+    "classify": ("voxel", 11),
     "label_chase": ("voxel", 0),
 }
+LIST_SIZED = ("apply", "check_incr", "csr_fill")
+SEQUENCES = ("incremental", "slot_gather", "propose", "reserve")
+DIGESTS = {"512": "grf_512_digest.json", "256": "gauss32_256_digest.json"}
 
 
 def workload(name):
@@ -77,6 +93,7 @@ def workload(name):
 
 
 def units(fam, st, n):
+    """Units one launch of family fam processes (host-known for ALG_BYTES)."""
     if ALG_BYTES[fam][0] == "saddle side":
         return st["n_saddle1"] + st["n_saddle2"]
     return n
@@ -91,18 +108,44 @@ def traffic_from_profiles(fam, workload_name):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(threads):
-    """Oracle on a bounded sample: a 80^3 field built by the workload's recipe."""
+def cpu_baseline(threads, workload_name):
+    """The oracle on the 64^3 CPU-sized case (sum of 8 Gaussians + 1e-3 noise,
+    seed 1, er 1e-3), plus the oracle's time on the full workload from the
+    digest fixture (measured in the build container when it was written)."""
     from extrem_z_amd import synth
     from oracle import oracle
-    o, d, xi = synth.make_case("gauss", (80, 80, 80), seed=2, er=1e-4, n_gauss=2)
+    o, d, xi = synth.config(1)
     t = time.perf_counter()
     f, c, st = oracle.correct(o, d, xi, threads=threads)
     dt = time.perf_counter() - t
-    return {"value": o.size / dt, "unit": "voxels/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ restatement, OpenMP {threads} threads) on one 80^3 field, "
-                      f"2 Gaussians + 1e-3 noise, er=1e-4: {dt:.2f} s, "
-                      f"{st['outer_iters']} outer iterations"}
+    out = {"value": o.size / dt, "unit": "voxels/s", "cores": threads, "kind": "port",
+           "sample": f"oracle (C++ restatement, OpenMP {threads} threads) on the 64^3 CPU case "
+                     f"(8 Gaussians + 1e-3 noise, seed 1, er 1e-3): {dt:.2f} s, "
+                     f"{st['outer_iters']} outer iterations"}
+    dig = DIGESTS.get(workload_name)
+    path = os.path.join(ROOT, "tests", "golden", dig) if dig else None
+    if path and os.path.exists(path):
+        ref = json.load(open(path))
+        n = int(np.prod(ref["shape"]))
+        out["full_workload"] = {"seconds": ref["oracle_seconds"], "threads": ref["oracle_threads"],
+                                "voxels_per_s": n / ref["oracle_seconds"],
+                                "source": f"tests/golden/{dig} (8-CPU build container)"}
+    return out
+
+
+def verify_digest(workload_name, fixed_bytes, counts_bytes, st):
+    """Compares one corrected field with the oracle's digest of the workload."""
+    import hashlib
+    dig = DIGESTS.get(workload_name)
+    path = os.path.join(ROOT, "tests", "golden", dig) if dig else None
+    if not path or not os.path.exists(path):
+        return None
+    ref = json.load(open(path))
+    same_f = hashlib.sha256(fixed_bytes).hexdigest() == ref["sha256_fixed"]
+    same_c = hashlib.sha256(counts_bytes).hexdigest() == ref["sha256_counts"]
+    same_s = all(st[k] == v for k, v in ref["stats"].items())
+    return {"equal": bool(same_f and same_c and same_s), "fixed_sha256": same_f,
+            "counts_sha256": same_c, "counters": same_s, "digest": f"tests/golden/{dig}"}
 
 
 def calibrate_copy():
@@ -358,11 +401,11 @@ def run_serve(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="256")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="512")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--lanes", type=int, default=8)
     ap.add_argument("--budget-gib", type=float, default=200.0)
@@ -389,8 +432,7 @@ def main():
     orig, decp, xi, desc = workload(args.workload)
     D, H, W = orig.shape
     N = orig.size
-    print(f"bench: workload {desc}, N={N}, steps={args.steps}, warmup={args.warmup}",
-          file=sys.stderr, flush=True)
+    log(f"workload {desc}, N={N}, steps={args.steps}, warmup={args.warmup}")
     d_orig = DeviceBuffer.from_numpy(orig.ravel())
     copies = [DeviceBuffer.from_numpy(decp.ravel()) for _ in range(args.steps + max(1, args.warmup))]
     d_counts = DeviceBuffer.zeros(N, np.uint8)
@@ -400,9 +442,12 @@ def main():
     for i in range(max(1, args.warmup)):
         warm = ctx.correct_device(d_orig.ptr, copies[i].ptr, (W, H, D), xi,
                                   d_counts.ptr, timing=True)
-        print(f"bench: warmup {i} done, {warm['wall_ms']:.1f} ms", file=sys.stderr, flush=True)
-    # dominant single kernel of the warm-up call
-    dom = max(ALG_BYTES, key=lambda k: warm["kernel_ms"][k])
+        log(f"warmup {i} done, {warm['wall_ms']:.1f} ms")
+    # dominant single kernel of the warm-up call: every family that times one
+    # kernel per event pair (the sequences are several kernels each)
+    singles = [k for k in warm["kernel_ms"] if k not in SEQUENCES]
+    dom = max(singles, key=lambda k: warm["kernel_ms"][k])
+    seq_top = max(SEQUENCES, key=lambda k: warm["kernel_ms"][k])
     mask = 1 << ez.KERNEL_FAMILIES.index(dom)
     synchronize()
     stats = []
@@ -418,17 +463,31 @@ def main():
         assert st["status"] == 0, st
     ms_per_step = elapsed * 1e3 / args.steps
     value = N * args.steps / elapsed
+    # the last timed step's result against the oracle's digest (outside the
+    # timed region)
+    verified = verify_digest(args.workload, copies[base + args.steps - 1].numpy().tobytes(),
+                             d_counts.numpy().tobytes(), stats[-1])
+    log(f"verified against the oracle's digest: {verified}")
 
     launches = sum(s["kernel_launches"][dom] for s in stats)
     avg_ms = sum(s["kernel_ms"][dom] for s in stats) / max(1, launches)
-    unit, per_unit = ALG_BYTES[dom]
-    bytes_launch = per_unit * units(dom, stats[-1], N)
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+    roofline = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "traffic": traffic_from_profiles(dom, args.workload),
-                "kernel": dom, "launches": launches, "avg_launch_ms": round(avg_ms, 4),
-                "alg_bytes_per_launch": bytes_launch, "alg_bytes_unit": f"{per_unit} B/{unit}"}
+                "kernel": dom, "launches": launches, "avg_launch_ms": round(avg_ms, 4)}
+    if dom in ALG_BYTES:
+        unit, per_unit = ALG_BYTES[dom]
+        n_units = units(dom, stats[-1], N)
+        bytes_launch = per_unit * n_units
+        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+        roofline.update({"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "alg_bytes_per_launch": bytes_launch, "units_per_launch": n_units,
+                         "alg_bytes_unit": f"{per_unit} B/{unit}"})
+    else:  # a list-sized kernel: its unit counts stay on the device
+        roofline.update({"achieved": None, "frac": None, "alg_bytes_per_launch": None})
+    roofline = {k: roofline[k] for k in ("bound", "achieved", "peak", "unit", "frac", "traffic",
+                                         "kernel", "launches", "avg_launch_ms",
+                                         "alg_bytes_per_launch", "units_per_launch",
+                                         "alg_bytes_unit") if k in roofline}
     out = {
         "metric": METRIC,
         "value": value,
@@ -442,21 +501,24 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
+        "verified": verified,
         "config": {"workload": desc, "shape": [W, H, D], "xi": xi,
                    "outer_iters": stats[-1]["outer_iters"],
                    "c_loop_iters": stats[-1]["c_loop_iters"],
                    "label_passes": stats[-1]["label_passes"],
+                   "inc_passes": stats[-1]["inc_passes"],
                    "edited": stats[-1]["edited"],
                    "host_syncs": stats[-1]["host_syncs"],
                    "call_latency_ms": {"p50": pct([s["wall_ms"] for s in stats], 50),
                                        "p99": pct([s["wall_ms"] for s in stats], 99)},
                    "kernel_ms_warmup_call": {k: round(v, 3) for k, v in warm["kernel_ms"].items()},
-                   "launches_per_call": warm["kernel_launches"]},
+                   "launches_per_call": warm["kernel_launches"],
+                   "largest_sequence_family": seq_top},
         "roofline": roofline,
     }
     if not args.no_cpu:
-        print("bench: cpu baseline (oracle on a bounded sample)", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.cpu_threads)
+        log("cpu baseline (oracle on the 64^3 CPU case)")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_threads, args.workload)
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -30,7 +30,9 @@ STAT_KEYS = ("status", "c_loop_iters", "outer_iters", "r_loop_calls", "s_loop_ca
              "saddle_loop_calls", "c_loops_calls", "edited", "n_max", "n_min", "n_saddle1",
              "n_saddle2", "label_passes", "inc_passes", "host_syncs", "frontier_items")
 KERNEL_FAMILIES = ("classify", "check_full", "check_incr", "label_tile", "incremental",
-                   "label_chase", "saddle_pass", "slot_gather", "propose", "apply", "reserve")
+                   "label_chase", "saddle_pass", "slot_gather", "propose", "apply", "reserve",
+                   "scan_propose", "saddle_orig", "csr_fill", "_14", "_15")
+N_FAMILIES = 16
 TIME_ALL = (1 << len(KERNEL_FAMILIES)) - 1
 
 ERRORS = {-3: "iteration cap reached", -5: "HIP runtime error", -11: "not finished",
@@ -51,14 +53,15 @@ class Opts(ctypes.Structure):
 
 
 class Stats(ctypes.Structure):
-    _fields_ = [("counts", ctypes.c_int64 * 16), ("kernel_ms", ctypes.c_double * 11),
-                ("kernel_launches", ctypes.c_int64 * 11), ("wall_ms", ctypes.c_double)]
+    _fields_ = [("counts", ctypes.c_int64 * 16), ("kernel_ms", ctypes.c_double * N_FAMILIES),
+                ("kernel_launches", ctypes.c_int64 * N_FAMILIES), ("wall_ms", ctypes.c_double)]
 
     def as_dict(self):
         d = {k: int(self.counts[i]) for i, k in enumerate(STAT_KEYS)}
-        d["kernel_ms"] = {k: float(self.kernel_ms[i]) for i, k in enumerate(KERNEL_FAMILIES)}
+        d["kernel_ms"] = {k: float(self.kernel_ms[i]) for i, k in enumerate(KERNEL_FAMILIES)
+                          if not k.startswith("_")}
         d["kernel_launches"] = {k: int(self.kernel_launches[i])
-                                for i, k in enumerate(KERNEL_FAMILIES)}
+                                for i, k in enumerate(KERNEL_FAMILIES) if not k.startswith("_")}
         d["wall_ms"] = float(self.wall_ms)
         return d
 
@@ -111,6 +114,7 @@ def lib():
         L.extremz_correct_device.argtypes = [vp, vp, vp, i32, i32, i32, dbl, vp, vp, vp]
         L.extremz_classify.argtypes = [vp, i32, i32, i32, vp, vp]
         L.extremz_labels.argtypes = [vp, i32, i32, i32, vp, vp]
+        L.extremz_label_pass.argtypes = [vp, i32, i32, i32, vp, vp, vp, vp]
         L.extremz_lut.argtypes = [vp, i32]
         L.extremz_encode.argtypes = [vp, vp, vp, vp, i64, dbl, vp, vp, vp, vp, vp]
         L.extremz_last_error.restype = ctypes.c_char_p
@@ -171,6 +175,18 @@ def labels(field):
     asc = np.empty(f.shape, np.int32)
     _check(lib().extremz_labels(_p(f), *_dims(f), _p(ds), _p(asc)))
     return ds, asc
+
+
+def label_pass(field):
+    """Types, lower masks and both root labels from one tile pass (the
+    original field's pass of the correction).  Returns (types, masks, ds, as)."""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    t = np.empty(f.shape, np.uint8)
+    m = np.empty(f.shape, np.uint16)
+    ds = np.empty(f.shape, np.int32)
+    asc = np.empty(f.shape, np.int32)
+    _check(lib().extremz_label_pass(_p(f), *_dims(f), _p(ds), _p(asc), _p(t), _p(m)))
+    return t, m, ds, asc
 
 
 def _timing_mask(timing):
@@ -299,8 +315,10 @@ class Pool:
         rc = lib().extremz_serve_wait(self.handle, int(ticket), int(timeout_ms))
         if rc == EAGAIN:
             return None
-        job, _ = self._held.pop(ticket)
-        return job
+        held = self._held.pop(ticket, None)
+        if held is None:  # another thread received this ticket's status
+            raise ExtremzError(rc, lib().extremz_last_error().decode(errors="replace"))
+        return held[0]
 
     def wait_any(self, timeout_ms=-1):
         """Returns (ticket, Job) of the next finished job, or None on time-out."""

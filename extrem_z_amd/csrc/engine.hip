@@ -223,7 +223,8 @@ class Run {
   int correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
               uint8_t* d_counts);
   void classify(const double* d_f, int W, int H, int D, uint8_t* d_t, uint16_t* d_l);
-  int labels(const double* d_f, int W, int H, int D, int* d_ds, int* d_as);
+  int labels(const double* d_f, int W, int H, int D, int* d_ds, int* d_as,
+             uint8_t* d_t = nullptr, uint16_t* d_l = nullptr);
   void split_tree(const double* d_f, int W, int H, int D, TreeOut& out);
 
   int64_t counts_[EXTREMZ_STAT_COUNT_] = {};
@@ -281,7 +282,7 @@ class Run {
   // list)
   void scan_propose(int* fbuf, int fctr) {
     const ColGrid cg = col_grid();
-    tbegin(EXTREMZ_K_CHECK_FULL);
+    tbegin(EXTREMZ_K_SCAN_PROPOSE);
     k_scan_propose<<<cg.blocks, dim3(64, 4, 1), 0, st()>>>(s_, fbuf, fctr, cg.nzc, cg.zlen);
     tend();
   }
@@ -442,7 +443,7 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
       set_ctr(C_PCHUNK, 0);
       return;
     }
-    tbegin(EXTREMZ_K_SADDLE_PASS);
+    tbegin(mode == kModeOrig ? EXTREMZ_K_SADDLE_ORIG : EXTREMZ_K_SADDLE_PASS);
     auto* kern = mode == kModeOrig ? k_saddle_list<kModeOrig> : k_saddle_list<kModeDec>;
     const int nb = nb2 + nb1;
     kern<<<std::min(nb, std::max(kSaddleBlocks, nb / 4)), 256, 0, st()>>>(
@@ -488,9 +489,11 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
     ez::hip_check(hipMemcpyAsync(s_.csr_cur, s_.csr_off, sizeof(int) * (ns + 1),
                                  hipMemcpyDeviceToDevice, st()),
                   "hipMemcpyAsync");
-    tbegin(EXTREMZ_K_SLOT_GATHER);
+    tbegin(EXTREMZ_K_CSR_FILL);
     k_csr_fill<<<std::max(1, std::min(nb2 + nb1, 8192)), 256, 0, st()>>>(
         s_, c_->pairs.p, c_->pcnt.p, nb2 + nb1);
+    tend();
+    tbegin(EXTREMZ_K_SLOT_GATHER);
     k_slot_pull<<<std::max(1, (ns + 31) / 32), 256, 0, st()>>>(s_, c_->pkey.p, c_->pid.p);
     k_slot_pull_long<<<512, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p);
     k_slot_pull_join<<<64, 256, 0, st()>>>(s_, c_->ckey.p, c_->cpos.p, c_->pkey.p, c_->pid.p);
@@ -877,11 +880,13 @@ void Run::classify(const double* d_f, int W, int H, int D, uint8_t* d_t, uint16_
   ez::hip_check(hipStreamSynchronize(st()), "sync");
 }
 
-int Run::labels(const double* d_f, int W, int H, int D, int* d_ds, int* d_as) {
+int Run::labels(const double* d_f, int W, int H, int D, int* d_ds, int* d_as, uint8_t* d_t,
+                uint16_t* d_l) {
   setup_geo(W, H, D);
   c_->ctr.ensure(C_COUNT_);
   s_.ctr = c_->ctr.p;
-  label_pass(d_f, d_ds, d_as, nullptr);
+  if (d_t && c_->lut.p == nullptr) throw HipError("context has no LUT");
+  label_pass(d_f, d_ds, d_as, d_l, nullptr, d_t);
   ez::hip_check(hipGetLastError(), "labels");
   ez::hip_check(hipStreamSynchronize(st()), "sync");
   return 0;
@@ -1579,6 +1584,33 @@ int extremz_labels(const double* field, int W, int H, int D, int32_t* ds, int32_
     r.labels(c->orig_tmp.p, W, H, D, c->oDS.p, c->oAS.p);
     ez::hip_check(hipMemcpyAsync(ds, c->oDS.p, N * 4, hipMemcpyDeviceToHost, c->stream), "d2h");
     ez::hip_check(hipMemcpyAsync(as, c->oAS.p, N * 4, hipMemcpyDeviceToHost, c->stream), "d2h");
+    ez::hip_check(hipStreamSynchronize(c->stream), "sync");
+    return EXTREMZ_OK;
+  });
+}
+
+int extremz_label_pass(const double* field, int W, int H, int D, int32_t* ds, int32_t* as,
+                       uint8_t* types, uint16_t* lower_masks) {
+  if (!field || !ds || !as || !types || !lower_masks || !ez::valid_shape(W, H, D))
+    return EXTREMZ_EINVAL;
+  std::lock_guard<std::mutex> lk(ez::g_cache_mu);
+  return ez::guarded([&] {
+    extremz_ctx* c = ez::cached_ctx(0);
+    if (!c) return EXTREMZ_ENODEV;
+    ez::hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const size_t N = (size_t)W * H * D;
+    c->orig_tmp.ensure(N);
+    c->oDS.ensure(N);
+    c->oAS.ensure(N);
+    c->otype.ensure(N);
+    c->olow.ensure(N);
+    ez::hip_check(hipMemcpyAsync(c->orig_tmp.p, field, N * 8, hipMemcpyHostToDevice, c->stream), "h2d");
+    ez::Run r(c, nullptr);
+    r.labels(c->orig_tmp.p, W, H, D, c->oDS.p, c->oAS.p, c->otype.p, c->olow.p);
+    ez::hip_check(hipMemcpyAsync(ds, c->oDS.p, N * 4, hipMemcpyDeviceToHost, c->stream), "d2h");
+    ez::hip_check(hipMemcpyAsync(as, c->oAS.p, N * 4, hipMemcpyDeviceToHost, c->stream), "d2h");
+    ez::hip_check(hipMemcpyAsync(types, c->otype.p, N, hipMemcpyDeviceToHost, c->stream), "d2h");
+    ez::hip_check(hipMemcpyAsync(lower_masks, c->olow.p, N * 2, hipMemcpyDeviceToHost, c->stream), "d2h");
     ez::hip_check(hipStreamSynchronize(c->stream), "sync");
     return EXTREMZ_OK;
   });

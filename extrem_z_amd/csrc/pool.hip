@@ -209,21 +209,29 @@ int submit(extremz_pool* p, extremz_job* job, int64_t* ticket) {
   return EXTREMZ_OK;
 }
 
+// Another thread may wait for the same ticket and retire (free) its job
+// first, so the job is looked up again after every wake, never held.
 int wait_ticket(extremz_pool* p, int64_t ticket, int timeout_ms) {
   std::unique_lock<std::mutex> lk(p->mu);
-  auto it = p->live.find(ticket);
-  if (it == p->live.end()) {
+  if (p->live.find(ticket) == p->live.end()) {
     ez::g_err = "unknown ticket";
     return EXTREMZ_EINVAL;
   }
-  PJob* j = it->second;
+  auto settled = [&] {
+    auto it = p->live.find(ticket);
+    return it == p->live.end() || it->second->done;
+  };
   if (timeout_ms < 0) {
-    p->cv_done.wait(lk, [&] { return j->done; });
-  } else if (!p->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms),
-                                  [&] { return j->done; })) {
+    p->cv_done.wait(lk, settled);
+  } else if (!p->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), settled)) {
     return EXTREMZ_EAGAIN;
   }
-  return p->retire(j);
+  auto it = p->live.find(ticket);
+  if (it == p->live.end()) {
+    ez::g_err = "ticket already waited for by another caller";
+    return EXTREMZ_EINVAL;
+  }
+  return p->retire(it->second);
 }
 
 }  // namespace

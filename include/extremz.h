@@ -53,21 +53,31 @@ enum {
 };
 
 /* Kernel families timed with HIP events on the context stream.  Each event
- * pair brackets one kernel launch, except EXTREMZ_K_SLOT_GATHER, EXTREMZ_K_PROPOSE
- * and EXTREMZ_K_RESERVE, which bracket the short kernel sequences named below. */
+ * pair brackets one launch of one kernel, except the families marked
+ * "sequence", whose event pairs bracket the short kernel sequences named. */
 enum {
-  EXTREMZ_K_CLASSIFY = 0, /* full-grid classification of the original */
-  EXTREMZ_K_CHECK_FULL,   /* full-grid false-critical-point scan */
-  EXTREMZ_K_CHECK_INCR,   /* frontier / edited-neighbourhood re-check */
-  EXTREMZ_K_LABEL_TILE,   /* steepest pointers + lower masks, resolved inside LDS tiles */
-  EXTREMZ_K_INCREMENTAL,  /* incremental CP pass of decp: pointer, root and saddle updates */
-  EXTREMZ_K_LABEL_CHASE,  /* full-grid walk of the remaining pointers to their roots */
-  EXTREMZ_K_SADDLE_PASS,  /* saddle-list pass: triplet ends, candidates, changed saddles */
-  EXTREMZ_K_SLOT_GATHER,  /* largest / smallest saddle per extremum (gather + merge + ties) */
-  EXTREMZ_K_PROPOSE,      /* correction proposals (all loops) */
-  EXTREMZ_K_APPLY,        /* apply_delta over touched vertices */
-  EXTREMZ_K_RESERVE,      /* wrong-neighbour reservation rounds */
-  EXTREMZ_K_FAMILIES_ = 11
+  EXTREMZ_K_CLASSIFY = 0, /* k_classify: full-grid classification (extremz_classify) */
+  EXTREMZ_K_CHECK_FULL,   /* k_mask_cols: full-grid false-critical-point rescan */
+  EXTREMZ_K_CHECK_INCR,   /* k_check_list / k_check_two: frontier and edited-neighbourhood
+                             re-checks */
+  EXTREMZ_K_LABEL_TILE,   /* k_labels_tile: steepest pointers + lower masks, resolved
+                             inside LDS tiles */
+  EXTREMZ_K_INCREMENTAL,  /* sequence: incremental CP pass of decp (pointer, root and
+                             saddle updates) */
+  EXTREMZ_K_LABEL_CHASE,  /* k_chase: full-grid walk of the remaining pointers to roots */
+  EXTREMZ_K_SADDLE_PASS,  /* k_saddle_list (decp mode): triplet ends, candidates,
+                             changed saddles */
+  EXTREMZ_K_SLOT_GATHER,  /* sequence: largest / smallest saddle per extremum (gather,
+                             merge, ties) */
+  EXTREMZ_K_PROPOSE,      /* sequence: correction proposals (all loops) */
+  EXTREMZ_K_APPLY,        /* k_apply: apply_delta over touched vertices */
+  EXTREMZ_K_RESERVE,      /* sequence: wrong-neighbour reservation rounds */
+  EXTREMZ_K_SCAN_PROPOSE, /* k_scan_propose: the first full false-critical-point scan of
+                             a call, which makes the frontier's proposals */
+  EXTREMZ_K_SADDLE_ORIG,  /* k_saddle_list (original mode): triplet ends and the
+                             extremum-saddle adjacency counts of the original field */
+  EXTREMZ_K_CSR_FILL,     /* k_csr_fill: the extremum -> saddle index of the original */
+  EXTREMZ_K_FAMILIES_ = 16
 };
 
 typedef struct extremz_opts {
@@ -125,6 +135,14 @@ int extremz_classify(const double* field, int W, int H, int D, uint8_t* types,
  * (src/ExtreM.cpp:144-183, 502-541).  Host pointers. */
 int extremz_labels(const double* field, int W, int H, int D, int32_t* ds,
                    int32_t* as);
+
+/* Phases (a) and (b) of the original field in one tile pass, as the
+ * correction computes them (main.cpp:85-89: classifyAllVertices, then
+ * ComputeDescendingManifold / ComputeAscendingManifold): the labels of
+ * extremz_labels with the types and lower masks of extremz_classify, the
+ * latter from the tile pass's own stencil scan.  Host pointers. */
+int extremz_label_pass(const double* field, int W, int H, int D, int32_t* ds, int32_t* as,
+                       uint8_t* types, uint16_t* lower_masks);
 
 /* The classification table in the layout of LUT.bin (24 800 int32 over the
  * 27 boundary keys of src/ExtreM_cuda.cu:84-110), generated from the link
@@ -269,7 +287,9 @@ int extremz_batch_correct(extremz_pool* pool, extremz_job* jobs, int n);
 int extremz_serve_submit(extremz_pool* pool, extremz_job* job, int64_t* ticket);
 /* Waits up to timeout_ms (< 0: no limit) for the job of ticket; returns its
  * status, or EXTREMZ_EAGAIN if it is still queued or running.  A ticket can
- * be waited for until it has returned a status once. */
+ * be waited for until it has returned a status once; several threads may
+ * wait for one ticket (or for it and any ticket): one of them receives the
+ * status, the others EXTREMZ_EINVAL. */
 int extremz_serve_wait(extremz_pool* pool, int64_t ticket, int timeout_ms);
 /* Waits up to timeout_ms for any finished job that has not been waited for;
  * stores its ticket.  Returns its status or EXTREMZ_EAGAIN. */

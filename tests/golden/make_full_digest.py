@@ -1,11 +1,16 @@
-"""Writes the full-size digest fixture: the oracle's corrected field and edit
-counts for the 256^3 benchmark workload (sum of 32 Gaussians + 1e-3 * range
-noise, seed 2, er = 1e-4), as sha256 digests plus the iteration counters.
+"""Writes a full-size digest fixture: the oracle's corrected field and edit
+counts for one large workload, as sha256 digests plus the iteration counters.
 
-The oracle needs about 50 minutes on 6 threads for this field, so the result
-is committed (gauss32_256_digest.json) and the GPU test compares digests.
+Workloads:
+  256  sum of 32 Gaussians + 1e-3 * range noise, 256^3, seed 2, er = 1e-4
+       (gauss32_256_digest.json; about 50 minutes on 6 threads)
+  512  Gaussian random field with power spectrum k^-3, 512^3, seed 3,
+       normalised to [0, 1], xi = 1e-3 (grf_512_digest.json)
 
-    python tests/golden/make_full_digest.py [threads]
+The oracle takes tens of minutes to hours for these fields, so the results
+are committed and the GPU tests compare digests.
+
+    python tests/golden/make_full_digest.py [256|512] [threads]
 """
 import hashlib
 import json
@@ -19,23 +24,35 @@ sys.path.insert(0, ROOT)
 from extrem_z_amd import synth  # noqa: E402
 from oracle import oracle  # noqa: E402
 
-SHAPE = (256, 256, 256)
-RECIPE = dict(kind="gauss", seed=2, er=1e-4, n_gauss=32)
+WORKLOADS = {
+    "256": dict(shape=(256, 256, 256), recipe=dict(kind="gauss", seed=2, er=1e-4, n_gauss=32),
+                out="gauss32_256_digest.json"),
+    "512": dict(shape=(512, 512, 512), recipe=dict(kind="grf", seed=3, xi=1e-3),
+                out="grf_512_digest.json"),
+}
+
+
+def make(name):
+    w = WORKLOADS[name]
+    r = dict(w["recipe"])
+    kind = r.pop("kind")
+    return synth.make_case(kind, w["shape"], **r)
 
 
 def main():
-    threads = int(sys.argv[1]) if len(sys.argv) > 1 else 6
-    o, d, xi = synth.make_case(RECIPE["kind"], SHAPE, seed=RECIPE["seed"], er=RECIPE["er"],
-                               n_gauss=RECIPE["n_gauss"])
+    name = sys.argv[1] if len(sys.argv) > 1 else "256"
+    threads = int(sys.argv[2]) if len(sys.argv) > 2 else 6
+    w = WORKLOADS[name]
+    o, d, xi = make(name)
     t = time.time()
     fixed, counts, stats = oracle.correct(o, d, xi, threads=threads)
     out = {
-        "shape": list(SHAPE), "recipe": RECIPE, "xi": xi,
+        "shape": list(w["shape"]), "recipe": w["recipe"], "xi": xi,
         "sha256_fixed": hashlib.sha256(fixed.tobytes()).hexdigest(),
         "sha256_counts": hashlib.sha256(counts.tobytes()).hexdigest(),
         "stats": stats, "oracle_seconds": round(time.time() - t, 1), "oracle_threads": threads,
     }
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gauss32_256_digest.json")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), w["out"])
     with open(path, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out))

@@ -199,3 +199,26 @@ def test_incremental_passes_equal_full_passes(shape, seed, kind, er, monkeypatch
     if shape[0] * shape[1] * shape[2] <= 40 ** 3:
         f0, c0, _ = oracle.correct(orig, decp, xi)
         assert f0.tobytes() == f1.tobytes()
+
+
+@pytest.mark.parametrize("shape,seed,kind", [
+    ((130, 9, 11), 51, "noise"),   # crosses the 64-wide x tile edges
+    ((65, 5, 9), 52, "gauss"),     # one voxel past a tile in x, y and z
+    ((64, 4, 4), 53, "noise"),     # exactly one tile
+    ((48, 40, 1), 54, "gauss"),
+    ((3, 70, 6), 55, "noise"),
+    ((2, 9, 13), 56, "noise"),     # W < 3: the tile pass's general path
+])
+def test_label_pass_classification_equals_k_classify(shape, seed, kind):
+    """The original field's types and lower masks come out of the label tile
+    pass (its fast interior path and LUT lookup); extremz_classify runs
+    k_classify.  Both, and the labels, against the oracle."""
+    orig, decp, xi = make_field(shape, seed, kind, 1e-2)
+    for f in (orig, decp):
+        t, m, ds, asc = ez().label_pass(f)
+        tc, mc = ez().classify(f)
+        t0, m0 = oracle.classify(f)
+        assert np.array_equal(t, tc) and np.array_equal(m, mc)
+        assert np.array_equal(t, t0) and np.array_equal(m, m0)
+        a0, b0 = oracle.labels(f)
+        assert np.array_equal(ds, a0) and np.array_equal(asc, b0)

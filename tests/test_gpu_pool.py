@@ -1,6 +1,7 @@
 """Batched and served corrections (extremz_pool): every job's result equals the
 single-field correction byte for byte, alone or in the mix, under any lane
 count and memory budget; the budget bounds the pool's device footprint."""
+import ctypes
 import threading
 import time
 
@@ -177,3 +178,42 @@ def test_failing_job_does_not_disturb_the_batch(cases):
         j, kp = pool.host_job(cases[4][0], cases[4][1].copy(), cases[4][2])
         t = pool.submit(j, keep=kp)
         assert pool.wait(t).status == 0 and kp[1].tobytes() == cases[4][3].tobytes()
+
+
+def test_two_waiters_on_one_ticket(cases):
+    """Two threads wait for the same ticket: one receives the status, the
+    other a clean error (the job record is not read after it is freed)."""
+    orig, decp, xi, fixed, _ = cases[6]
+    with ez().Pool(0, lanes=1) as pool:
+        for rep in range(3):
+            j, kp = pool.host_job(orig, decp.copy(), xi)
+            t = pool.submit(j, keep=kp)
+            got = []
+
+            def waiter():
+                rc = ez().lib().extremz_serve_wait(pool.handle, t, -1)
+                got.append(rc)
+
+            ths = [threading.Thread(target=waiter) for _ in range(2)]
+            for th in ths:
+                th.start()
+            for th in ths:
+                th.join(120)
+            assert sorted(got) == [-22, 0], got
+            assert kp[1].tobytes() == fixed.tobytes()
+            pool._held.pop(t, None)
+        # a waiter on one ticket beside a wait_any reaper
+        j, kp = pool.host_job(orig, decp.copy(), xi)
+        t = pool.submit(j, keep=kp)
+        res = {}
+
+        def one():
+            res["one"] = ez().lib().extremz_serve_wait(pool.handle, t, -1)
+
+        th = threading.Thread(target=one)
+        th.start()
+        tk = ctypes.c_int64(0)
+        res["any"] = ez().lib().extremz_serve_wait_any(pool.handle, 60000, ctypes.byref(tk))
+        th.join(120)
+        assert sorted([res["one"], res["any"]]) == [-22, 0], res
+        pool._held.pop(t, None)
