@@ -213,7 +213,11 @@ def test_two_waiters_on_one_ticket(cases):
         th = threading.Thread(target=one)
         th.start()
         tk = ctypes.c_int64(0)
-        res["any"] = ez().lib().extremz_serve_wait_any(pool.handle, 60000, ctypes.byref(tk))
+        res["any"] = ez().lib().extremz_serve_wait_any(pool.handle, 20000, ctypes.byref(tk))
         th.join(120)
-        assert sorted([res["one"], res["any"]]) == [-22, 0], res
+        # exactly one receives the status; the other finds the ticket gone
+        # (-22) or, for wait_any, no unclaimed finished job (-11 at its time-out)
+        assert sorted([res["one"], res["any"]]) in ([-22, 0], [-11, 0]), res
+        if res["any"] == 0:
+            assert tk.value == t
         pool._held.pop(t, None)
