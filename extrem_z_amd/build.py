@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libextremz.so")
-SOURCES = ["engine.hip", "kernels.hip", "pool.hip", "cost.hip"]
+SOURCES = ["engine.hip", "kernels.hip", "pool.hip", "cost.hip", "encode_noalloc.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -165,6 +165,22 @@ int extremz_encode(const double* orig, const double* decp_before,
                    double xi, uint8_t* index_bitmap, double* edits,
                    uint64_t* mantissas, int32_t* exponents, int64_t* n_edits);
 
+/* No counterpart in the reference: extremz_encode for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same outputs as extremz_encode, on
+ * device pointers, queued on ctx's stream; it returns without waiting (read
+ * *d_n_edits, an int64 in device memory, after synchronising the stream).
+ * It allocates nothing: its only scratch is a static device array of
+ * per-block edit counts (4 MiB on each device, held until the process ends)
+ * shared by every call on a device, so n is at most
+ * EXTREMZ_ENCODE_NOALLOC_MAX_N (else EXTREMZ_EINVAL) and calls that overlap
+ * on one device race.  The edit arrays (nullable) need room for n entries. */
+#define EXTREMZ_ENCODE_NOALLOC_MAX_N ((int64_t)1 << 30)
+int extremz_encode_noalloc(extremz_ctx* ctx, const double* d_orig, const double* d_decp_before,
+                           const double* d_decp_after, const uint8_t* d_counts, int64_t n,
+                           double xi, uint8_t* d_index_bitmap, double* d_edits,
+                           uint64_t* d_mantissas, int32_t* d_exponents, int64_t* d_n_edits);
+
 /* The edit container and statistics of cost() (src/ExtreM.cpp:272-389):
  * the 3-bit index bitmap and the snapped edits of extremz_encode, each
  * compressed to one zstd frame (the reference runs `zstd -f` on the two
