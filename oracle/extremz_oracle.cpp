@@ -264,6 +264,34 @@ struct Engine {
   Stats st{};
   bool failed = false;
 
+  // Test instrumentation (tests/test_known_answers.py).  trace, when set,
+  // receives every value apply_delta (or the pre-snap) writes, tagged with
+  // the loop that proposed it.  mutation != 0 plants one plausible mistake
+  // in one rule, so a known-answer test can show that it tells the rule
+  // from that alternative; the correction itself always runs with 0.
+  struct Edit {
+    int phase, vertex;
+    double before, after;
+  };
+  enum Phase { P_PRESNAP = 0, P_C_LOOPS, P_R_LOOPS, P_S_LOOPS, P_S_LOOPS_JOIN, P_SADDLE,
+               P_SADDLE_JOIN };
+  enum Mutation {
+    M_NONE = 0,
+    M_WN_ANY_LABEL,      // wrong neighbours: label compared with any original label
+    M_WN_ALL,            // wrong neighbours: every mismatch flagged, no return
+    M_WN_DESC_ORDER,     // wrong neighbours: saddles in descending id order
+    M_FIXPATH_TRUE,      // fixpath direction 0 lowers the true (original) index
+    M_FIXPATH_FALSE,     // fixpath direction 1 lowers the false (decp) index
+    M_WI_LAST_WINS,      // wrong index max/min: the last saddle per extremum wins
+    M_SLOOPS_SWAP,       // s_loops / s_loops_join: the two fix directions swapped
+    M_SADDLE_NO_GUARD,   // fix_wrong_index_saddle without its early return
+    M_SADDLE_JOIN_LOWER  // fix_wrong_index_saddle direction 1 lowers the true index
+  };
+  std::vector<Edit>* trace = nullptr;
+  int phase = P_PRESNAP;
+  int mutation = M_NONE;
+  bool apply_enabled = true;  // false: apply_delta leaves decp as it is (rule probes)
+
   void classify_all(const double* f, uint8_t* types, uint16_t* ls) {
     // classifyAllVertices, src/ExtreM.cpp:1088-1097
 #pragma omp parallel for schedule(static)
@@ -359,10 +387,12 @@ struct Engine {
   // rule, applyDeltaBuffer's edit_type 0 branch (src/ExtreM_cuda.cu:3065-3076)
   // with threshold q.
   void apply_delta() {
+    if (!apply_enabled) return;
     const double sentinel = -4.0 * bound;
     for (int64_t t = 0; t < g.N; t++) {
       double dl = delta[t];
       if (dl == sentinel) continue;
+      if (trace) trace->push_back({phase, (int)t, decp[t], 0.0});
       if (step_q) {
         if (counter[t] < step_q - 1 && std::abs(orig[t] - (decp[t] + dl)) <= bound) {
           decp[t] += dl;
@@ -371,6 +401,7 @@ struct Engine {
           counter[t] = (uint8_t)step_q;
           decp[t] = orig[t] - bound;
         }
+        if (trace) trace->back().after = decp[t];
         continue;
       }
       if (std::abs(dl) > 1e-15 && counter[t] < 5 &&
@@ -381,11 +412,13 @@ struct Engine {
         decp[t] = orig[t] - bound;
         counter[t] = 6;
       }
+      if (trace) trace->back().after = decp[t];
     }
   }
 
   // c_loops, src/ExtreM.cpp:1505-1583
   void c_loops() {
+    phase = P_C_LOOPS;
     st.c_loops_calls++;
     classify_all(decp, dvt.data(), dlstar.data());
     get_false_critical_points();
@@ -570,32 +603,44 @@ struct Engine {
     wn_idx.clear();
     wn_ds_idx.clear();
     int nb[kNbr];
-    for (const Triplet& t : ocp.trip2) {
+    // mismatch of the c-th side label (positional, label_count); with
+    // M_WN_ANY_LABEL, a label found nowhere among the original ones
+    auto differs = [&](const std::vector<int>& map, int s, int c, int l) {
+      if (mutation != M_WN_ANY_LABEL) return l != map[(size_t)s * kNbr + c];
+      for (int k = 0; k < kNbr; k++)
+        if (map[(size_t)s * kNbr + k] == l) return false;
+      return true;
+    };
+    const bool all = mutation == M_WN_ALL;
+    const int n2 = (int)ocp.trip2.size(), n1 = (int)ocp.trip1.size();
+    for (int q = 0; q < n2; q++) {
+      const Triplet& t = ocp.trip2[mutation == M_WN_DESC_ORDER ? n2 - 1 - q : q];
       int s = t.saddle, c = 0, m = g.nbrs(s, nb);
       for (int k = 0; k < m; k++) {
         int n = nb[k];
         if (!islarger(n, s, decp)) continue;
         int l = compute_max_label(n, decp);
-        if (l != or_max_map[(size_t)s * kNbr + c] && !wn[n]) {
+        if (differs(or_max_map, s, c, l) && !wn[n]) {
           wn[n] = 1;
           wn_idx.push_back(n);
           nfc++;
-          break;
+          if (!all) break;
         }
         c++;
       }
     }
-    for (const Triplet& t : ocp.trip1) {
+    for (int q = 0; q < n1; q++) {
+      const Triplet& t = ocp.trip1[mutation == M_WN_DESC_ORDER ? n1 - 1 - q : q];
       int s = t.saddle, c = 0, m = g.nbrs(s, nb);
       for (int k = 0; k < m; k++) {
         int n = nb[k];
         if (!isless(n, s, decp)) continue;
         int l = compute_min_label(n, decp);
-        if (l != or_min_map[(size_t)s * kNbr + c] && !wn_ds[n]) {
+        if (differs(or_min_map, s, c, l) && !wn_ds[n]) {
           wn_ds[n] = 1;
           wn_ds_idx.push_back(n);
           nfc1++;
-          break;
+          if (!all) break;
         }
         c++;
       }
@@ -628,6 +673,8 @@ struct Engine {
       cur = nxt;
     }
     if (false_index == true_index) return;
+    if (mutation == M_FIXPATH_TRUE && direction == 0) std::swap(false_index, true_index);
+    if (mutation == M_FIXPATH_FALSE && direction == 1) std::swap(false_index, true_index);
     if (direction == 0)
       propose(false_index, lower_mid(false_index));
     else
@@ -636,11 +683,18 @@ struct Engine {
 
   // r_loops, src/ExtreM.cpp:2577-2593
   void r_loops() {
+    phase = P_R_LOOPS;
     st.r_loop_calls++;
     init_delta();
     for (int id : wn_idx) fixpath(id, 0);
     for (int id : wn_ds_idx) fixpath(id, 1);
     apply_delta();
+  }
+
+  // M_WI_LAST_WINS: a later saddle replaces the pair recorded for extremum id
+  static void overwrite(std::vector<int>& pairs, int id, int false_id) {
+    for (size_t k = 0; k + 1 < pairs.size(); k += 2)
+      if (pairs[k] == id) pairs[k + 1] = false_id;
   }
 
   // get_wrong_index_max, src/ExtreM.cpp:2595-2630 (D11)
@@ -660,6 +714,8 @@ struct Engine {
           wrmi.push_back(d.first());
           flag_a[id] = 1;
           wmc++;
+        } else if (id != NONE && mutation == M_WI_LAST_WINS) {
+          overwrite(wrmi, id, d.first());
         }
       }
       if (o.last() != d.last()) {
@@ -669,6 +725,8 @@ struct Engine {
           wrmi2.push_back(d.last());
           flag_b[id] = 1;
           wmc2++;
+        } else if (id != NONE && mutation == M_WI_LAST_WINS) {
+          overwrite(wrmi2, id, d.last());
         }
       }
     }
@@ -691,6 +749,8 @@ struct Engine {
           wrmini2.push_back(d.first());
           flag_b[id] = 1;
           wminc2++;
+        } else if (id != NONE && mutation == M_WI_LAST_WINS) {
+          overwrite(wrmini2, id, d.first());
         }
       }
       if (o.last() != d.last()) {
@@ -700,6 +760,8 @@ struct Engine {
           wrmini.push_back(d.last());
           flag_a[id] = 1;
           wminc++;
+        } else if (id != NONE && mutation == M_WI_LAST_WINS) {
+          overwrite(wrmini, id, d.last());
         }
       }
     }
@@ -732,40 +794,53 @@ struct Engine {
   // fix_wrong_index_saddle, src/ExtreM.cpp:2811-2836 (D13)
   void fix_wrong_index_saddle(int t, int f, int direction) {
     if (t == NONE || f == NONE) return;
-    if (decp[t] > decp[f] || (decp[t] == decp[f] && t > f)) return;
+    if (mutation != M_SADDLE_NO_GUARD && (decp[t] > decp[f] || (decp[t] == decp[f] && t > f)))
+      return;
     if (direction == 0)
       propose(f, lower_mid(f));
+    else if (mutation == M_SADDLE_JOIN_LOWER)
+      propose(t, lower_mid(t));
     else
       propose(t, raise_mid(t));
   }
 
   // s_loops, src/ExtreM.cpp:2838-2862
   void s_loops() {
+    phase = P_S_LOOPS;
     st.s_loop_calls++;
     get_wrong_index_max();
     init_delta();
-    for (int i = 0; i < wmc; i++) propose(wrmi[2 * i], lower_mid(wrmi[2 * i]));
-    for (int i = 0; i < wmc2; i++) {
-      int f = wrmi2[2 * i + 1];
-      if (f != NONE) propose(f, lower_mid(f));
+    const int a = mutation == M_SLOOPS_SWAP ? 1 : 0;  // dir 0: the true index
+    for (int i = 0; i < wmc; i++) {
+      int v = wrmi[2 * i + a];
+      if (v != NONE) propose(v, lower_mid(v));
+    }
+    for (int i = 0; i < wmc2; i++) {  // dir 1: the false index
+      int v = wrmi2[2 * i + 1 - a];
+      if (v != NONE) propose(v, lower_mid(v));
     }
     apply_delta();
   }
   // s_loops_join, src/ExtreM.cpp:2864-2886
   void s_loops_join() {
+    phase = P_S_LOOPS_JOIN;
     st.s_loop_calls++;
     get_wrong_index_min();
     init_delta();
-    for (int i = 0; i < wminc; i++)
-      propose(wrmini[2 * i], lower_mid(wrmini[2 * i]));
-    for (int i = 0; i < wminc2; i++) {
-      int f = wrmini2[2 * i + 1];
-      if (f != NONE) propose(f, lower_mid(f));
+    const int a = mutation == M_SLOOPS_SWAP ? 1 : 0;  // dir 0: the true index
+    for (int i = 0; i < wminc; i++) {
+      int v = wrmini[2 * i + a];
+      if (v != NONE) propose(v, lower_mid(v));
+    }
+    for (int i = 0; i < wminc2; i++) {  // dir 1: the false index
+      int v = wrmini2[2 * i + 1 - a];
+      if (v != NONE) propose(v, lower_mid(v));
     }
     apply_delta();
   }
   // saddle_loops / saddle_loops_join, src/ExtreM.cpp:2888-2916
   void saddle_loops() {
+    phase = P_SADDLE;
     st.saddle_loop_calls++;
     wsc = wrong_index_saddles(dcp.maximum, dcp.largest, ocp.largest,
                               dcp.saddles2, ocp.saddles2, wrsi);
@@ -774,6 +849,7 @@ struct Engine {
     apply_delta();
   }
   void saddle_loops_join() {
+    phase = P_SADDLE_JOIN;
     st.saddle_loop_calls++;
     wscj = wrong_index_saddles(dcp.minimum, dcp.smallest, ocp.smallest,
                                dcp.saddles1, ocp.saddles1, wrsji);
@@ -823,7 +899,9 @@ struct Engine {
       if (isless(i, gmin, orig)) gmin = i;
       if (islarger(i, gmax, orig)) gmax = i;
     }
+    if (trace) trace->push_back({P_PRESNAP, gmin, decp[gmin], orig[gmin] - bound});
     decp[gmin] = orig[gmin] - bound;
+    if (trace) trace->push_back({P_PRESNAP, gmax, decp[gmax], orig[gmax] - bound});
     decp[gmax] = orig[gmax] - bound;
 
     // main.cpp:85-89
@@ -845,8 +923,14 @@ struct Engine {
     sort_by_saddle(ocp.trip1);
 
     // main.cpp:122-131 (preserve_join_tree = true, main.cpp:21)
-    for (int s : ocp.saddles2) decp[s] = orig[s] - bound;
-    for (int s : ocp.saddles1) decp[s] = orig[s] - bound;
+    for (int s : ocp.saddles2) {
+      if (trace) trace->push_back({P_PRESNAP, s, decp[s], orig[s] - bound});
+      decp[s] = orig[s] - bound;
+    }
+    for (int s : ocp.saddles1) {
+      if (trace) trace->push_back({P_PRESNAP, s, decp[s], orig[s] - bound});
+      decp[s] = orig[s] - bound;
+    }
 
     c_loops();  // main.cpp:139
     if (failed) return fail();
@@ -1160,6 +1244,149 @@ int oracle_correct(const double* orig, double* decp, int W, int H, int D,
 // rules, q > 0 for the GPU fork's fixed step -xi/q and threshold q
 // (edit_type 0: src/ExtreM_cuda.cu:2905 and the other `-bound / (q)` sites,
 // applyDeltaBuffer :3065-3076).
+// One rule of the outer loop on a given state, for known-answer tests: the
+// original field's and decp's classification, labels, CP lists, triplets and
+// extremum -> saddle maps are computed as the loop has them (decp with its
+// own types), then rule r runs once:
+//   0 get_wrong_split/join_neighbors   out: flagged split ids, then join ids
+//   1 r_loops' fixpath proposals        out: (vertex, delta) pairs
+//   2 get_wrong_index_max + s_loops     out: pairs (wmc), pairs (wmc2), proposals
+//   3 get_wrong_index_min + s_loops_join  (as 2)
+//   4 get_wrong_index_saddles + saddle_loops' fixes (split)  out: pairs, proposals
+//   5 the join versions of 4
+// Integer outputs go to iout (counts first: how many of each list), proposals
+// to (pv, pd).  Nothing is applied.  Returns 0, or -22.
+int oracle_rule_probe(const double* orig, double* decp, int W, int H, int D, double xi, int r,
+                      int mutation, int32_t* iout, int icap, int32_t* pv, double* pd,
+                      int pcap, int32_t* np) {
+  if (W <= 0 || H <= 0 || D <= 0 || !(xi > 0) || r < 0 || r > 5) return -22;
+  Engine e;
+  e.g = Grid{W, H, D, (int64_t)W * H * D};
+  e.orig = orig;
+  e.decp = decp;
+  e.bound = xi;
+  e.mutation = mutation;
+  const int64_t N = e.g.N;
+  e.delta.assign(N, -4.0 * xi);
+  e.counter.assign(N, 0);
+  e.vt.assign(N, 5);
+  e.dvt.assign(N, 5);
+  e.lstar.assign(N, 0);
+  e.dlstar.assign(N, 0);
+  e.wn.assign(N, 0);
+  e.wn_ds.assign(N, 0);
+  e.flag_a.assign(N, 0);
+  e.flag_b.assign(N, 0);
+  e.classify_all(orig, e.vt.data(), e.lstar.data());
+  e.classify_all(decp, e.dvt.data(), e.dlstar.data());
+  e.manifold(orig, e.DS, true);
+  e.manifold(orig, e.AS, false);
+  e.compute_cp(e.ocp, orig, e.DS, e.AS, e.vt.data());
+  e.nMax = (int)e.ocp.maximum.size();
+  e.nMin = (int)e.ocp.minimum.size();
+  Engine::extremum_saddle_map(e.nMax, e.ocp.trip2, e.ocp.largest, e.ocp.maximum);
+  Engine::extremum_saddle_map(e.nMin, e.ocp.trip1, e.ocp.smallest, e.ocp.minimum);
+  Engine::sort_by_saddle(e.ocp.trip2);
+  Engine::sort_by_saddle(e.ocp.trip1);
+  e.compute_saddle_maps();
+  e.dec_cp_block();
+  std::vector<int> ints;
+  auto list = [&](const std::vector<int>& v, int n) {
+    ints.push_back(n);
+    ints.insert(ints.end(), v.begin(), v.begin() + std::min<size_t>(v.size(), 2 * (size_t)n));
+  };
+  e.init_delta();
+  switch (r) {
+    case 0:
+    case 1:
+      e.wrong_neighbors();
+      ints.push_back(e.nfc);
+      ints.insert(ints.end(), e.wn_idx.begin(), e.wn_idx.end());
+      ints.push_back(e.nfc1);
+      ints.insert(ints.end(), e.wn_ds_idx.begin(), e.wn_ds_idx.end());
+      if (r == 1) {
+        for (int id : e.wn_idx) e.fixpath(id, 0);
+        for (int id : e.wn_ds_idx) e.fixpath(id, 1);
+      }
+      break;
+    case 2:
+    case 3: {
+      // s_loops / s_loops_join without their apply_delta
+      e.apply_enabled = false;
+      if (r == 2) {
+        e.s_loops();
+        list(e.wrmi, e.wmc);
+        list(e.wrmi2, e.wmc2);
+      } else {
+        e.s_loops_join();
+        list(e.wrmini, e.wminc);
+        list(e.wrmini2, e.wminc2);
+      }
+      break;
+    }
+    case 4:
+    case 5:
+      e.apply_enabled = false;
+      if (r == 4) {
+        e.saddle_loops();
+        list(e.wrsi, e.wsc);
+      } else {
+        e.saddle_loops_join();
+        list(e.wrsji, e.wscj);
+      }
+      break;
+  }
+  if ((int)ints.size() > icap) return -22;
+  std::copy(ints.begin(), ints.end(), iout);
+  int k = 0;
+  const double sent = -4.0 * xi;
+  for (int64_t t = 0; t < N; t++)
+    if (e.delta[t] != sent) {
+      if (k >= pcap) return -22;
+      pv[k] = (int32_t)t;
+      pd[k] = e.delta[t];
+      k++;
+    }
+  *np = k;
+  return 0;
+}
+
+// The correction with its edit trace (phase, vertex, before, after per
+// written value, at most cap entries; returns the number of edits, or -1 -
+// needed when cap is too small) and one planted mistake (mutation, 0 = none).
+int oracle_correct_trace(const double* orig, double* decp, int W, int H, int D, double xi,
+                         int mutation, uint8_t* counts, int64_t* stats, int32_t* phase,
+                         int32_t* vertex, double* before, double* after, int cap) {
+  if (W <= 0 || H <= 0 || D <= 0 || !(xi > 0)) return -22;
+  Engine e;
+  std::vector<Engine::Edit> tr;
+  e.g = Grid{W, H, D, (int64_t)W * H * D};
+  e.orig = orig;
+  e.decp = decp;
+  e.bound = xi;
+  e.max_c_iters = 1000;
+  e.max_outer = 1000;
+  e.trace = &tr;
+  e.mutation = mutation;
+  int rc = e.run();
+  if (counts) std::memcpy(counts, e.counter.data(), e.counter.size());
+  if (stats) {
+    const Stats& s = e.st;
+    int64_t v[12] = {rc, s.c_loop_iters, s.outer_iters, s.r_loop_calls,
+                     s.s_loop_calls, s.saddle_loop_calls, s.c_loops_calls,
+                     s.edited, s.n_max, s.n_min, s.n_saddle1, s.n_saddle2};
+    std::memcpy(stats, v, sizeof(v));
+  }
+  if ((int)tr.size() > cap) return -1 - (int)tr.size();
+  for (size_t i = 0; i < tr.size(); i++) {
+    phase[i] = tr[i].phase;
+    vertex[i] = tr[i].vertex;
+    before[i] = tr[i].before;
+    after[i] = tr[i].after;
+  }
+  return (int)tr.size();
+}
+
 int oracle_correct_rule(const double* orig, double* decp, int W, int H, int D, double xi,
                         int max_c_iters, int max_outer, uint8_t* counts, int64_t* stats,
                         int threads, int step_q) {

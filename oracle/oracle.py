@@ -151,3 +151,88 @@ def split_tree(field):
     return {"max": mx[:n].copy(), "pair_saddle": ps[:n].copy(), "parent": pa[:n].copy(),
             "branch": [bv[off[i]:off[i + 1]].tolist() for i in range(n)],
             "rounds": rounds.value}
+
+
+PHASES = ("presnap", "c_loops", "r_loops", "s_loops", "s_loops_join", "saddle_loops",
+          "saddle_loops_join")
+MUTATIONS = ("none", "wn_any_label", "wn_all", "wn_desc_order", "fixpath_true",
+             "fixpath_false", "wi_last_wins", "sloops_swap", "saddle_no_guard",
+             "saddle_join_lower")
+
+
+def correct_trace(orig, decp, xi, mutation="none"):
+    """The correction with its edit trace: (fixed, counts, stats, edits) where
+    edits lists (phase name, vertex, value before, value after) for every
+    value the pre-snap or an apply_delta wrote, in order.  mutation plants
+    one of MUTATIONS (a plausible mistake in one rule) for known-answer tests."""
+    L = lib()
+    if not hasattr(L.oracle_correct_trace, "_set"):
+        vp, i32, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.oracle_correct_trace.argtypes = [vp, vp, i32, i32, i32, dbl, i32, vp, vp, vp, vp, vp,
+                                           vp, i32]
+        L.oracle_correct_trace._set = True
+    o = np.ascontiguousarray(orig, dtype=np.float64)
+    d = np.array(decp, dtype=np.float64, order="C", copy=True)
+    counts = np.zeros(o.shape, np.uint8)
+    st = np.zeros(12, np.int64)
+    cap = 100000
+    ph = np.zeros(cap, np.int32)
+    vx = np.zeros(cap, np.int32)
+    bf = np.zeros(cap, np.float64)
+    af = np.zeros(cap, np.float64)
+    n = L.oracle_correct_trace(_p(o), _p(d), *_dims(o), float(xi), MUTATIONS.index(mutation),
+                               _p(counts), _p(st), _p(ph), _p(vx), _p(bf), _p(af), cap)
+    if n < 0:
+        raise ValueError(f"oracle_correct_trace failed ({n})")
+    edits = [(PHASES[ph[i]], int(vx[i]), float(bf[i]), float(af[i])) for i in range(n)]
+    return d, counts, dict(zip(STAT_KEYS, (int(x) for x in st))), edits
+
+
+RULES = ("wrong_neighbors", "fixpath", "wrong_index_max", "wrong_index_min", "saddles",
+         "saddles_join")
+
+
+def rule_probe(orig, decp, xi, rule, mutation="none"):
+    """One rule of the outer loop on the state (orig, decp), nothing applied.
+
+    Returns a dict: wrong_neighbors / fixpath: 'split' and 'join' flagged
+    vertex lists; wrong_index_max / _min: 'pairs_0' (direction 0: the first
+    ([0]) comparison for maxima, the last for minima) and 'pairs_1' as
+    (true, false) tuples; saddles / saddles_join: 'pairs'.  Every rule but
+    wrong_neighbors also returns 'proposals', {vertex: delta} of the delta
+    buffer before apply_delta."""
+    L = lib()
+    if not hasattr(L.oracle_rule_probe, "_set"):
+        vp, i32, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.oracle_rule_probe.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, vp, i32, vp, vp,
+                                        i32, vp]
+        L.oracle_rule_probe._set = True
+    o = np.ascontiguousarray(orig, dtype=np.float64)
+    d = np.array(decp, dtype=np.float64, order="C", copy=True)
+    cap = 4 * o.size + 16
+    iout = np.zeros(cap, np.int32)
+    pv = np.zeros(o.size, np.int32)
+    pd = np.zeros(o.size, np.float64)
+    npr = ctypes.c_int(0)
+    r = RULES.index(rule)
+    rc = L.oracle_rule_probe(_p(o), _p(d), *_dims(o), float(xi), r, MUTATIONS.index(mutation),
+                             _p(iout), cap, _p(pv), _p(pd), o.size, ctypes.byref(npr))
+    if rc:
+        raise ValueError(f"oracle_rule_probe failed ({rc})")
+    out = {}
+    if r in (0, 1):
+        n = int(iout[0])
+        out["split"] = iout[1:1 + n].tolist()
+        m = int(iout[1 + n])
+        out["join"] = iout[2 + n:2 + n + m].tolist()
+    else:
+        pos = 0
+        names = ("pairs_0", "pairs_1") if r in (2, 3) else ("pairs",)
+        for nm in names:
+            n = int(iout[pos])
+            flat = iout[pos + 1:pos + 1 + 2 * n].tolist()
+            out[nm] = [(flat[2 * i], flat[2 * i + 1]) for i in range(n)]
+            pos += 1 + 2 * n
+    if r != 0:
+        out["proposals"] = {int(pv[i]): float(pd[i]) for i in range(npr.value)}
+    return out
