@@ -1277,6 +1277,10 @@ int oracle_rule_probe(const double* orig, double* decp, int W, int H, int D, dou
   e.wn_ds.assign(N, 0);
   e.flag_a.assign(N, 0);
   e.flag_b.assign(N, 0);
+#ifdef _OPENMP
+  const int threads = omp_get_max_threads();
+  omp_set_num_threads(1);
+#endif
   e.classify_all(orig, e.vt.data(), e.lstar.data());
   e.classify_all(decp, e.dvt.data(), e.dlstar.data());
   e.manifold(orig, e.DS, true);
@@ -1336,6 +1340,9 @@ int oracle_rule_probe(const double* orig, double* decp, int W, int H, int D, dou
       }
       break;
   }
+#ifdef _OPENMP
+  omp_set_num_threads(threads);
+#endif
   if ((int)ints.size() > icap) return -22;
   std::copy(ints.begin(), ints.end(), iout);
   int k = 0;
@@ -1364,11 +1371,20 @@ int oracle_correct_trace(const double* orig, double* decp, int W, int H, int D, 
   e.orig = orig;
   e.decp = decp;
   e.bound = xi;
-  e.max_c_iters = 1000;
-  e.max_outer = 1000;
+  // the known-answer fields are tiny: one thread, and a planted mistake that
+  // never converges stops early
+  e.max_c_iters = 200;
+  e.max_outer = 40;
   e.trace = &tr;
   e.mutation = mutation;
+#ifdef _OPENMP
+  const int threads = omp_get_max_threads();
+  omp_set_num_threads(1);
+#endif
   int rc = e.run();
+#ifdef _OPENMP
+  omp_set_num_threads(threads);
+#endif
   if (counts) std::memcpy(counts, e.counter.data(), e.counter.size());
   if (stats) {
     const Stats& s = e.st;
