@@ -2,7 +2,7 @@
 //
 //   extremz_cli <path,W,H,D> <er> <compressor_id> [--decp decp.bin]
 //               [--orig-f32] [--out fixed.bin] [--counts counts.bin]
-//               [--cost prefix] [--cmp-bytes n] [--fixed-step q]
+//               [--cost prefix] [--cmp-bytes n] [--fixed-step q] [--neighbors 6|14]
 //
 // Same positional arguments as main.cpp:13-37: a raw float64 field (getdata,
 // src/ExtreM.cpp:95-118; --orig-f32 for float32), the relative bound er
@@ -17,6 +17,9 @@
 // <prefix>stats.txt; --cmp-bytes is the lossy compressor's output size
 // (SZ3's cmpSize in the reference).  --fixed-step q selects the GPU fork's
 // fixed-step rule set (its edit_type 0 with q, src/ExtreM_cuda.cu:4958-4963).
+// --neighbors 6 selects the CPU path's adjacency, maxNeighbors = 6 as main.cpp:20
+// sets it (+-x and +-y on a field of depth 1); the default is the
+// 14-neighbour stencil.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -59,7 +62,7 @@ int main(int argc, char** argv) {
     std::fprintf(stderr,
                  "usage: %s <path,W,H,D> <er> <compressor_id> [--decp decp.bin] [--orig-f32] "
                  "[--out fixed.bin] [--counts counts.bin] [--cost prefix] [--cmp-bytes n] "
-                 "[--fixed-step q]\n",
+                 "[--fixed-step q] [--neighbors 6|14]\n",
                  argv[0]);
     return 2;
   }
@@ -76,7 +79,7 @@ int main(int argc, char** argv) {
   std::string compressor_id = argv[3];
   std::string decp_path, out_path = "fixed.bin", counts_path, cost_prefix;
   long long cmp_bytes = 0;
-  int fixed_q = 0;
+  int fixed_q = 0, adjacency = EXTREMZ_ADJ_STENCIL14;
   bool f32 = false;
   for (int i = 4; i < argc; i++) {
     std::string a = argv[i];
@@ -87,6 +90,14 @@ int main(int argc, char** argv) {
     else if (a == "--cost" && i + 1 < argc) cost_prefix = argv[++i];
     else if (a == "--cmp-bytes" && i + 1 < argc) cmp_bytes = std::atoll(argv[++i]);
     else if (a == "--fixed-step" && i + 1 < argc) fixed_q = std::atoi(argv[++i]);
+    else if (a == "--neighbors" && i + 1 < argc) {
+      const int m = std::atoi(argv[++i]);
+      if (m != 6 && m != 14) {
+        std::fprintf(stderr, "--neighbors takes 6 or 14\n");
+        return 2;
+      }
+      adjacency = m == 6 ? EXTREMZ_ADJ_PLANE4 : EXTREMZ_ADJ_STENCIL14;
+    }
     else {
       std::fprintf(stderr, "unknown argument %s\n", a.c_str());
       return 2;
@@ -123,7 +134,7 @@ int main(int argc, char** argv) {
   std::vector<double> before = decp;
   std::vector<uint8_t> counts(N);
   extremz_stats st;
-  extremz_opts opts = {0, 0, 0, fixed_q};
+  extremz_opts opts = {0, 0, 0, fixed_q, adjacency};
   int rc = extremz_correct(orig.data(), EXTREMZ_DTYPE_F64, decp.data(), W, H, D, xi, &opts, &st,
                            counts.data());
   if (rc != 0) {

@@ -87,7 +87,13 @@ enum Ctr {
 struct Geo {
   int W, H, D, WH, N;
   int off[kNbr];  // index offset of each stencil direction
+  // 1: the plane adjacency of the reference's CPU path (maxNeighbors = 6 over
+  // its direction table, so +-x and +-y on a depth-1 field), boundary classes
+  // kPlanarCls + (x state) + 4 (y state); 0: the 14-neighbour stencil
+  int planar;
 };
+constexpr int kPlanarCls = 64;   // first boundary class of the plane adjacency
+constexpr int kNumCls = 80;      // boundary classes (64 stencil + 16 plane)
 
 // Everything a kernel needs, passed by value.
 struct St {

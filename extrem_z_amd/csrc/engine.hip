@@ -69,7 +69,7 @@ static int link_component_type(int key, int cfg) {
 struct LutTable {
   std::vector<uint8_t> table;
   int ref_entries = 0;
-  int cls_key[64], cls_off[64];
+  int cls_key[kNumCls], cls_off[kNumCls];
   LutTable() {
     std::vector<int> key_off(1 << kNbr, -1);
     auto add = [&](int key) {
@@ -95,6 +95,19 @@ struct LutTable {
       if (key_off[key] < 0) add(key);  // axes of length 1: extension classes
       cls_key[cls] = key;
       cls_off[cls] = key_off[key];
+    }
+    // the plane adjacency of the CPU path (computeAdjacency with
+    // maxNeighbors = 6, src/ExtreM.cpp:543-575, on a depth-1 field): the
+    // existing +-x / +-y directions of the depth-1 class.  Its link rule,
+    // two neighbours joined when they share a triangle with the vertex
+    // (classifyVertex1, src/ExtreM.cpp:760-818, over the triangles of
+    // src/ExtreM.cpp:597-627), joins +x with +y and -x with -y: the
+    // stencil's own rule restricted to these four directions.
+    for (int c = 0; c < kNumCls - kPlanarCls; c++) {
+      const int key = cls_key[c + 48] & 0xF;
+      if (key_off[key] < 0) add(key);
+      cls_key[kPlanarCls + c] = key;
+      cls_off[kPlanarCls + c] = key_off[key];
     }
   }
 };
@@ -214,6 +227,7 @@ class Run {
     c_->timing = o ? o->timing : 0;
     c_->ev_used = 0;
     step_q_ = o ? o->fixed_step_q : 0;
+    planar_ = o ? o->adjacency : 0;
     const char* e = std::getenv("EXTREMZ_FULL_PASSES");
     inc_off_ = e && *e && *e != '0';
     e = std::getenv("EXTREMZ_VERIFY_INC");
@@ -234,6 +248,7 @@ class Run {
   St s_{};
   int max_c_, max_outer_;
   int step_q_ = 0;
+  int planar_ = 0;  // the CPU path's plane adjacency (Geo::planar)
   int epoch_ = 0;
   bool clean_ = false;
   // the last CP pass of decp was a full one: the next incremental pass resets
@@ -249,6 +264,7 @@ class Run {
   hipStream_t st() const { return c_->stream; }
 
   void setup_geo(int W, int H, int D) {
+    s_.g.planar = planar_;
     s_.g.W = W;
     s_.g.H = H;
     s_.g.D = D;
@@ -1089,6 +1105,10 @@ void Run::split_tree(const double* d_f, int W, int H, int D, TreeOut& out) {
 
 int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
                  uint8_t* d_counts) {
+  if (planar_ && D != 1) {
+    g_err = "the plane adjacency (EXTREMZ_ADJ_PLANE4) needs a field of depth 1";
+    return EXTREMZ_EINVAL;
+  }
   setup_geo(W, H, D);
   const long long N = s_.g.N;
   allocate(N);
@@ -1413,7 +1433,8 @@ static void fill_stats(extremz_ctx* c, const Run& r, int status, double wall_ms,
 }
 
 bool valid_opts(const extremz_opts* o) {
-  return !o || (o->fixed_step_q >= 0 && o->fixed_step_q <= 254);  // counts are uint8
+  return !o || (o->fixed_step_q >= 0 && o->fixed_step_q <= 254 &&  // counts are uint8
+                (o->adjacency == EXTREMZ_ADJ_STENCIL14 || o->adjacency == EXTREMZ_ADJ_PLANE4));
 }
 
 bool valid_shape(int W, int H, int D) {

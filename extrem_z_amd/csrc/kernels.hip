@@ -15,16 +15,16 @@ __constant__ int c_dx[kNbr], c_dy[kNbr], c_dz[kNbr];
 __device__ constexpr int kSX[kNbr] = {1, -1, 0, 0, 0, 0, -1, 1, 0, 0, -1, 1, -1, 1};
 __device__ constexpr int kSY[kNbr] = {0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1};
 __device__ constexpr int kSZ[kNbr] = {0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1};
-__constant__ int c_cls_key[64];
-__constant__ int c_lut_off[64];
+__constant__ int c_cls_key[kNumCls];
+__constant__ int c_lut_off[kNumCls];
 
 void upload_constants(const int* dx, const int* dy, const int* dz,
                       const int* cls_key, const int* lut_off) {
   hip_check(hipMemcpyToSymbol(HIP_SYMBOL(c_dx), dx, sizeof(int) * kNbr));
   hip_check(hipMemcpyToSymbol(HIP_SYMBOL(c_dy), dy, sizeof(int) * kNbr));
   hip_check(hipMemcpyToSymbol(HIP_SYMBOL(c_dz), dz, sizeof(int) * kNbr));
-  hip_check(hipMemcpyToSymbol(HIP_SYMBOL(c_cls_key), cls_key, sizeof(int) * 64));
-  hip_check(hipMemcpyToSymbol(HIP_SYMBOL(c_lut_off), lut_off, sizeof(int) * 64));
+  hip_check(hipMemcpyToSymbol(HIP_SYMBOL(c_cls_key), cls_key, sizeof(int) * kNumCls));
+  hip_check(hipMemcpyToSymbol(HIP_SYMBOL(c_lut_off), lut_off, sizeof(int) * kNumCls));
 }
 
 // ------------------------------------------------------------ helpers ----
@@ -56,6 +56,7 @@ __device__ __forceinline__ int axis_state(int c, int n) {
   return n == 1 ? 3 : (c == 0 ? 1 : (c == n - 1 ? 2 : 0));
 }
 __device__ __forceinline__ int class_of(const Geo& g, int x, int y, int z) {
+  if (g.planar) return kPlanarCls + axis_state(x, g.W) + 4 * axis_state(y, g.H);
   return axis_state(x, g.W) + 4 * axis_state(y, g.H) + 16 * axis_state(z, g.D);
 }
 
@@ -120,8 +121,14 @@ __device__ __forceinline__ int classify_at(const Geo& g, const uint8_t* lut,
     k++;
   }
   *lower_out = lm;
-  if (lc == k) return 4;   // no upper neighbour
-  if (lc == 0) return 0;   // no lower neighbour
+  // the GPU path's shortcuts (src/ExtreM_cuda.cu:3846-3853); the plane
+  // adjacency keeps classifyVertex1's rule, whose link may be disconnected
+  // (src/ExtreM.cpp:880-886: a vertex with two upper link components and no
+  // lower one is a 2-saddle), so its table holds every configuration
+  if (cls < kPlanarCls) {
+    if (lc == k) return 4;   // no upper neighbour
+    if (lc == 0) return 0;   // no lower neighbour
+  }
   return lut[c_lut_off[cls] + cfg];
 }
 
@@ -143,8 +150,10 @@ __device__ __forceinline__ int type_of_mask(const uint8_t* lut, int cls, unsigne
       k++;
     }
   }
-  if (lc == k) return 4;
-  if (lc == 0) return 0;
+  if (cls < kPlanarCls) {  // as classify_at
+    if (lc == k) return 4;
+    if (lc == 0) return 0;
+  }
   return lut[c_lut_off[cls] + cfg];
 }
 

@@ -89,7 +89,17 @@ typedef struct extremz_opts {
                        (edit_type 0, src/ExtreM_cuda.cu:2905, 3065-3076): every
                        proposal is -xi/q, an edit count reaching q - 1 snaps
                        to orig - xi with count q */
+  int adjacency;    /* EXTREMZ_ADJ_STENCIL14 (0): the 14-neighbour stencil of
+                       src/ExtreM_cuda.cu:155-162 for every field;
+                       EXTREMZ_ADJ_PLANE4 (1): the CPU path's plane adjacency,
+                       maxNeighbors = 6 (main.cpp:20) over the direction table
+                       of src/ExtreM.cpp:16-21, i.e. +-x and +-y, with the
+                       triangle links of src/ExtreM.cpp:597-627, 703-891;
+                       fields of depth 1 only (else EXTREMZ_EINVAL) */
 } extremz_opts;
+
+#define EXTREMZ_ADJ_STENCIL14 0
+#define EXTREMZ_ADJ_PLANE4 1
 
 typedef struct extremz_stats {
   int64_t counts[EXTREMZ_STAT_COUNT_];

@@ -147,6 +147,21 @@ struct Lut {
           }
           if (offset[key] < 0) add(key);
         }
+    // the CPU path's plane adjacency: +-x / +-y of each depth-1 class; the
+    // triangle links of classifyVertex1 (src/ExtreM.cpp:760-818) join +x
+    // with +y and -x with -y, which is link_type's rule on these keys
+    for (int ax = 0; ax < 4; ax++)
+      for (int ay = 0; ay < 4; ay++) {
+        int key = 0;
+        for (int d = 0; d < 4; d++) {
+          const int s = kDir[d][0] + kDir[d][1];  // the one non-zero component
+          const int st = kDir[d][0] ? ax : ay;
+          if (s < 0 && !(st == 0 || st == 2)) continue;
+          if (s > 0 && !(st == 0 || st == 1)) continue;
+          key |= 1 << d;
+        }
+        if (offset[key] < 0) add(key);
+      }
   }
   void add(int key) {
     offset[key] = (int)table.size();
@@ -161,9 +176,16 @@ const Lut& lut() {
 }
 
 // --------------------------------------------------------------- grid ----
+// Directions in use for every grid built after oracle_set_adjacency.
+int g_dmask = (1 << kNbr) - 1;
+
 struct Grid {
   int W, H, D;
   int64_t N;
+  // directions in use: the 14-neighbour stencil (D1), or with the CPU path's
+  // plane adjacency (maxNeighbors = 6 over the table of src/ExtreM.cpp:16-21
+  // on a depth-1 field) only +-x and +-y
+  int dmask = g_dmask;
   // Existing neighbours of v in direction order: computeAdjacency,
   // src/ExtreM.cpp:543-575 (with the 14 directions, D1).  Returns the count;
   // dirs[] receives the direction index of each neighbour.
@@ -171,6 +193,7 @@ struct Grid {
     int x = v % W, y = (v / W) % H, z = v / (W * H);
     int m = 0;
     for (int d = 0; d < kNbr; d++) {
+      if (!((dmask >> d) & 1)) continue;
       int nx = x + kDir[d][0], ny = y + kDir[d][1], nz = z + kDir[d][2];
       if (nx < 0 || nx >= W || ny < 0 || ny >= H || nz < 0 || nz >= D)
         continue;
@@ -212,8 +235,14 @@ int classify_vertex(const Grid& g, const double* f, int v, uint16_t* lower) {
     }
   }
   *lower = lm;
-  if (uc == 0) return 4;
-  if (lc == 0) return 0;
+  // The GPU path's shortcuts (src/ExtreM_cuda.cu:3846-3853).  The plane
+  // adjacency keeps classifyVertex1's rule (src/ExtreM.cpp:880-886) for every
+  // configuration: its link may be disconnected, and a vertex with no lower
+  // neighbour but two upper link components is a 2-saddle there.
+  if (g.dmask == (1 << kNbr) - 1) {
+    if (uc == 0) return 4;
+    if (lc == 0) return 0;
+  }
   const Lut& L = lut();
   return L.table[L.offset[key] + cfg];
 }
@@ -1190,6 +1219,13 @@ int oracle_lut(int32_t* out, int cap) {
 
 // Link-component type for one boundary key and upper config.
 int oracle_link_type(int key, int config) { return link_type(key, config); }
+
+// Adjacency of every later call: 0 the 14-neighbour stencil, 1 the CPU
+// path's plane adjacency (+-x, +-y; depth-1 fields).
+int oracle_set_adjacency(int plane) {
+  g_dmask = plane ? 0xF : (1 << kNbr) - 1;
+  return 0;
+}
 
 int oracle_classify(const double* f, int W, int H, int D, uint8_t* types,
                     uint16_t* lower) {

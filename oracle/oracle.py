@@ -42,6 +42,7 @@ def lib():
         L.oracle_bitmap3.argtypes = [vp, i64, vp]
         L.oracle_bitmap3.restype = i64
         L.oracle_cost.argtypes = [vp, vp, vp, vp, i64, dbl, vp, vp, vp]
+        L.oracle_set_adjacency.argtypes = [i32]
         _lib = L
     return _lib
 
@@ -236,3 +237,16 @@ def rule_probe(orig, decp, xi, rule, mutation="none"):
     if r != 0:
         out["proposals"] = {int(pv[i]): float(pd[i]) for i in range(npr.value)}
     return out
+
+
+class plane_adjacency:
+    """Context manager: the oracle uses the CPU path's plane adjacency (+-x,
+    +-y; maxNeighbors = 6 over src/ExtreM.cpp:16-21 on a depth-1 field)
+    inside the block, the 14-neighbour stencil outside it."""
+
+    def __enter__(self):
+        lib().oracle_set_adjacency(1)
+        return self
+
+    def __exit__(self, *a):
+        lib().oracle_set_adjacency(0)

@@ -118,3 +118,27 @@ def test_cli_writes_fixed_field_and_container(tmp_path):
     assert ez().zstd_decompress(open(pre + "index.bin.zst", "rb").read()) == bm.tobytes()
     assert ez().zstd_decompress(open(pre + "edits.bin.zst", "rb").read()) == edits.tobytes()
     assert "OCR: " in open(pre + "stats.txt").read()
+
+
+@pytest.mark.gpu
+def test_cli_plane_adjacency(tmp_path):
+    """--neighbors 6: the CPU path's maxNeighbors = 6 adjacency on a plane
+    field, against the oracle in the same mode."""
+    import subprocess
+    orig, _, _ = make_field((30, 22, 1), 54, "gauss", 1e-3)
+    src = tmp_path / "field.bin"
+    orig.astype("<f8").tofile(src)
+    er = 1e-3
+    xi = (orig.max() - orig.min()) * er
+    from extrem_z_amd import synth
+    decp = synth.quantise(orig, xi)
+    dp = tmp_path / "decp.bin"
+    decp.astype("<f8").tofile(dp)
+    cli = os.path.join(os.path.dirname(ez().LIB_PATH), "extremz_cli")
+    out = tmp_path / "fixed.bin"
+    r = subprocess.run([cli, f"{src},30,22,1", str(er), "sz3", "--decp", str(dp), "--out", str(out),
+                        "--neighbors", "6"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    with oracle.plane_adjacency():
+        fixed, _, _ = oracle.correct(orig, decp, xi)
+    assert np.fromfile(out, "<f8").tobytes() == fixed.ravel().tobytes()

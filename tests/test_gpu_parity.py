@@ -222,3 +222,50 @@ def test_label_pass_classification_equals_k_classify(shape, seed, kind):
         assert np.array_equal(t, t0) and np.array_equal(m, m0)
         a0, b0 = oracle.labels(f)
         assert np.array_equal(ds, a0) and np.array_equal(asc, b0)
+
+
+PLANE_CASES = [
+    ((48, 40, 1), 71, "gauss", 1e-3),
+    ((37, 29, 1), 72, "noise", 2e-2),
+    ((64, 64, 1), 73, "noise", 1e-2),
+    ((130, 9, 1), 74, "gauss", 1e-3),
+    ((5, 3, 1), 75, "noise", 5e-2),
+    ((256, 200, 1), 76, "gauss", 1e-4),
+]
+
+
+@pytest.mark.parametrize("shape,seed,kind,er", PLANE_CASES)
+def test_plane_adjacency_parity(shape, seed, kind, er):
+    """The CPU path's plane adjacency (maxNeighbors = 6 on a depth-1 field:
+    +-x, +-y, classifyVertex1's triangle links and type rule) against the
+    oracle in the same mode, byte for byte with the counters."""
+    orig, decp, xi = make_field(shape, seed, kind, er)
+    with oracle.plane_adjacency():
+        f0, c0, s0 = oracle.correct(orig, decp, xi)
+    f1, c1, s1 = ez().correct(orig, decp, xi, adjacency=ez().ADJ_PLANE4)
+    assert s0["status"] == 0
+    assert f0.tobytes() == f1.tobytes()
+    assert np.array_equal(c0, c1)
+    assert {k: s0[k] for k in COUNTERS} == {k: s1[k] for k in COUNTERS}
+    # and it is a different mode from the stencil's
+    f2, _, s2 = ez().correct(orig, decp, xi)
+    assert (s2["n_max"], s2["n_min"], s2["n_saddle1"]) != (s1["n_max"], s1["n_min"],
+                                                         s1["n_saddle1"])
+
+
+def test_plane_adjacency_needs_depth_one():
+    orig, decp, xi = make_field((8, 8, 2), 77, "noise", 1e-2)
+    with pytest.raises(ez().ExtremzError) as e:
+        ez().correct(orig, decp, xi, adjacency=ez().ADJ_PLANE4)
+    assert e.value.code == -22
+
+
+def test_plane_adjacency_incremental_equals_full(monkeypatch):
+    orig, decp, xi = make_field((256, 200, 1), 76, "gauss", 1e-4)
+    monkeypatch.setenv("EXTREMZ_VERIFY_INC", "1")
+    f1, c1, s1 = ez().correct(orig, decp, xi, adjacency=ez().ADJ_PLANE4)
+    assert s1["status"] == 0
+    monkeypatch.setenv("EXTREMZ_VERIFY_INC", "0")
+    monkeypatch.setenv("EXTREMZ_FULL_PASSES", "1")
+    f2, c2, s2 = ez().correct(orig, decp, xi, adjacency=ez().ADJ_PLANE4)
+    assert f1.tobytes() == f2.tobytes() and np.array_equal(c1, c2)

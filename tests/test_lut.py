@@ -173,11 +173,13 @@ def mesh_type(flat, v, tris):
 @pytest.mark.parametrize("seed", range(4))
 @pytest.mark.parametrize("W,H", [(6, 5), (9, 3), (2, 7), (11, 11)])
 def test_depth1_is_the_reference_triangle_mesh(W, H, seed):
-    """A field of depth 1 is the reference's 2-D mode: its in-plane stencil
-    directions (+-x, +-y, +-(-1,1,0)) are exactly the edges of the reference's
-    planar triangulation, so each vertex's type equals the one its link in
-    that mesh gives (interior saddles have as many lower as upper link
-    components, so they carry both saddle bits)."""
+    """With the 14-neighbour stencil, a field of depth 1 is the planar
+    triangle mesh of src/ExtreM.cpp:597-627 with its full links: the in-plane
+    stencil directions (+-x, +-y, +-(-1,1,0)) are exactly the mesh's edges, so
+    each vertex's type equals the one its link in that mesh gives (interior
+    saddles have as many lower as upper link components, so they carry both
+    saddle bits).  The reference's CPU run itself (maxNeighbors = 6) is the
+    plane adjacency, below."""
     rng = np.random.default_rng(100 + seed)
     f = rng.integers(0, 5, size=(1, H, W)).astype(np.float64)
     types, _ = oracle.classify(f)
@@ -188,3 +190,98 @@ def test_depth1_is_the_reference_triangle_mesh(W, H, seed):
     # interior vertices: saddle means both bits (lower and upper cycles alternate)
     t2 = types.reshape(H, W)[1:-1, 1:-1]
     assert not np.isin(t2, (1, 2)).any()
+
+
+def classify_vertex1(flat, v, W, H, tris):
+    """classifyVertex1 (src/ExtreM.cpp:703-891) with maxNeighbors = 6 on a
+    depth-1 field, restated: the stars are the adjacency neighbours of
+    computeAdjacency (src/ExtreM.cpp:543-575: the first six directions of
+    src/ExtreM.cpp:16-21, so +x, -x, +y, -y in the plane); two star
+    neighbours of one side are joined when they are the other two corners of
+    a triangle around v (vertex_cells, src/ExtreM.cpp:1585-1595); the type
+    rule of src/ExtreM.cpp:880-886."""
+    x, y = v % W, v // W
+    adj = [(x + dx) + W * (y + dy) for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1))
+           if 0 <= x + dx < W and 0 <= y + dy < H]
+    key = (flat[v], v)
+    lower = [u for u in adj if (flat[u], u) < key]
+    upper = [u for u in adj if (flat[u], u) > key]
+    parent = {u: u for u in adj}
+
+    def find(a):
+        while parent[a] != a:
+            a = parent[a]
+        return a
+    for t in tris:
+        if v not in t:
+            continue
+        a, b = (u for u in t if u != v)
+        for side in (lower, upper):
+            if a in side and b in side:
+                parent[find(a)] = find(b)
+    lc = len({find(u) for u in lower})
+    uc = len({find(u) for u in upper})
+    if lc == 0 and uc == 1:
+        return 0
+    if lc == 1 and uc == 0:
+        return 4
+    if lc == 1 and uc == 1:
+        return 5
+    if lc > 1 and uc > 1:
+        return 3
+    if lc > 1:
+        return 1
+    if uc > 1:
+        return 2
+    return 5
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("W,H", [(6, 5), (9, 3), (2, 7), (11, 11), (7, 1)])
+def test_plane_adjacency_is_classifyVertex1(W, H, seed):
+    """With the CPU path's plane adjacency, the oracle's LUT classification
+    and lower stars equal classifyVertex1's, vertex by vertex, on tie-heavy
+    fields; its labels are the steepest paths over the four neighbours."""
+    rng = np.random.default_rng(300 + seed)
+    f = rng.integers(0, 5, size=(1, H, W)).astype(np.float64)
+    flat = f.ravel()
+    tris = reference_triangles(W, H) if W > 1 and H > 1 else []
+    with oracle.plane_adjacency():
+        types, masks = oracle.classify(f)
+        ds, asc = oracle.labels(f)
+        assert all(ds.ravel()[v] == oracle.max_label(f, v) for v in range(W * H))
+    assert types.ravel().tolist() == [classify_vertex1(flat, v, W, H, tris)
+                                      for v in range(W * H)]
+    # lower stars use the four directions only (bits 0-3)
+    assert not (masks.astype(np.int64) & ~0xF).any()
+    # steepest ascent over the four neighbours, by (value, id)
+    for v in range(W * H):
+        x, y = v % W, v // W
+        cur = v
+        while True:
+            cx, cy = cur % W, cur // W
+            nb = [(cx + dx) + W * (cy + dy) for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1))
+                  if 0 <= cx + dx < W and 0 <= cy + dy < H]
+            best = max([cur] + nb, key=lambda u: (flat[u], u))
+            if best == cur:
+                break
+            cur = best
+        assert ds.ravel()[v] == cur
+    # the stencil mode is back outside the block: the diagonals are present
+    t14, _ = oracle.classify(f)
+    assert t14.shape == types.shape
+
+
+def test_plane_adjacency_correction_terminates_with_invariants():
+    from tests.conftest import make_field
+    for shape, seed, kind, er in [((40, 30, 1), 61, "gauss", 1e-3), ((25, 19, 1), 62, "noise", 2e-2)]:
+        orig, decp, xi = make_field(shape, seed, kind, er)
+        with oracle.plane_adjacency():
+            fixed, counts, st = oracle.correct(orig, decp, xi)
+            t0, m0 = oracle.classify(orig)
+            t1, m1 = oracle.classify(fixed)
+        assert st["status"] == 0
+        assert np.all(np.abs(fixed - orig) <= xi + np.spacing(np.abs(orig) + xi))
+        assert np.array_equal(t0, t1)
+        sad = (t0 >= 1) & (t0 <= 3)
+        assert np.array_equal(m0[sad], m1[sad])
