@@ -50,11 +50,13 @@ class ExtremzError(RuntimeError):
 class Opts(ctypes.Structure):
     _fields_ = [("max_c_iters", ctypes.c_int), ("max_outer", ctypes.c_int),
                 ("timing", ctypes.c_int), ("fixed_step_q", ctypes.c_int),
-                ("adjacency", ctypes.c_int)]
+                ("adjacency", ctypes.c_int), ("rule_set", ctypes.c_int)]
 
 
 ADJ_STENCIL14 = 0  # the 14-neighbour stencil
 ADJ_PLANE4 = 1     # the CPU path's plane adjacency (+-x, +-y; depth-1 fields)
+RULES_CPU = 0      # the CPU path's midpoint / snap rules (or the fixed step)
+RULES_F32 = 1      # the float32 fork's rule set
 
 
 class Stats(ctypes.Structure):
@@ -207,13 +209,14 @@ def _timing_mask(timing):
 
 
 def correct(orig, decp, xi, max_c_iters=0, max_outer=0, timing=False, fixed_step_q=0,
-            adjacency=ADJ_STENCIL14):
+            adjacency=ADJ_STENCIL14, rule_set=RULES_CPU):
     """Corrects decp against orig within xi.  Returns (fixed, counts, stats).
 
     orig may be float32 or float64 (promoted exactly); inputs are not modified.
     timing selects the kernel families timed with HIP events (see _timing_mask).
     fixed_step_q > 0 selects the fixed-step rule set (step -xi/q, threshold q).
-    adjacency=ADJ_PLANE4 selects the CPU path's plane adjacency (depth 1).
+    adjacency=ADJ_PLANE4 selects the CPU path's plane adjacency (depth 1);
+    rule_set=RULES_F32 the float32 fork's rule set.
     """
     o = np.ascontiguousarray(orig)
     if o.dtype == np.float32:
@@ -227,7 +230,7 @@ def correct(orig, decp, xi, max_c_iters=0, max_outer=0, timing=False, fixed_step
     counts = np.zeros(o.shape, np.uint8)
     st = Stats()
     opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), int(fixed_step_q),
-                int(adjacency))
+                int(adjacency), int(rule_set))
     rc = lib().extremz_correct(_p(o), dt, _p(d), *_dims(o), float(xi), ctypes.byref(opts),
                                ctypes.byref(st), _p(counts))
     _check(rc)

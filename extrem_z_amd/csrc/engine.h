@@ -100,7 +100,9 @@ struct St {
   Geo g;
   double xi;
   int step_q;           // 0: midpoint / snap rules; q > 0: fixed step `step`, threshold q
-  double step;          // -xi / q
+  int f32;              // 1: the float32 fork's rule set (fixed step `step` = -xi/4 and the
+                        //    star fixes in float32, applyDeltaBuffer's cap 6 -> 7)
+  double step;          // -xi / q, or the float32 -xi / 4
   long long sent;       // ordered key of the -4*xi sentinel
   const uint8_t* lut;
   const double* orig;
@@ -222,7 +224,7 @@ __global__ void k_fill_i32(int* p, int v, long long n);
 __global__ void k_ctr_set(int* ctr, int idx, int v);
 __global__ void k_ctr_set2(int* ctr, int a, int b, int v);
 __global__ void k_f32_to_f64(const float* in, double* out, long long n);
-__global__ void k_check_finite(const double* a, long long n, int* ctr);
+__global__ void k_check_finite(const double* a, long long n, int* ctr, int f32);
 __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* type,
                            uint16_t* lower);
 template <int MODE>

@@ -228,6 +228,7 @@ class Run {
     c_->ev_used = 0;
     step_q_ = o ? o->fixed_step_q : 0;
     planar_ = o ? o->adjacency : 0;
+    f32_ = o ? o->rule_set == EXTREMZ_RULES_F32 : false;
     const char* e = std::getenv("EXTREMZ_FULL_PASSES");
     inc_off_ = e && *e && *e != '0';
     e = std::getenv("EXTREMZ_VERIFY_INC");
@@ -249,6 +250,7 @@ class Run {
   int max_c_, max_outer_;
   int step_q_ = 0;
   int planar_ = 0;  // the CPU path's plane adjacency (Geo::planar)
+  bool f32_ = false;  // the float32 fork's rule set (St::f32)
   int epoch_ = 0;
   bool clean_ = false;
   // the last CP pass of decp was a full one: the next incremental pass resets
@@ -1113,9 +1115,12 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   const long long N = s_.g.N;
   allocate(N);
   extremz_ctx* c = c_;
+  // the float32 fork keeps bound in float (src/ExtreM_cuda_f.cu:124)
+  if (f32_) xi = (double)(float)xi;
   s_.xi = xi;
   s_.step_q = step_q_;
-  s_.step = step_q_ ? -xi / step_q_ : 0.0;
+  s_.f32 = f32_ ? 1 : 0;
+  s_.step = step_q_ ? -xi / step_q_ : (f32_ ? (double)(-(float)xi / 4.0f) : 0.0);
   {
     double sd = -4.0 * xi;
     long long b;
@@ -1191,8 +1196,8 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   hipStream_t q = st();
   const int bN = blocks_for(N);
   ez::hip_check(hipMemsetAsync(s_.ctr, 0, sizeof(int) * C_COUNT_, q), "memset");
-  k_check_finite<<<bN, 256, 0, q>>>(d_orig, N, s_.ctr);
-  k_check_finite<<<bN, 256, 0, q>>>(d_decp, N, s_.ctr);
+  k_check_finite<<<bN, 256, 0, q>>>(d_orig, N, s_.ctr, s_.f32);
+  k_check_finite<<<bN, 256, 0, q>>>(d_decp, N, s_.ctr, s_.f32);
   k_fill_i64<<<bN, 256, 0, q>>>(s_.delta, s_.sent, N);
   ez::hip_check(hipMemsetAsync(s_.cnt, 0, N, q), "memset");
   ez::hip_check(hipMemsetAsync(s_.stamp, 0, sizeof(int) * N, q), "memset");
@@ -1434,7 +1439,9 @@ static void fill_stats(extremz_ctx* c, const Run& r, int status, double wall_ms,
 
 bool valid_opts(const extremz_opts* o) {
   return !o || (o->fixed_step_q >= 0 && o->fixed_step_q <= 254 &&  // counts are uint8
-                (o->adjacency == EXTREMZ_ADJ_STENCIL14 || o->adjacency == EXTREMZ_ADJ_PLANE4));
+                (o->adjacency == EXTREMZ_ADJ_STENCIL14 || o->adjacency == EXTREMZ_ADJ_PLANE4) &&
+                (o->rule_set == EXTREMZ_RULES_CPU ||
+                 (o->rule_set == EXTREMZ_RULES_F32 && o->fixed_step_q == 0)));
 }
 
 bool valid_shape(int W, int H, int D) {

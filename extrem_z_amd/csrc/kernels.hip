@@ -278,7 +278,23 @@ using Touched = StagedList<int, kStage>;
 // swap + the caller's `d > oldValue` gate, sequentially a max (D5).  The
 // first proposal above the sentinel lists the vertex as touched.
 __device__ __forceinline__ void propose(const St& s, int t, double d, Touched& tl) {
-  if (s.step_q) d = s.step;  // fixed-step rule: every proposal is -xi/q
+  // fixed-step rule: every proposal is -xi/q; the float32 fork's fixes are
+  // -bound / (1 << 2) (src/ExtreM_cuda_f.cu:2703, 2749, 2804)
+  if (s.step_q || s.f32) d = s.step;
+  long long k = okey(d);
+  if (k <= s.sent) return;
+  long long old = atomicMax(&s.delta[t], k);
+  if (old == s.sent) tl.push(t);
+}
+// The star fixes of get_false_stars, (orig - xi) - decp of a vertex: with the
+// float32 rule set in float32 (src/ExtreM_cuda_f.cu:2583-2600), with the
+// fixed-step rule the step.
+__device__ __forceinline__ double star_fix(const St& s, double o, double x) {
+  if (s.step_q) return s.step;
+  if (s.f32) return (double)(((float)o - (float)s.xi) - (float)x);
+  return (o - s.xi) - x;
+}
+__device__ __forceinline__ void propose_star(const St& s, int t, double d, Touched& tl) {
   long long k = okey(d);
   if (k <= s.sent) return;
   long long old = atomicMax(&s.delta[t], k);
@@ -320,10 +336,12 @@ __global__ void k_f32_to_f64(const float* in, double* out, long long n) {
     out[i] = (double)in[i];
 }
 // Sets ctr[C_FAIL] when any value is not finite.
-__global__ void k_check_finite(const double* a, long long n, int* ctr) {
+// Non-finite input fails the call; with the float32 rule set, so does a
+// value float32 cannot hold exactly (the fork keeps its fields in float).
+__global__ void k_check_finite(const double* a, long long n, int* ctr, int f32) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
-    if (!isfinite(a[i])) ctr[C_FAIL] = 1;
+    if (!isfinite(a[i]) || (f32 && (double)(float)a[i] != a[i])) ctr[C_FAIL] = 1;
 }
 
 // ---------------------------------------------------- classification ----
@@ -517,7 +535,7 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
       for (int d = 0; d < kNbr; d++) {
         old[d] = 0;
         if (!((a >> d) & 1)) continue;
-        const long long k = okey(s.step_q ? s.step : (po[d] - s.xi) - pd[d]);
+        const long long k = okey(star_fix(s, po[d], pd[d]));
         if (k <= s.sent) continue;
         old[d] = atomicMax(&s.delta[v + s.g.off[d]], k);
         sent |= 1u << d;
@@ -525,7 +543,7 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
 #pragma unroll
       for (int d = 0; d < kNbr; d++)
         if (((sent >> d) & 1) && old[d] == s.sent) tl.push(v + s.g.off[d]);
-      if (dl & ~ol) propose(s, v, (s.orig[v] - s.xi) - s.decp[v], tl);
+      if (dl & ~ol) propose_star(s, v, star_fix(s, s.orig[v], s.decp[v]), tl);
     }
   }
   tl.flush();
@@ -552,7 +570,16 @@ __global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
     double o = s.orig[t], x = s.decp[t];
     int c = s.cnt[t];
     double nx;
-    if (s.step_q) {  // applyDeltaBuffer, edit_type 0 (src/ExtreM_cuda.cu:3065-3076)
+    if (s.f32) {  // applyDeltaBuffer of the float32 fork (src/ExtreM_cuda_f.cu:2854-2866)
+      const float df = (float)dd, xf = (float)x, of = (float)o, bf = (float)s.xi;
+      if (fabsf(df) > 1e-6 && c < 6 && fabsf(of - (xf + df)) < bf) {
+        nx = (double)(xf + df);
+        c += 1;
+      } else {
+        nx = (double)(of - bf);
+        c = 7;
+      }
+    } else if (s.step_q) {  // applyDeltaBuffer, edit_type 0 (src/ExtreM_cuda.cu:3065-3076)
       if (c < s.step_q - 1 && fabs(o - (x + dd)) <= s.xi) {
         nx = x + dd;
         c += 1;
@@ -1078,11 +1105,12 @@ __global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr
         const double dv = ring[k % 5][(lx + 2) + kQX * (ly + 2)];
         long long key = s.sent;
         if (kinds & 1u) {
-          const long long ka = okey(s.step_q ? s.step : (og - s.xi) - dv);
+          const long long ka = okey(star_fix(s, og, dv));
           key = ka > key ? ka : key;
         }
         if (kinds & 0x8000u) {
-          const long long kb = okey(s.step_q ? s.step : ((og - s.xi) + dv) / 2.0 - dv);
+          const long long kb =
+              okey((s.step_q || s.f32) ? s.step : ((og - s.xi) + dv) / 2.0 - dv);
           key = kb > key ? kb : key;
         }
         if (key > s.sent) {
@@ -3145,7 +3173,7 @@ __global__ void k_presnap(St s) {
   if (v >= s.g.N) return;
   const int t = s.otype[v];
   if ((t >= 1 && t <= 3) || v == s.ctr[C_GMIN] || v == s.ctr[C_GMAX])
-    s.decp[v] = s.orig[v] - s.xi;
+    s.decp[v] = s.f32 ? (double)((float)s.orig[v] - (float)s.xi) : s.orig[v] - s.xi;
 }
 
 __global__ void k_argext(const double* f, int n, int* slot_min, int* slot_max) {

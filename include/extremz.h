@@ -96,10 +96,22 @@ typedef struct extremz_opts {
                        of src/ExtreM.cpp:16-21, i.e. +-x and +-y, with the
                        triangle links of src/ExtreM.cpp:597-627, 703-891;
                        fields of depth 1 only (else EXTREMZ_EINVAL) */
+  int rule_set;     /* EXTREMZ_RULES_CPU (0): the rules above;
+                       EXTREMZ_RULES_F32 (1): the float32 fork's rule set
+                       (src/ExtreM_cuda_f.cu:2583-2600, 2703, 2749, 2804,
+                       2854-2866) within the same control flow: xi, orig and
+                       decp in float32 (inputs float32 cannot hold exactly are
+                       EXTREMZ_EINVAL), every fix proposal -xi/4 and the star
+                       fixes (orig - xi) - decp in float32, an edit applied
+                       while |delta| > 1e-6, count < 6 and
+                       |orig - (decp + delta)| < xi, else decp := orig - xi
+                       with count 7.  Needs fixed_step_q 0. */
 } extremz_opts;
 
 #define EXTREMZ_ADJ_STENCIL14 0
 #define EXTREMZ_ADJ_PLANE4 1
+#define EXTREMZ_RULES_CPU 0
+#define EXTREMZ_RULES_F32 1
 
 typedef struct extremz_stats {
   int64_t counts[EXTREMZ_STAT_COUNT_];

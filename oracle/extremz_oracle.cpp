@@ -276,6 +276,7 @@ struct Engine {
   double bound;        // bound (xi)
   int max_c_iters, max_outer;
   int step_q = 0;  // 0: the CPU path's rules; q > 0: fixed step -xi/q, threshold q
+  bool f32 = false;  // the float32 fork's rule set (src/ExtreM_cuda_f.cu)
   std::vector<double> delta;       // d_deltaBuffer
   std::vector<uint8_t> counter;    // delta_counter
   std::vector<uint8_t> vt, dvt;    // vertex_type, dec_vertex_type
@@ -335,7 +336,22 @@ struct Engine {
   // src/ExtreM.cpp:1120-1150 with the caller's `d > oldValue` gate.
   void propose(int v, double d) {
     if (step_q) d = -bound / step_q;  // fixed-step rule: every proposal is -xi/q
+    // float32 fork: every fix proposal is -bound / (1 << 2) in float
+    // (src/ExtreM_cuda_f.cu:2703, 2749, 2804)
+    if (f32) d = (double)(-(float)bound / 4.0f);
     if (d > delta[v]) delta[v] = d;
+  }
+  // get_false_stars' (orig - xi) - decp; the fixed-step rule proposes its
+  // step there too, the float32 fork keeps the value, in float
+  // (src/ExtreM_cuda_f.cu:2583-2600)
+  void propose_star(int v) {
+    double d = (orig[v] - bound) - decp[v];
+    if (step_q) d = -bound / step_q;
+    if (f32) d = (double)(((float)orig[v] - (float)bound) - (float)decp[v]);
+    if (d > delta[v]) delta[v] = d;
+  }
+  double snap(int v) const {  // orig - xi (in float with the float32 fork)
+    return f32 ? (double)((float)orig[v] - (float)bound) : orig[v] - bound;
   }
   double lower_mid(int v) const {  // ((orig-xi)+decp)/2 - decp
     return ((orig[v] - bound) + decp[v]) / 2.0 - decp[v];
@@ -406,10 +422,9 @@ struct Engine {
     int m = g.nbrs(index, nb, dr);
     // findDifferences(lowerStar, o_lowerStar): original-lower, dec-upper
     for (int k = 0; k < m; k++)
-      if ((ol >> dr[k] & 1) && !(dl >> dr[k] & 1))
-        propose(nb[k], (orig[nb[k]] - bound) - decp[nb[k]]);
+      if ((ol >> dr[k] & 1) && !(dl >> dr[k] & 1)) propose_star(nb[k]);
     // findDifferences(o_lowerStar, lowerStar): dec-lower, original-upper
-    if (dl & ~ol) propose(index, (orig[index] - bound) - decp[index]);
+    if (dl & ~ol) propose_star(index);
   }
 
   // apply_delta, split, src/ExtreM.cpp:1486-1503 (D8); with the fixed-step
@@ -422,6 +437,18 @@ struct Engine {
       double dl = delta[t];
       if (dl == sentinel) continue;
       if (trace) trace->push_back({phase, (int)t, decp[t], 0.0});
+      if (f32) {  // applyDeltaBuffer, src/ExtreM_cuda_f.cu:2854-2866
+        const float df = (float)dl, xf = (float)decp[t], of = (float)orig[t], bf = (float)bound;
+        if (std::fabs(df) > 1e-6 && counter[t] < 6 && std::fabs(of - (xf + df)) < bf) {
+          decp[t] = (double)(xf + df);
+          counter[t] += 1;
+        } else {
+          counter[t] = 7;
+          decp[t] = (double)(of - bf);
+        }
+        if (trace) trace->back().after = decp[t];
+        continue;
+      }
       if (step_q) {
         if (counter[t] < step_q - 1 && std::abs(orig[t] - (decp[t] + dl)) <= bound) {
           decp[t] += dl;
@@ -928,10 +955,10 @@ struct Engine {
       if (isless(i, gmin, orig)) gmin = i;
       if (islarger(i, gmax, orig)) gmax = i;
     }
-    if (trace) trace->push_back({P_PRESNAP, gmin, decp[gmin], orig[gmin] - bound});
-    decp[gmin] = orig[gmin] - bound;
-    if (trace) trace->push_back({P_PRESNAP, gmax, decp[gmax], orig[gmax] - bound});
-    decp[gmax] = orig[gmax] - bound;
+    if (trace) trace->push_back({P_PRESNAP, gmin, decp[gmin], snap(gmin)});
+    decp[gmin] = snap(gmin);
+    if (trace) trace->push_back({P_PRESNAP, gmax, decp[gmax], snap(gmax)});
+    decp[gmax] = snap(gmax);
 
     // main.cpp:85-89
     classify_all(orig, vt.data(), lstar.data());
@@ -953,12 +980,12 @@ struct Engine {
 
     // main.cpp:122-131 (preserve_join_tree = true, main.cpp:21)
     for (int s : ocp.saddles2) {
-      if (trace) trace->push_back({P_PRESNAP, s, decp[s], orig[s] - bound});
-      decp[s] = orig[s] - bound;
+      if (trace) trace->push_back({P_PRESNAP, s, decp[s], snap(s)});
+      decp[s] = snap(s);
     }
     for (int s : ocp.saddles1) {
-      if (trace) trace->push_back({P_PRESNAP, s, decp[s], orig[s] - bound});
-      decp[s] = orig[s] - bound;
+      if (trace) trace->push_back({P_PRESNAP, s, decp[s], snap(s)});
+      decp[s] = snap(s);
     }
 
     c_loops();  // main.cpp:139
@@ -1439,20 +1466,39 @@ int oracle_correct_trace(const double* orig, double* decp, int W, int H, int D, 
   return (int)tr.size();
 }
 
+int oracle_correct_rules(const double* orig, double* decp, int W, int H, int D, double xi,
+                         int max_c_iters, int max_outer, uint8_t* counts, int64_t* stats,
+                         int threads, int step_q, int f32);
 int oracle_correct_rule(const double* orig, double* decp, int W, int H, int D, double xi,
                         int max_c_iters, int max_outer, uint8_t* counts, int64_t* stats,
                         int threads, int step_q) {
+  return oracle_correct_rules(orig, decp, W, H, D, xi, max_c_iters, max_outer, counts, stats,
+                              threads, step_q, 0);
+}
+// f32: the float32 fork's rule set (bound, orig and decp in float; inputs
+// that float cannot hold exactly are refused).
+int oracle_correct_rules(const double* orig, double* decp, int W, int H, int D, double xi,
+                         int max_c_iters, int max_outer, uint8_t* counts, int64_t* stats,
+                         int threads, int step_q, int f32) {
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #else
   (void)threads;
 #endif
   if (W <= 0 || H <= 0 || D <= 0 || !(xi > 0) || step_q < 0 || step_q > 254) return -22;
+  if (f32 && step_q) return -22;
+  const int64_t n = (int64_t)W * H * D;
+  if (f32) {
+    xi = (double)(float)xi;
+    for (int64_t i = 0; i < n; i++)
+      if ((double)(float)orig[i] != orig[i] || (double)(float)decp[i] != decp[i]) return -22;
+  }
   Engine e;
   e.g = Grid{W, H, D, (int64_t)W * H * D};
   e.orig = orig;
   e.decp = decp;
   e.bound = xi;
+  e.f32 = f32 != 0;
   e.step_q = step_q;
   e.max_c_iters = max_c_iters > 0 ? max_c_iters : 100000;
   e.max_outer = max_outer > 0 ? max_outer : 10000;

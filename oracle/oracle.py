@@ -39,6 +39,8 @@ def lib():
         L.oracle_min_label.argtypes = [vp, i32, i32, i32, i32]
         L.oracle_correct.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, vp, vp, i32]
         L.oracle_correct_rule.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, vp, vp, i32, i32]
+        L.oracle_correct_rules.argtypes = [vp, vp, i32, i32, i32, dbl, i32, i32, vp, vp, i32, i32,
+                                           i32]
         L.oracle_bitmap3.argtypes = [vp, i64, vp]
         L.oracle_bitmap3.restype = i64
         L.oracle_cost.argtypes = [vp, vp, vp, vp, i64, dbl, vp, vp, vp]
@@ -92,16 +94,20 @@ def min_label(field, v):
     return lib().oracle_min_label(_p(f), *_dims(f), int(v))
 
 
-def correct(orig, decp, xi, max_c_iters=0, max_outer=0, threads=0, fixed_step_q=0):
+def correct(orig, decp, xi, max_c_iters=0, max_outer=0, threads=0, fixed_step_q=0, f32=False):
     """Returns (fixed, counts, stats dict).  Inputs are not modified.
-    fixed_step_q > 0 selects the fixed-step rule set (step -xi/q, threshold q)."""
+    fixed_step_q > 0 selects the fixed-step rule set (step -xi/q, threshold q);
+    f32 the float32 fork's rule set."""
     o = np.ascontiguousarray(orig, dtype=np.float64)
     d = np.array(decp, dtype=np.float64, order="C", copy=True)
     counts = np.zeros(o.shape, np.uint8)
     st = np.zeros(12, np.int64)
-    lib().oracle_correct_rule(_p(o), _p(d), *_dims(o), float(xi), int(max_c_iters),
-                              int(max_outer), _p(counts), _p(st), int(threads),
-                              int(fixed_step_q))
+    rc = lib().oracle_correct_rules(_p(o), _p(d), *_dims(o), float(xi), int(max_c_iters),
+                                    int(max_outer), _p(counts), _p(st), int(threads),
+                                    int(fixed_step_q), int(bool(f32)))
+    if rc == -22:
+        raise ValueError("invalid input (shape, bound, rule options, or values the float32 "
+                         "rule set cannot hold)")
     return d, counts, dict(zip(STAT_KEYS, (int(x) for x in st)))
 
 

@@ -269,3 +269,46 @@ def test_plane_adjacency_incremental_equals_full(monkeypatch):
     monkeypatch.setenv("EXTREMZ_FULL_PASSES", "1")
     f2, c2, s2 = ez().correct(orig, decp, xi, adjacency=ez().ADJ_PLANE4)
     assert f1.tobytes() == f2.tobytes() and np.array_equal(c1, c2)
+
+
+def f32_case(shape, seed, kind, er):
+    """orig, decp and xi that float32 holds exactly."""
+    orig, decp, xi = make_field(shape, seed, kind, er)
+    return (orig.astype(np.float32).astype(np.float64), decp.astype(np.float32).astype(np.float64),
+            float(np.float32(xi)))
+
+
+def test_float32_rules_known_answers_on_gpu():
+    o = np.array([0.0, 0.5, 0.515625, 1.0]).reshape(1, 1, 4)
+    for d, want_f, want_c, iters in (([0.0, 0.734375, 0.265625, 1.0], [-0.25, 0.25, 0.265625, 0.75],
+                                      [0, 7, 0, 0], 7),
+                                     ([0.0, 0.3125, 0.265625, 1.0], [-0.25, 0.25, 0.265625, 0.75],
+                                      [0, 7, 0, 0], 1)):
+        f, c, st = ez().correct(o, np.array(d).reshape(1, 1, 4), 0.25, rule_set=ez().RULES_F32)
+        assert f.ravel().tolist() == want_f and c.ravel().tolist() == want_c
+        assert st["c_loop_iters"] == iters
+    orig = np.array([0.0, 1.0 + 2.0 ** -23]).reshape(1, 1, 2)
+    f, _, _ = ez().correct(orig, orig, 3 * 2.0 ** -25, rule_set=ez().RULES_F32)
+    assert f.ravel().tolist() == [-3 * 2.0 ** -25, 1.0]
+
+
+@pytest.mark.parametrize("shape,seed,kind,er", CASES[:6] + [((64, 64, 64), 81, "gauss", 1e-3)])
+def test_float32_rules_parity(shape, seed, kind, er):
+    orig, decp, xi = f32_case(shape, seed, kind, er)
+    f0, c0, s0 = oracle.correct(orig, decp, xi, f32=True)
+    f1, c1, s1 = ez().correct(orig.astype(np.float32), decp, xi, rule_set=ez().RULES_F32)
+    assert s0["status"] == 0
+    assert f0.tobytes() == f1.tobytes()
+    assert np.array_equal(c0, c1) and int(c1.max()) <= 7
+    assert {k: s0[k] for k in COUNTERS} == {k: s1[k] for k in COUNTERS}
+    assert np.all(f1.astype(np.float32).astype(np.float64) == f1)
+
+
+def test_float32_rules_reject_inexact_input():
+    orig, decp, xi = make_field((8, 8, 8), 82, "noise", 1e-2)
+    with pytest.raises(ez().ExtremzError) as e:
+        ez().correct(orig, decp + 1e-12, xi, rule_set=ez().RULES_F32)
+    assert e.value.code == -22
+    with pytest.raises(ez().ExtremzError) as e:
+        ez().correct(orig, decp, xi, rule_set=ez().RULES_F32, fixed_step_q=4)
+    assert e.value.code == -22

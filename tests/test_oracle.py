@@ -122,3 +122,53 @@ def test_vectorised_violations_equal_the_loops(shape, seed, kind, er):
     fixed, _, _ = oracle.correct(orig, decp, xi)
     for g in (decp, fixed, 0.5 * (decp + orig)):
         assert violations(orig, g) == violations_fast(orig, g)
+
+
+# The float32 fork's rule set (src/ExtreM_cuda_f.cu): fix proposals
+# -bound / (1 << 2), applyDeltaBuffer's |delta| > 1e-6, count < 6 and strict
+# |orig - (decp + delta)| < bound, else decp := orig - bound with count 7,
+# all in float.  Values below are exact in float32.
+F32_ORIG = np.array([0.0, 0.5, 0.515625, 1.0]).reshape(1, 1, 4)
+
+
+def test_known_answer_float32_cap():
+    # Pre-snap: v0 -> -0.25, v3 -> 0.75.  v1 (47/64) is a false maximum above
+    # v2 (17/64), which is a false minimum whose smallest original neighbour
+    # is v1: both propose -0.0625 to v1.  Six steps (count 1..6):
+    # 0.671875, 0.609375, 0.546875, 0.484375, 0.421875, 0.359375, each
+    # within the bound and still above v2; the seventh iteration finds
+    # count 6 and snaps v1 to 0.5 - 0.25 with count 7.
+    decp = np.array([0.0, 0.734375, 0.265625, 1.0]).reshape(1, 1, 4)
+    fixed, counts, st = oracle.correct(F32_ORIG, decp, 0.25, f32=True)
+    assert st["status"] == 0 and st["c_loop_iters"] == 7 and st["outer_iters"] == 0
+    assert fixed.ravel().tolist() == [-0.25, 0.25, 0.265625, 0.75]
+    assert counts.ravel().tolist() == [0, 7, 0, 0]
+    # the fixed-step rule with q = 4 caps at three steps (count 4)
+    _, c4, s4 = oracle.correct(F32_ORIG, decp, 0.25, fixed_step_q=4)
+    assert c4.ravel().tolist() == [0, 4, 0, 0] and s4["c_loop_iters"] == 4
+
+
+def test_known_answer_float32_strict_bound():
+    # v1 = 0.3125 steps to 0.25 = orig - xi: |0.5 - 0.25| = 0.25 is not below
+    # the bound, so the apply snaps (count 7) instead of stepping (count 1).
+    decp = np.array([0.0, 0.3125, 0.265625, 1.0]).reshape(1, 1, 4)
+    fixed, counts, st = oracle.correct(F32_ORIG, decp, 0.25, f32=True)
+    assert fixed.ravel().tolist() == [-0.25, 0.25, 0.265625, 0.75]
+    assert counts.ravel().tolist() == [0, 7, 0, 0] and st["c_loop_iters"] == 1
+
+
+def test_known_answer_float32_rounding():
+    # Pre-snap of the global maximum 1 + 2^-23 with bound 3 * 2^-25: in float
+    # (1 + 2^-23) - 3 * 2^-25 = 1 + 2^-25 rounds to 1.0; in double it is exact.
+    orig = np.array([0.0, 1.0 + 2.0 ** -23]).reshape(1, 1, 2)
+    xi = 3 * 2.0 ** -25
+    fixed, _, st = oracle.correct(orig, orig, xi, f32=True)
+    assert st["status"] == 0
+    assert fixed.ravel().tolist() == [-xi, 1.0]
+    f64, _, _ = oracle.correct(orig, orig, xi)
+    assert f64.ravel().tolist() == [-xi, 1.0 + 2.0 ** -25]
+
+
+def test_float32_rules_refuse_inexact_inputs():
+    with pytest.raises(ValueError):
+        oracle.correct(F32_ORIG, F32_ORIG + 0.1, 0.25, f32=True)
