@@ -50,7 +50,8 @@ class ExtremzError(RuntimeError):
 class Opts(ctypes.Structure):
     _fields_ = [("max_c_iters", ctypes.c_int), ("max_outer", ctypes.c_int),
                 ("timing", ctypes.c_int), ("fixed_step_q", ctypes.c_int),
-                ("adjacency", ctypes.c_int), ("rule_set", ctypes.c_int)]
+                ("adjacency", ctypes.c_int), ("rule_set", ctypes.c_int),
+                ("persistence", ctypes.c_double)]
 
 
 ADJ_STENCIL14 = 0  # the 14-neighbour stencil
@@ -209,14 +210,15 @@ def _timing_mask(timing):
 
 
 def correct(orig, decp, xi, max_c_iters=0, max_outer=0, timing=False, fixed_step_q=0,
-            adjacency=ADJ_STENCIL14, rule_set=RULES_CPU):
+            adjacency=ADJ_STENCIL14, rule_set=RULES_CPU, persistence=0.0):
     """Corrects decp against orig within xi.  Returns (fixed, counts, stats).
 
     orig may be float32 or float64 (promoted exactly); inputs are not modified.
     timing selects the kernel families timed with HIP events (see _timing_mask).
     fixed_step_q > 0 selects the fixed-step rule set (step -xi/q, threshold q).
     adjacency=ADJ_PLANE4 selects the CPU path's plane adjacency (depth 1);
-    rule_set=RULES_F32 the float32 fork's rule set.
+    rule_set=RULES_F32 the float32 fork's rule set; persistence > 0 the GPU
+    fork's persistence filter of the 2-saddle triplets (absolute threshold).
     """
     o = np.ascontiguousarray(orig)
     if o.dtype == np.float32:
@@ -230,7 +232,7 @@ def correct(orig, decp, xi, max_c_iters=0, max_outer=0, timing=False, fixed_step
     counts = np.zeros(o.shape, np.uint8)
     st = Stats()
     opts = Opts(int(max_c_iters), int(max_outer), _timing_mask(timing), int(fixed_step_q),
-                int(adjacency), int(rule_set))
+                int(adjacency), int(rule_set), float(persistence))
     rc = lib().extremz_correct(_p(o), dt, _p(d), *_dims(o), float(xi), ctypes.byref(opts),
                                ctypes.byref(st), _p(counts))
     _check(rc)

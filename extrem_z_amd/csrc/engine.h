@@ -103,6 +103,8 @@ struct St {
   int f32;              // 1: the float32 fork's rule set (fixed step `step` = -xi/4 and the
                         //    star fixes in float32, applyDeltaBuffer's cap 6 -> 7)
   double step;          // -xi / q, or the float32 -xi / 4
+  double thr;           // > 0: 2-saddle triplets keep the maxima at least thr above or
+                        //    below the saddle (findAscPaths, src/ExtreM_cuda.cu:1878)
   long long sent;       // ordered key of the -4*xi sentinel
   const uint8_t* lut;
   const double* orig;

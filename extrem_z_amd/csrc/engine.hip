@@ -229,6 +229,7 @@ class Run {
     step_q_ = o ? o->fixed_step_q : 0;
     planar_ = o ? o->adjacency : 0;
     f32_ = o ? o->rule_set == EXTREMZ_RULES_F32 : false;
+    thr_ = o ? o->persistence : 0.0;
     const char* e = std::getenv("EXTREMZ_FULL_PASSES");
     inc_off_ = e && *e && *e != '0';
     e = std::getenv("EXTREMZ_VERIFY_INC");
@@ -251,6 +252,7 @@ class Run {
   int step_q_ = 0;
   int planar_ = 0;  // the CPU path's plane adjacency (Geo::planar)
   bool f32_ = false;  // the float32 fork's rule set (St::f32)
+  double thr_ = 0.0;  // the persistence filter of the 2-saddle triplets (St::thr)
   int epoch_ = 0;
   bool clean_ = false;
   // the last CP pass of decp was a full one: the next incremental pass resets
@@ -1120,6 +1122,11 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
   s_.xi = xi;
   s_.step_q = step_q_;
   s_.f32 = f32_ ? 1 : 0;
+  s_.thr = thr_;
+  // with the persistence filter an original extremum -> saddle index lacks
+  // the pairs the filter dropped, through which a decp change could reach a
+  // side: every CP pass of decp is a full pass
+  if (thr_ > 0.0) inc_off_ = true;
   s_.step = step_q_ ? -xi / step_q_ : (f32_ ? (double)(-(float)xi / 4.0f) : 0.0);
   {
     double sd = -4.0 * xi;
@@ -1441,7 +1448,8 @@ bool valid_opts(const extremz_opts* o) {
   return !o || (o->fixed_step_q >= 0 && o->fixed_step_q <= 254 &&  // counts are uint8
                 (o->adjacency == EXTREMZ_ADJ_STENCIL14 || o->adjacency == EXTREMZ_ADJ_PLANE4) &&
                 (o->rule_set == EXTREMZ_RULES_CPU ||
-                 (o->rule_set == EXTREMZ_RULES_F32 && o->fixed_step_q == 0)));
+                 (o->rule_set == EXTREMZ_RULES_F32 && o->fixed_step_q == 0)) &&
+                std::isfinite(o->persistence) && o->persistence >= 0.0);
 }
 
 bool valid_shape(int W, int H, int D) {

@@ -1242,9 +1242,10 @@ __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* 
                                             const int* Lo, int v, unsigned star, unsigned ostar,
                                             const int* s_off, int* F, int* La, int o,
                                             unsigned& cand, int (&U)[kNbr], int& f_end,
-                                            int& l_end) {
+                                            int& l_end, bool& thr_changed) {
   const Geo& g = s.g;
   int R[kNbr], O[kNbr];
+  thr_changed = false;
 #pragma unroll
   for (int d = 0; d < kNbr; d++) {
     const bool in = (star >> d) & 1;
@@ -1261,10 +1262,25 @@ __device__ __forceinline__ void saddle_side(const St& s, int dec, const double* 
   }
   double V[kNbr];
 #pragma unroll
-  for (int d = 0; d < kNbr; d++) {
-    V[d] = ((uniq >> d) & 1) ? f[R[d]] : 0.0;
-    U[d] = ((uniq >> d) & 1) ? R[d] : kNone;  // the distinct roots
+  for (int d = 0; d < kNbr; d++) V[d] = ((uniq >> d) & 1) ? f[R[d]] : 0.0;
+  if (SIDE == 2 && s.thr > 0.0) {
+    // the persistence filter of findAscPaths (src/ExtreM_cuda.cu:1877-1881):
+    // a maximum closer than thr to the saddle leaves its triplet; a decp side
+    // whose kept set differs from the original's is a changed side
+    const double fv = f[v], ov = s.orig[v];
+    unsigned keep = 0, okeep = 0;
+#pragma unroll
+    for (int d = 0; d < kNbr; d++) {
+      if (!((uniq >> d) & 1)) continue;
+      if (fabs(V[d] - fv) >= s.thr) keep |= 1u << d;
+      if (dec && fabs(s.orig[R[d]] - ov) >= s.thr) okeep |= 1u << d;
+    }
+    thr_changed = dec && keep != okeep;
+    uniq = keep;
   }
+#pragma unroll
+  for (int d = 0; d < kNbr; d++)
+    U[d] = ((uniq >> d) & 1) ? R[d] : kNone;  // the distinct roots kept
   int lo = kNone, hi = kNone;
   double flo = 0.0, fhi = 0.0;
 #pragma unroll
@@ -1423,11 +1439,12 @@ __device__ __forceinline__ void saddle_entry(const St& s, int mode, const double
   int E[kNbr];
   unsigned cand = 0;
   int f_end, l_end;
+  bool thr_changed;
   saddle_side<SIDE>(s, dec, f, L, SIDE == 2 ? s.oDS : s.oAS, v, star, ostar, s_off, F, La,
-                    act ? i : -1, cand, E, f_end, l_end);
+                    act ? i : -1, cand, E, f_end, l_end, thr_changed);
   // list positions of the distinct roots (maxima even, minima odd; roots
   // outside the lists have none) where this saddle pushes or is indexed
-  const bool changed = dec && (star != ostar || cand != 0);
+  const bool changed = dec && (star != ostar || cand != 0 || thr_changed);
   if (!dec || changed) {
 #pragma unroll
     for (int d = 0; d < kNbr; d++) {
@@ -2934,7 +2951,9 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
     const int i = WL[k];
     int sv = S[i];
     int a = of[i], b = ol[i], c = df[i], d = dl[i];
-    bool mf = a != c, ml = b != d;
+    // a NONE end (an empty triplet: only the persistence filter makes one)
+    // is no mismatch on either side
+    bool mf = a != c && a != kNone && c != kNone, ml = b != d && b != kNone && d != kNone;
     if (phase == 0) {
       if (mf) atomicMin(&s.win_a[a], sv);
       if (ml) atomicMin(&s.win_b[b], sv);
@@ -2986,18 +3005,20 @@ __global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do
     s.ctr[side == 2 ? C_WMC : C_WMINC2] = 0;
     s.ctr[side == 2 ? C_WMC2 : C_WMINC] = 0;
   }
+  // a NONE end (an empty triplet) is no mismatch, as in k_wrong_index
+  auto differs = [](int o, int dv) { return o != dv && o != kNone && dv != kNone; };
   for (int k = t; k < n; k += 1024) {  // register
     const int i = WL[k];
     const int sv = S[i], a = of[i], b = ol[i];
-    if (a != df[i]) atomicMin(&s.win_a[a], sv);
-    if (b != dl[i]) atomicMin(&s.win_b[b], sv);
+    if (differs(a, df[i])) atomicMin(&s.win_a[a], sv);
+    if (differs(b, dl[i])) atomicMin(&s.win_b[b], sv);
   }
   __syncthreads();
   for (int k = t; k < n; k += 1024) {  // count and fix
     const int i = WL[k];
     const int sv = S[i];
     const int a = of[i], b = ol[i], c = df[i], d = dl[i];
-    if (a != c &&
+    if (differs(a, c) &&
         __hip_atomic_load(&s.win_a[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == sv) {
       cf.inc();
       if (side == 2) {
@@ -3006,7 +3027,7 @@ __global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do
         if (do_fix && c != kNone) propose(s, c, lower_mid(s, c), tl);
       }
     }
-    if (b != d &&
+    if (differs(b, d) &&
         __hip_atomic_load(&s.win_b[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == sv) {
       cl.inc();
       if (side == 2) {
@@ -3020,9 +3041,9 @@ __global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do
   for (int k = t; k < n; k += 1024) {  // release
     const int i = WL[k];
     const int a = of[i], b = ol[i];
-    if (a != df[i])
+    if (differs(a, df[i]))
       __hip_atomic_store(&s.win_a[a], 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (b != dl[i])
+    if (differs(b, dl[i]))
       __hip_atomic_store(&s.win_b[b], 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   tl.flush();

@@ -106,6 +106,11 @@ typedef struct extremz_opts {
                        while |delta| > 1e-6, count < 6 and
                        |orig - (decp + delta)| < xi, else decp := orig - xi
                        with count 7.  Needs fixed_step_q 0. */
+  double persistence; /* > 0: the GPU fork's persistence threshold thr
+                       (findAscPaths, src/ExtreM_cuda.cu:1877-1881; thr =
+                       (max - min) * sim, :231): a 2-saddle's triplet keeps
+                       only the maxima with |f(max) - f(saddle)| >= thr, in
+                       each field's own values.  Absolute, like xi; 0 = off. */
 } extremz_opts;
 
 #define EXTREMZ_ADJ_STENCIL14 0

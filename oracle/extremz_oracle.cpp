@@ -44,6 +44,11 @@
 //       (getSortedPositions, src/ExtreM.cpp:1840-1860, leaves them to std::sort).
 //   D15 the split tree of each field uses that field's own types, labels and
 //       value order (main.cpp:459-461 sorts the original's lists by decp).
+//   D16 a triplet end that is NONE (an empty triplet, D11) on either side of
+//       a get_wrong_index_max/min comparison is no mismatch.  Only the
+//       persistence filter empties a triplet; the GPU fork skips a saddle
+//       whose original triplet is empty (src/ExtreM_cuda.cu:3115, 3146) and
+//       reads stale entries for an empty decp one.
 //   LUT extension: fields with an axis of length 1 give boundary classes the
 //   reference has no LUT segment for; they use the same link-component rule.
 //   Iteration caps turn the reference's unbounded loops into an error code.
@@ -178,6 +183,8 @@ const Lut& lut() {
 // --------------------------------------------------------------- grid ----
 // Directions in use for every grid built after oracle_set_adjacency.
 int g_dmask = (1 << kNbr) - 1;
+// Persistence threshold of the 2-saddle triplets (oracle_set_persistence).
+double g_thr = 0.0;
 
 struct Grid {
   int W, H, D;
@@ -277,6 +284,7 @@ struct Engine {
   int max_c_iters, max_outer;
   int step_q = 0;  // 0: the CPU path's rules; q > 0: fixed step -xi/q, threshold q
   bool f32 = false;  // the float32 fork's rule set (src/ExtreM_cuda_f.cu)
+  double thr = g_thr;  // persistence filter of the 2-saddle triplets (0: none)
   std::vector<double> delta;       // d_deltaBuffer
   std::vector<uint8_t> counter;    // delta_counter
   std::vector<uint8_t> vt, dvt;    // vertex_type, dec_vertex_type
@@ -587,8 +595,11 @@ struct Engine {
       int s = cp.saddles2[i];
       Triplet& t = cp.trip2[i];
       int m = g.nbrs(s, nb);
+      // the GPU fork's persistence filter (findAscPaths,
+      // src/ExtreM_cuda.cu:1877-1881): a maximum closer than thr stays out
       for (int k = 0; k < m; k++)
-        if (islarger(nb[k], s, f)) t.roots.push_back(ds[nb[k]]);
+        if (islarger(nb[k], s, f) && (thr <= 0.0 || std::fabs(f[ds[nb[k]]] - f[s]) >= thr))
+          t.roots.push_back(ds[nb[k]]);
       // rank ascending over a value-ascending maximum list == (value,id) asc
       std::sort(t.roots.begin(), t.roots.end(), asc);
       t.roots.erase(std::unique(t.roots.begin(), t.roots.end()), t.roots.end());
@@ -763,7 +774,7 @@ struct Engine {
     size_t n = std::min(dcp.trip2.size(), ocp.trip2.size());
     for (size_t i = 0; i < n; i++) {
       const Triplet &o = ocp.trip2[i], &d = dcp.trip2[i];
-      if (o.first() != d.first()) {
+      if (o.first() != d.first() && o.first() != NONE && d.first() != NONE) {  // D16
         int id = o.first();
         if (id != NONE && !flag_a[id]) {
           wrmi.push_back(o.first());
@@ -774,7 +785,7 @@ struct Engine {
           overwrite(wrmi, id, d.first());
         }
       }
-      if (o.last() != d.last()) {
+      if (o.last() != d.last() && o.last() != NONE && d.last() != NONE) {  // D16
         int id = o.last();
         if (id != NONE && !flag_b[id]) {
           wrmi2.push_back(o.last());
@@ -798,7 +809,7 @@ struct Engine {
     size_t n = std::min(dcp.trip1.size(), ocp.trip1.size());
     for (size_t i = 0; i < n; i++) {
       const Triplet &o = ocp.trip1[i], &d = dcp.trip1[i];
-      if (o.first() != d.first()) {
+      if (o.first() != d.first() && o.first() != NONE && d.first() != NONE) {  // D16
         int id = o.first();
         if (id != NONE && !flag_b[id]) {
           wrmini2.push_back(o.first());
@@ -809,7 +820,7 @@ struct Engine {
           overwrite(wrmini2, id, d.first());
         }
       }
-      if (o.last() != d.last()) {
+      if (o.last() != d.last() && o.last() != NONE && d.last() != NONE) {  // D16
         int id = o.last();
         if (id != NONE && !flag_a[id]) {
           wrmini.push_back(o.last());
@@ -1251,6 +1262,12 @@ int oracle_link_type(int key, int config) { return link_type(key, config); }
 // path's plane adjacency (+-x, +-y; depth-1 fields).
 int oracle_set_adjacency(int plane) {
   g_dmask = plane ? 0xF : (1 << kNbr) - 1;
+  return 0;
+}
+// Persistence threshold of the 2-saddle triplets for every later correction
+// (0: none).
+int oracle_set_persistence(double thr) {
+  g_thr = thr;
   return 0;
 }
 

@@ -45,6 +45,7 @@ def lib():
         L.oracle_bitmap3.restype = i64
         L.oracle_cost.argtypes = [vp, vp, vp, vp, i64, dbl, vp, vp, vp]
         L.oracle_set_adjacency.argtypes = [i32]
+        L.oracle_set_persistence.argtypes = [dbl]
         _lib = L
     return _lib
 
@@ -256,3 +257,19 @@ class plane_adjacency:
 
     def __exit__(self, *a):
         lib().oracle_set_adjacency(0)
+
+
+class persistence:
+    """Context manager: inside the block the oracle's 2-saddle triplets keep
+    only the maxima at least thr away from the saddle (the GPU fork's
+    findAscPaths filter)."""
+
+    def __init__(self, thr):
+        self.thr = float(thr)
+
+    def __enter__(self):
+        lib().oracle_set_persistence(self.thr)
+        return self
+
+    def __exit__(self, *a):
+        lib().oracle_set_persistence(0.0)

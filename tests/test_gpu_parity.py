@@ -312,3 +312,33 @@ def test_float32_rules_reject_inexact_input():
     with pytest.raises(ez().ExtremzError) as e:
         ez().correct(orig, decp, xi, rule_set=ez().RULES_F32, fixed_step_q=4)
     assert e.value.code == -22
+
+
+@pytest.mark.parametrize("shape,seed,kind,er,sim", [
+    ((16, 16, 16), 3, "gauss", 1e-3, 2e-3), ((48, 40, 1), 10, "gauss", 1e-3, 5e-3),
+    ((48, 40, 1), 102, "gauss", 1e-3, 1e-2)])
+def test_persistence_filter_parity(shape, seed, kind, er, sim):
+    """The persistence filter (thr = (max - min) * sim, src/ExtreM_cuda.cu:231)
+    against the oracle, byte for byte with the counters; on the last field the
+    filter changes the result (no outer iteration instead of one)."""
+    orig, decp, xi = make_field(shape, seed, kind, er)
+    thr = float(orig.max() - orig.min()) * sim
+    with oracle.persistence(thr):
+        f0, c0, s0 = oracle.correct(orig, decp, xi)
+    f1, c1, s1 = ez().correct(orig, decp, xi, persistence=thr)
+    assert s0["status"] == 0
+    assert f0.tobytes() == f1.tobytes() and np.array_equal(c0, c1)
+    assert {k: s0[k] for k in COUNTERS} == {k: s1[k] for k in COUNTERS}
+
+
+def test_persistence_filter_cap():
+    """A filter that leaves the two fields' triplets apart for good: the loop
+    ends at its cap on both sides (EXTREMZ_ENOTCONV)."""
+    orig, decp, xi = make_field((20, 14, 10), 4, "noise", 2e-2)
+    thr = float(orig.max() - orig.min()) * 0.05
+    with oracle.persistence(thr):
+        _, _, s0 = oracle.correct(orig, decp, xi, max_outer=20)
+    assert s0["status"] == -3
+    with pytest.raises(ez().ExtremzError) as e:
+        ez().correct(orig, decp, xi, persistence=thr, max_outer=20)
+    assert e.value.code == -3

@@ -358,3 +358,56 @@ def test_rule_states_corrected_on_gpu(state):
     assert s0["status"] == s1["status"] == 0
     assert f0.tobytes() == f1.tobytes() and np.array_equal(c0, c1)
     assert {k: s0[k] for k in oracle.STAT_KEYS} == {k: s1[k] for k in oracle.STAT_KEYS}
+
+
+# ------------------------------------------------- persistence filter (thr)
+# The GPU fork's findAscPaths keeps a maximum in a 2-saddle's triplet only
+# when |f(max) - f(saddle)| >= thr (src/ExtreM_cuda.cu:1877-1881), each field
+# with its own values.  Case E after the pre-snap: saddle 9 (orig 8, decp 2)
+# reaches maxima 1 (orig 40, decp 40) and 7 (orig 44, decp 38).
+E_PRESNAP = [16, 40, -6, 33, 26, -2, 14, 38, 25, 2, 12, 18]
+
+
+def test_persistence_filter_single_rule():
+    # thr = 34/64: the original triplet drops 1 (40 - 8 = 32 < 34) and keeps 7
+    # (36): first = last = 7.  decp keeps both (38, 36): ascending 7, 1.  The
+    # first ends agree; the last ends give the pair (7, 1) only, whose
+    # direction-1 fix lowers the false index 1 by 3/64.
+    o, d = field(CASES["E"]["orig"]), field(E_PRESNAP)
+    plain = oracle.rule_probe(o, d, XI, "wrong_index_max")
+    assert plain["pairs_0"] == [(1, 7)] and plain["pairs_1"] == [(7, 1)]
+    with oracle.persistence(34 * U):
+        out = oracle.rule_probe(o, d, XI, "wrong_index_max")
+    assert out["pairs_0"] == [] and out["pairs_1"] == [(7, 1)]
+    assert {v: x / U for v, x in out["proposals"].items()} == {1: -3}
+
+
+def test_persistence_filter_full_run():
+    # thr = 40/64 empties both triplets (36 and 32 / 38 and 36 are below it):
+    # no wrong index, nothing to fix, the pre-snapped field stands.
+    o, d = field(CASES["E"]["orig"]), field(CASES["E"]["decp"])
+    with oracle.persistence(40 * U):
+        fixed, counts, st = oracle.correct(o, d, XI)
+    assert st["status"] == 0 and st["outer_iters"] == 0
+    assert fixed.tobytes() == field(E_PRESNAP).tobytes() and not counts.any()
+    # thr = 34/64: the first round lowers 1 to 37 (the single-rule state
+    # above); then decp keeps {1 (37 - 2 = 35), 7 (36)}, first 1, while the
+    # original keeps {7}: the pair (7, 1) lowers 7, which sits at orig - xi
+    # (the midpoint step is 0, so it snaps there again), and the mismatch
+    # stays: the loop ends at its cap with 1 at 37.
+    with oracle.persistence(34 * U):
+        fixed, counts, st = oracle.correct(o, d, XI, max_outer=20)
+    assert st["status"] == -3
+    assert fixed.ravel()[1] == 37 * U and fixed.ravel()[7] == 38 * U
+
+
+@pytest.mark.gpu
+def test_persistence_filter_on_gpu():
+    import extrem_z_amd as ez
+    o, d = field(CASES["E"]["orig"]), field(CASES["E"]["decp"])
+    fixed, counts, st = ez.correct(o, d, XI, persistence=40 * U)
+    assert st["status"] == 0 and st["outer_iters"] == 0
+    assert fixed.tobytes() == field(E_PRESNAP).tobytes() and not counts.any()
+    with pytest.raises(ez.ExtremzError) as e:
+        ez.correct(o, d, XI, persistence=34 * U, max_outer=20)
+    assert e.value.code == -3
