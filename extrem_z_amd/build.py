@@ -21,6 +21,39 @@ HIPCC_FLAGS = [
 ]
 
 
+def build_compat(hipcc, verbose=False):
+    """libExtreMz: the reference library's API (include/api/ExtreM.h) over
+    libextremz.so, as a shared library and as the static archive the
+    reference's CMake target ExtreMz::ExtreMz names."""
+    src = os.path.join(SRC, "compat.hip")
+    obj = os.path.join(OUT_DIR, "compat.o")
+    flags = [f for f in HIPCC_FLAGS if f not in ("-shared", "-ldl", "-pthread")]
+    cmd = [hipcc, *flags, "-c", src, "-o", obj + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(obj + ".tmp", obj)
+    so = os.path.join(OUT_DIR, "libExtreMz.so")
+    subprocess.run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", obj, "-o", so + ".tmp",
+                    "-L" + OUT_DIR, "-lextremz", "-Wl,-rpath,$ORIGIN"], check=True)
+    os.replace(so + ".tmp", so)
+    ar = os.path.join(OUT_DIR, "libExtreMz.a")
+    if os.path.exists(ar + ".tmp"):
+        os.remove(ar + ".tmp")
+    subprocess.run(["ar", "rcs", ar + ".tmp", obj], check=True)
+    os.replace(ar + ".tmp", ar)
+    # the test program that drives the API in main.cpp's order
+    root = os.path.dirname(HERE)
+    drv = os.path.join(root, "tests", "compat", "_build", "dropin_main")
+    os.makedirs(os.path.dirname(drv), exist_ok=True)
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fopenmp", "-I" + os.path.join(root, "include", "api"),
+                    os.path.join(root, "tests", "compat", "dropin_main.cpp"), "-o", drv + ".tmp",
+                    "-L" + OUT_DIR, "-lExtreMz", "-lextremz",
+                    "-Wl,-rpath,$ORIGIN/../../../extrem_z_amd/_build"], check=True)
+    os.replace(drv + ".tmp", drv)
+    return so
+
+
 def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -30,6 +63,7 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_compat(hipcc, verbose)
     cli = os.path.join(OUT_DIR, "extremz_cli")
     subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(SRC, "cli.cpp"), "-o", cli + ".tmp",
                     "-L" + OUT_DIR, "-lextremz", "-Wl,-rpath,$ORIGIN"], check=True)

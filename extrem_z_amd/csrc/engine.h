@@ -8,6 +8,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/extremz.h"
 
@@ -195,6 +196,54 @@ int run_correct(extremz_ctx* c, const double* d_orig, double* d_decp, int W, int
 int correct_staged(extremz_ctx* c, const void* orig, int orig_dtype, bool on_host, double* decp,
                    uint8_t* counts, int W, int H, int D, double xi, const extremz_opts* opts,
                    extremz_stats* stats);
+
+// Split tree of one field (compute_MergeT): per maximum in ascending
+// (value, id) order.
+struct TreeOut {
+  std::vector<int> max_v, pair_v, parent_v;
+  std::vector<std::vector<int>> branch;
+  int rounds = 0;
+};
+
+// Single passes over one device field on a context's stream (planar: the
+// plane adjacency of Geo::planar), synchronised before they return.
+void field_classify(extremz_ctx* c, int planar, const double* d_f, int W, int H, int D,
+                    uint8_t* d_types, uint16_t* d_lower);
+void field_labels(extremz_ctx* c, int planar, const double* d_f, int W, int H, int D, int* d_ds,
+                  int* d_as);
+void field_split_tree(extremz_ctx* c, int planar, const double* d_f, int W, int H, int D,
+                      TreeOut& out);
+
+// The correction of run_correct one step at a time, for a caller that drives
+// the reference's step functions itself (the main.cpp:65-446 sequence over
+// include/api/ExtreM.h).  Every step runs on the device state the previous
+// steps left; the steps throw HipError / std::range_error on failure.
+struct Stepper;
+Stepper* stepper_new(extremz_ctx* c, int planar, int max_c_iters);
+void stepper_free(Stepper* s);
+// main.cpp:65-131 on device buffers that stay the caller's for the stepper's
+// life: the original field's CP structures and the pre-snap of decp.
+void stepper_begin(Stepper* s, const double* d_orig, double* d_decp, uint8_t* d_counts, int W,
+                   int H, int D, double xi);
+void stepper_c_loops(Stepper* s);            // c_loops
+void stepper_cp(Stepper* s);                 // manifolds, computeCP and saddle maps of decp
+                                             // (nothing when decp is unchanged since the last)
+// The wrong-neighbour candidates of the last CP pass: {2 * saddle + side
+// (1 for a 1-saddle), mask of the stencil directions whose neighbour's decp
+// label differs from the original one at the same upper / lower position}.
+void stepper_candidates(Stepper* s, std::vector<int2>& out);
+// r_loops over the caller's wrong-neighbour lists (host arrays).
+void stepper_r_loops(Stepper* s, const int* l2, int n2, const int* l1, int n1);
+void stepper_s_loops(Stepper* s, int side, int do_fix);        // s_loops(_join) / get_wrong_index_*
+void stepper_saddle_loops(Stepper* s, int side, int do_fix);   // saddle_loops(_join) / get_wrong_index_saddles*
+// wrong_max_counter, wrong_max_counter_2, wrong_min_counter,
+// wrong_min_counter_2, wrong_saddle_counter, wrong_saddle_counter_join
+void stepper_counters(Stepper* s, int out[6]);
+// decp was changed by the caller: the next passes start from scratch.
+void stepper_decp_replaced(Stepper* s);
+const uint8_t* stepper_types(Stepper* s);   // the original field's types (device)
+// decp's root labels of the last CP pass (device): steepest ascent, descent
+void stepper_labels(Stepper* s, const int** d_ds, const int** d_as);
 
 // Runs fn with C++ errors translated into EXTREMZ_E* codes (message in g_err).
 template <class F>

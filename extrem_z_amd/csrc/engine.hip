@@ -211,14 +211,6 @@ namespace ez {
 
 thread_local std::string g_err;
 
-// Split tree of one field (compute_MergeT): per maximum in ascending
-// (value, id) order.
-struct TreeOut {
-  std::vector<int> max_v, pair_v, parent_v;
-  std::vector<std::vector<int>> branch;
-  int rounds = 0;
-};
-
 class Run {
  public:
   Run(extremz_ctx* c, const extremz_opts* o) : c_(c) {
@@ -236,6 +228,8 @@ class Run {
     inc_verify_ = e && *e && *e != '0';
   }
 
+  int begin(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
+            uint8_t* d_counts, bool presnap = true);
   int correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
               uint8_t* d_counts);
   void classify(const double* d_f, int W, int H, int D, uint8_t* d_t, uint16_t* d_l);
@@ -246,6 +240,7 @@ class Run {
   int64_t counts_[EXTREMZ_STAT_COUNT_] = {};
 
  private:
+  friend struct Stepper;
   extremz_ctx* c_;
   St s_{};
   int max_c_, max_outer_;
@@ -1107,8 +1102,10 @@ void Run::split_tree(const double* d_f, int W, int H, int D, TreeOut& out) {
   }
 }
 
-int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
-                 uint8_t* d_counts) {
+// main.cpp:65-131: checks, buffers, the original field's classification,
+// labels, CP lists and largest / smallest saddle maps, and the pre-snap.
+int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
+               uint8_t* d_counts, bool presnap) {
   if (planar_ && D != 1) {
     g_err = "the plane adjacency (EXTREMZ_ADJ_PLANE4) needs a field of depth 1";
     return EXTREMZ_EINVAL;
@@ -1292,8 +1289,17 @@ int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, doub
 
   slot_pass(0, d_orig, s_.oDS, s_.oAS, s_.olow, s_.of2, s_.ol2, s_.of1, s_.ol1, s_.olarg,
             s_.osmall);
-  // main.cpp:74-75, 122-131
-  k_presnap<<<(int)((N + 255) / 256), 256, 0, q>>>(s_);
+  // main.cpp:74-75, 122-131 (a caller that drives the steps snaps itself)
+  if (presnap) k_presnap<<<(int)((N + 255) / 256), 256, 0, q>>>(s_);
+  return 0;
+}
+
+int Run::correct(const double* d_orig, double* d_decp, int W, int H, int D, double xi,
+                 uint8_t* d_counts) {
+  if (int rc = begin(d_orig, d_decp, W, H, D, xi, d_counts)) return rc;
+  const long long N = s_.g.N;
+  hipStream_t q = st();
+  const int bN = blocks_for(N);
 
   c_loops();     // main.cpp:139
   cp_dec(true);  // main.cpp:172-191
@@ -1482,6 +1488,131 @@ int run_correct(extremz_ctx* c, const double* d_orig, double* d_decp, int W, int
   return status;
 }
 
+static extremz_opts adjacency_opts(int planar) {
+  extremz_opts o{};
+  o.adjacency = planar ? EXTREMZ_ADJ_PLANE4 : EXTREMZ_ADJ_STENCIL14;
+  return o;
+}
+
+void field_classify(extremz_ctx* c, int planar, const double* d_f, int W, int H, int D,
+                    uint8_t* d_types, uint16_t* d_lower) {
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  const extremz_opts o = adjacency_opts(planar);
+  Run r(c, &o);
+  r.classify(d_f, W, H, D, d_types, d_lower);
+}
+
+void field_labels(extremz_ctx* c, int planar, const double* d_f, int W, int H, int D, int* d_ds,
+                  int* d_as) {
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  const extremz_opts o = adjacency_opts(planar);
+  Run r(c, &o);
+  r.labels(d_f, W, H, D, d_ds, d_as);
+}
+
+void field_split_tree(extremz_ctx* c, int planar, const double* d_f, int W, int H, int D,
+                      TreeOut& out) {
+  hip_check(hipSetDevice(c->device), "hipSetDevice");
+  const extremz_opts o = adjacency_opts(planar);
+  Run r(c, &o);
+  r.split_tree(d_f, W, H, D, out);
+}
+
+// The correction's steps for a caller that drives them (friend of Run).
+struct Stepper {
+  extremz_opts opts;
+  Run run;
+  bool begun = false;
+  Stepper(extremz_ctx* c, const extremz_opts& o) : opts(o), run(c, &opts) {}
+
+  Run& r() {
+    if (!begun) throw HipError("correction step before its set-up");
+    return run;
+  }
+  void begin(const double* d_orig, double* d_decp, uint8_t* d_counts, int W, int H, int D,
+             double xi) {
+    hip_check(hipSetDevice(run.c_->device), "hipSetDevice");
+    run.c_->host_syncs = 0;
+    const int rc = run.begin(d_orig, d_decp, W, H, D, xi, d_counts, false);
+    if (rc) throw HipError(g_err.empty() ? "correction set-up failed" : g_err);
+    begun = true;
+  }
+  void c_loops() { r().c_loops(false); }
+  void cp() { r().cp_dec(!run.inc_ok_, false); }
+  void candidates(std::vector<int2>& out) {
+    Run& x = r();
+    x.pull();
+    const int n = x.h(C_ACT0);
+    out.resize(n);
+    if (n)
+      hip_check(hipMemcpyAsync(out.data(), x.c_->act0.p, sizeof(int2) * n,
+                               hipMemcpyDeviceToHost, x.st()),
+                "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(x.st()), "hipStreamSynchronize");
+  }
+  void r_loops(const int* l2, int n2, const int* l1, int n1) {
+    Run& x = r();
+    if (n2 < 0 || n1 < 0 || n2 > x.s_.g.N || n1 > x.s_.g.N)
+      throw HipError("wrong-neighbour list longer than the field");
+    hipStream_t q = x.st();
+    if (n2) hip_check(hipMemcpyAsync(x.s_.wnl2, l2, 4LL * n2, hipMemcpyHostToDevice, q), "h2d");
+    if (n1) hip_check(hipMemcpyAsync(x.s_.wnl1, l1, 4LL * n1, hipMemcpyHostToDevice, q), "h2d");
+    // the lists are the caller's: the device's own selection is not used
+    x.set_ctr(C_NFC, n2);
+    x.set_ctr(C_NFC1, n1);
+    x.pull();  // whether decp changed since its last CP pass (fixpath's fast path)
+    x.r_loops();
+    x.pull();
+  }
+  void s_loops(int side, int do_fix) {
+    r().s_loops(side, do_fix);
+    run.pull();
+  }
+  void saddle_loops(int side, int do_fix) {
+    r().saddle_loops(side, do_fix);
+    run.pull();
+  }
+  void counters(int out[6]) {
+    Run& x = r();
+    x.pull();
+    const int idx[6] = {C_WMC, C_WMC2, C_WMINC, C_WMINC2, C_WSC, C_WSCJ};
+    for (int k = 0; k < 6; k++) out[k] = x.h(idx[k]);
+  }
+  void decp_replaced() {
+    Run& x = r();
+    x.inc_ok_ = false;
+    x.clean_ = false;
+    x.set_ctr(C_ANY_CHANGE, 1);
+  }
+  const uint8_t* types() { return r().s_.otype; }
+  void labels(const int** ds, const int** as) {
+    *ds = r().s_.dDS;
+    *as = run.s_.dAS;
+  }
+};
+
+Stepper* stepper_new(extremz_ctx* c, int planar, int max_c_iters) {
+  extremz_opts o = adjacency_opts(planar);
+  o.max_c_iters = max_c_iters;
+  return new Stepper(c, o);
+}
+void stepper_free(Stepper* s) { delete s; }
+void stepper_begin(Stepper* s, const double* d_orig, double* d_decp, uint8_t* d_counts, int W,
+                   int H, int D, double xi) {
+  s->begin(d_orig, d_decp, d_counts, W, H, D, xi);
+}
+void stepper_c_loops(Stepper* s) { s->c_loops(); }
+void stepper_cp(Stepper* s) { s->cp(); }
+void stepper_candidates(Stepper* s, std::vector<int2>& out) { s->candidates(out); }
+void stepper_r_loops(Stepper* s, const int* l2, int n2, const int* l1, int n1) {
+  s->r_loops(l2, n2, l1, n1);
+}
+void stepper_s_loops(Stepper* s, int side, int do_fix) { s->s_loops(side, do_fix); }
+void stepper_saddle_loops(Stepper* s, int side, int do_fix) { s->saddle_loops(side, do_fix); }
+void stepper_counters(Stepper* s, int out[6]) { s->counters(out); }
+void stepper_decp_replaced(Stepper* s) { s->decp_replaced(); }
+const uint8_t* stepper_types(Stepper* s) { return s->types(); }
+void stepper_labels(Stepper* s, const int** d_ds, const int** d_as) { s->labels(d_ds, d_as); }
 
 int correct_staged(extremz_ctx* c, const void* orig, int orig_dtype, bool on_host, double* decp,
                    uint8_t* counts, int W, int H, int D, double xi, const extremz_opts* opts,
