@@ -94,6 +94,7 @@ struct Geo {
   int planar;
 };
 constexpr int kPlanarCls = 64;   // first boundary class of the plane adjacency
+constexpr int kLabelTileZ = 8;   // planes of a label tile (64 x 4 x kLabelTileZ voxels)
 constexpr int kNumCls = 80;      // boundary classes (64 stencil + 16 plane)
 
 // Everything a kernel needs, passed by value.
@@ -289,6 +290,7 @@ __global__ void k_check_two(St s, const int* flist, int nf, int* front, int fctr
                             int zero_idx);
 __global__ void k_propose_c(St s, const int* front, int fctr);
 __global__ void k_apply(St s, int zero_idx);
+template <bool FAST, bool TYPE>
 __global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t* lower,
                               uint8_t* pdir, const uint8_t* lut, uint8_t* type);
 __global__ void k_chase(int n, int* ds, int* as);

@@ -301,9 +301,10 @@ class Run {
     k_scan_propose<<<cg.blocks, dim3(64, 4, 1), 0, st()>>>(s_, fbuf, fctr, cg.nzc, cg.zlen);
     tend();
   }
-  // 64x4x4 label tiles
+  // 64x4xkLabelTileZ label tiles
   int tile_blocks() const {
-    return ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4) * ((s_.g.D + 3) / 4);
+    return ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4) *
+           ((s_.g.D + kLabelTileZ - 1) / kLabelTileZ);
   }
   // Capacity of the incremental pass's vertex lists: a pass with nch changed
   // vertices touches at most 15 * nch vertices, and runs only when that fits.
@@ -557,8 +558,12 @@ void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t
                      uint8_t* type) {
   counts_[EXTREMZ_STAT_LABEL_PASSES]++;
   tbegin(EXTREMZ_K_LABEL_TILE);
-  k_labels_tile<<<tile_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as, lower, pdir,
-                                                             c_->lut.p, type);
+  // every voxel on the ordered scan: W >= 3 on the stencil
+  const bool fast = s_.g.W >= 3 && !s_.g.planar;
+  auto* kern = fast ? (type ? k_labels_tile<true, true> : k_labels_tile<true, false>)
+                    : (type ? k_labels_tile<false, true> : k_labels_tile<false, false>);
+  kern<<<tile_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as, lower, pdir, c_->lut.p,
+                                                   type);
   tend();
   int nb = (s_.g.N + 255) / 256;
   tbegin(EXTREMZ_K_LABEL_CHASE);

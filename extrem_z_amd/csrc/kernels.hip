@@ -613,34 +613,47 @@ __global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
 }
 
 // ----------------------------------------------------- manifold labels ----
-// Tiles of 64x4x4 voxels, marched along z through a ring of three LDS planes
-// (66x6 values each, halo included; the next plane is prefetched into
-// registers while the current one is processed).  Steepest ascent / descent
-// pointers (first loops of src/ExtreM.cpp:146-157 and 504-515) are computed
-// from LDS and resolved inside the tile by walking them in LDS; a pointer then
-// names the root or the first vertex outside the tile on the steepest path.
-// With lower != nullptr the lower-star mask of every voxel is written too
-// (with type != nullptr also its vertex type from the LUT, the classification
-// of src/ExtreM_cuda.cu:3796-3853),
-// with pdir != nullptr its steepest directions ((up + 1) | (down + 1) << 4,
-// 0 for the voxel itself), the state the incremental CP pass starts from.
+// Tiles of 64x4x8 voxels (kLabelTileZ planes), marched along z through a ring
+// of three LDS planes (66x6 values each, halo included; the next plane is
+// prefetched into registers while the current one is processed).  Steepest
+// ascent / descent pointers (first loops of src/ExtreM.cpp:146-157 and
+// 504-515) are computed from LDS and resolved inside the tile by walking them
+// in LDS; a pointer then names the root or the first vertex outside the tile
+// on the steepest path.  With lower != nullptr the lower-star mask of every
+// voxel is written too (with TYPE also its vertex type from the LUT, the
+// classification of src/ExtreM_cuda.cu:3796-3853), with pdir != nullptr its
+// steepest directions ((up + 1) | (down + 1) << 4, 0 for the voxel itself),
+// the state the incremental CP pass starts from.
 //
-// Fast path: for an interior voxel of a field with W >= 3 the 15 candidate
-// ids (self + 14 neighbours) have the fixed order of their offsets
+// FAST (fields with W >= 3 on the stencil): the 15 candidate ids (self + 14
+// neighbours) have the fixed order of their offsets
 //   d9 < d13 < d5 < d11 < d3 < d7 < d1 < self < d0 < d6 < d2 < d10 < d4 < d12 < d8
-// so every (value, id) comparison of src/ExtreM.cpp:153 / 511 is one compare:
-// scanning in id order, ">=" keeps the highest (value, id), "<" the lowest.
-constexpr int kLTX = 64, kLTY = 4, kLTZ = 4, kLTN = kLTX * kLTY * kLTZ;
+// for every voxel whose neighbours exist, so each (value, id) comparison of
+// src/ExtreM.cpp:153 / 511 is one compare.  A neighbour outside the field
+// reads NaN in the ring, for which every comparison is false: it is neither
+// in the lower star nor ever a steepest choice, so boundary voxels take the
+// same straight-line code.  The scans start from the voxel itself, visit the
+// smaller ids in descending order (a tie keeps the larger id already held on
+// the way up, takes the smaller new id on the way down), then the larger ids
+// in ascending order (a tie takes the larger new id up, keeps the smaller
+// one down).  Otherwise (W < 3, or the plane adjacency) each voxel loops over
+// the directions its class key holds, with explicit id tie-breaks.
+constexpr int kLTX = 64, kLTY = 4, kLTZ = kLabelTileZ, kLTN = kLTX * kLTY * kLTZ;
 constexpr int kExit = (int)0x80000000u;  // label bit: a pointer out of the tile, not a root
 constexpr int kPX = kLTX + 2, kPY = kLTY + 2, kPN = kPX * kPY;
 constexpr int kPPer = (kPN + 255) / 256;
+// stencil directions leaving a tile through its -x / +x / -y / +y / -z / +z face
+constexpr unsigned kFaceXN = 0x1442u, kFaceXP = 0x2881u, kFaceYN = 0x2288u, kFaceYP = 0x1144u,
+                   kFaceZN = 0x2A20u, kFaceZP = 0x1510u;
 
+template <bool FAST, bool TYPE>
 __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int* ds, int* as,
                                                      uint16_t* lower, uint8_t* pdir,
                                                      const uint8_t* lut, uint8_t* type) {
   __shared__ double ring[3][kPN];
   __shared__ int sup[kLTN];
   __shared__ int sdn[kLTN];
+  __shared__ int s_goff[kNbr], s_loff[kNbr];  // grid and in-tile offset of each direction
   const int ntx = (g.W + kLTX - 1) / kLTX, ntz = (g.D + kLTZ - 1) / kLTZ;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   // z fastest: the consecutive tiles of an XCD stack along z and share their
@@ -648,33 +661,54 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
   const int col = L / ntz;
   const int x0 = (col % ntx) * kLTX, y0 = (col / ntx) * kLTY, z0 = (L % ntz) * kLTZ;
   const int lx = threadIdx.x, ly = threadIdx.y, tid = lx + 64 * ly;
-  const bool ordered = g.W >= 3;
+  if (tid == 0) {
+#pragma unroll
+    for (int d = 0; d < kNbr; d++) {
+      s_goff[d] = g.off[d];
+      s_loff[d] = kSX[d] + kLTX * (kSY[d] + kLTY * kSZ[d]);
+    }
+  }
+  const double miss = __longlong_as_double(0x7ff8000000000000LL);  // quiet NaN
+  const int lut0 = TYPE ? c_lut_off[0] : 0;  // the interior class's LUT segment
+  // this thread's halo slots of a plane: fixed offsets within every plane
+  int foff[kPPer];
+  bool fin[kPPer];
+#pragma unroll
+  for (int j = 0; j < kPPer; j++) {
+    const int i = tid + 256 * j;
+    const int px = i % kPX, py = i / kPX, gx = x0 + px - 1, gy = y0 + py - 1;
+    fin[j] = i < kPN && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H;
+    foff[j] = gx + g.W * gy;
+  }
   double pre[kPPer];
   auto fetch = [&](int z) {
+    const bool zin = z >= 0 && z < g.D;
+    const long long zo = (long long)z * g.WH;
 #pragma unroll
-    for (int j = 0; j < kPPer; j++) {
-      int i = tid + 256 * j;
-      double val = 0.0;
-      if (i < kPN) {
-        int px = i % kPX, py = i / kPX, gx = x0 + px - 1, gy = y0 + py - 1;
-        if (z >= 0 && z < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
-          val = f[gx + g.W * (gy + g.H * z)];
-      }
-      pre[j] = val;
-    }
+    for (int j = 0; j < kPPer; j++)
+      pre[j] = (fin[j] && zin) ? f[foff[j] + zo] : (FAST ? miss : 0.0);
   };
   auto store = [&](int slot) {
 #pragma unroll
     for (int j = 0; j < kPPer; j++) {
-      int i = tid + 256 * j;
+      const int i = tid + 256 * j;
       if (i < kPN) ring[slot][i] = pre[j];
     }
   };
+  // directions that stay inside the tile in x and y (z: per plane below)
+  unsigned in_xy = (1u << kNbr) - 1;
+  if (lx == 0) in_xy &= ~kFaceXN;
+  if (lx == kLTX - 1) in_xy &= ~kFaceXP;
+  if (ly == 0) in_xy &= ~kFaceYN;
+  if (ly == kLTY - 1) in_xy &= ~kFaceYP;
+  const int x = x0 + lx, y = y0 + ly;
+  const bool xy_in = x < g.W && y < g.H;
   fetch(z0 - 1);
   store(0);
   fetch(z0);
   store(1);
   fetch(z0 + 1);
+#pragma unroll
   for (int lz = 0; lz < kLTZ; lz++) {
     store((lz + 2) % 3);
     __syncthreads();
@@ -682,45 +716,55 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
     const double* P0 = ring[lz % 3];        // z - 1
     const double* P1 = ring[(lz + 1) % 3];  // z
     const double* P2 = ring[(lz + 2) % 3];  // z + 1
-    const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+    const int z = z0 + lz;
     const int li = lx + kLTX * (ly + kLTY * lz);
     int pu = li, pd = li;
-    if (x < g.W && y < g.H && z < g.D) {
+    if (xy_in && z < g.D) {
       const int v = x + g.W * (y + g.H * z);
-      const int cls = class_of(g, x, y, z);
-      const int key = c_cls_key[cls];
       const int c0 = (lx + 1) + kPX * (ly + 1);
       const double c = P1[c0];
-      int du = -1, dd = -1;
+      int du = -1, dd = -1, cls = 0;
       unsigned lm = 0;
-      if (ordered && key == 0x3FFF) {
+      if (FAST) {
         const double n0 = P1[c0 + 1], n1 = P1[c0 - 1], n2 = P1[c0 + kPX], n3 = P1[c0 - kPX];
         const double n4 = P2[c0], n5 = P0[c0], n6 = P1[c0 - 1 + kPX], n7 = P1[c0 + 1 - kPX];
         const double n8 = P2[c0 + kPX], n9 = P0[c0 - kPX], n10 = P2[c0 - 1], n11 = P0[c0 + 1];
         const double n12 = P2[c0 - 1 + kPX], n13 = P0[c0 + 1 - kPX];
-        // lower: neighbours with a smaller id count on ties
+        // lower: neighbours with a smaller id (odd directions) count on ties
         lm = (unsigned)(n0 < c) | (unsigned)(n1 <= c) << 1 | (unsigned)(n2 < c) << 2 |
              (unsigned)(n3 <= c) << 3 | (unsigned)(n4 < c) << 4 | (unsigned)(n5 <= c) << 5 |
              (unsigned)(n6 < c) << 6 | (unsigned)(n7 <= c) << 7 | (unsigned)(n8 < c) << 8 |
              (unsigned)(n9 <= c) << 9 | (unsigned)(n10 < c) << 10 | (unsigned)(n11 <= c) << 11 |
              (unsigned)(n12 < c) << 12 | (unsigned)(n13 <= c) << 13;
-        double bu = n9, bd = n9;
-        du = 9;
-        dd = 9;
-#define EZ_SCAN(val, dir)      \
-  if ((val) >= bu) {           \
-    bu = (val);                \
-    du = (dir);                \
-  }                            \
-  if ((val) < bd) {            \
-    bd = (val);                \
-    dd = (dir);                \
+        double bu = c, bd = c;
+#define EZ_LO(val, dir)   \
+  if ((val) > bu) {       \
+    bu = (val);           \
+    du = (dir);           \
+  }                       \
+  if ((val) <= bd) {      \
+    bd = (val);           \
+    dd = (dir);           \
   }
-        EZ_SCAN(n13, 13) EZ_SCAN(n5, 5) EZ_SCAN(n11, 11) EZ_SCAN(n3, 3) EZ_SCAN(n7, 7)
-        EZ_SCAN(n1, 1) EZ_SCAN(c, -1) EZ_SCAN(n0, 0) EZ_SCAN(n6, 6) EZ_SCAN(n2, 2)
-        EZ_SCAN(n10, 10) EZ_SCAN(n4, 4) EZ_SCAN(n12, 12) EZ_SCAN(n8, 8)
-#undef EZ_SCAN
+#define EZ_HI(val, dir)   \
+  if ((val) >= bu) {      \
+    bu = (val);           \
+    du = (dir);           \
+  }                       \
+  if ((val) < bd) {       \
+    bd = (val);           \
+    dd = (dir);           \
+  }
+        EZ_LO(n1, 1) EZ_LO(n7, 7) EZ_LO(n3, 3) EZ_LO(n11, 11) EZ_LO(n5, 5) EZ_LO(n13, 13)
+        EZ_LO(n9, 9)
+        EZ_HI(n0, 0) EZ_HI(n6, 6) EZ_HI(n2, 2) EZ_HI(n10, 10) EZ_HI(n4, 4) EZ_HI(n12, 12)
+        EZ_HI(n8, 8)
+#undef EZ_LO
+#undef EZ_HI
+        if (TYPE) cls = class_of(g, x, y, z);
       } else {
+        cls = class_of(g, x, y, z);
+        const int key = c_cls_key[cls];
         int up = v, dn = v;
         double fu = c, fd = c;
 #pragma unroll
@@ -735,18 +779,22 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
         }
       }
       if (lower) lower[v] = (uint16_t)lm;
-      if (type) type[v] = (uint8_t)type_of_mask(lut, cls, lm);  // classifyVertex
+      if (TYPE) {  // classifyVertex
+        int t;
+        if (cls == 0) {  // interior: the upper config is the complement of the lower mask
+          const unsigned cfg = ~lm & ((1u << kNbr) - 1);
+          t = cfg == 0 ? 4 : (cfg == (1u << kNbr) - 1 ? 0 : lut[lut0 + cfg]);
+        } else {
+          t = type_of_mask(lut, cls, lm);
+        }
+        type[v] = (uint8_t)t;
+      }
       if (pdir) pdir[v] = (uint8_t)((du + 1) | ((dd + 1) << 4));
-      if (du >= 0) {
-        int ux = lx + c_dx[du], uy = ly + c_dy[du], uz = lz + c_dz[du];
-        bool inside = ux >= 0 && ux < kLTX && uy >= 0 && uy < kLTY && uz >= 0 && uz < kLTZ;
-        pu = inside ? ux + kLTX * (uy + kLTY * uz) : ~(v + g.off[du]);
-      }
-      if (dd >= 0) {
-        int ux = lx + c_dx[dd], uy = ly + c_dy[dd], uz = lz + c_dz[dd];
-        bool inside = ux >= 0 && ux < kLTX && uy >= 0 && uy < kLTY && uz >= 0 && uz < kLTZ;
-        pd = inside ? ux + kLTX * (uy + kLTY * uz) : ~(v + g.off[dd]);
-      }
+      unsigned inside = in_xy;
+      if (lz == 0) inside &= ~kFaceZN;
+      if (lz == kLTZ - 1) inside &= ~kFaceZP;
+      if (du >= 0) pu = ((inside >> du) & 1) ? li + s_loff[du] : ~(v + s_goff[du]);
+      if (dd >= 0) pd = ((inside >> dd) & 1) ? li + s_loff[dd] : ~(v + s_goff[dd]);
     }
     sup[li] = pu;
     sdn[li] = pd;
@@ -773,20 +821,31 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
     sdn[li] = p >= 0 ? c : p;
   }
   __syncthreads();
+  // a root inside the tile is final as written; a pointer out of the tile
+  // carries the top bit, and only those are left to k_chase
+  const int vb = x0 + g.W * (y0 + g.H * z0);
 #pragma unroll
   for (int k = 0; k < kLTN / 256; k++) {
-    int li = tid + 256 * k;
-    int x = x0 + li % kLTX, y = y0 + (li / kLTX) % kLTY, z = z0 + li / (kLTX * kLTY);
-    if (x >= g.W || y >= g.H || z >= g.D) continue;
-    int v = x + g.W * (y + g.H * z);
-    // a root inside the tile is final as written; a pointer out of the tile
-    // carries the top bit, and only those are left to k_chase
+    const int li = tid + 256 * k;
+    const int xx = li % kLTX, yy = (li / kLTX) % kLTY, zz = li / (kLTX * kLTY);
+    if (x0 + xx >= g.W || y0 + yy >= g.H || z0 + zz >= g.D) continue;
+    const int v = vb + xx + g.W * (yy + g.H * zz);
     int p = sup[li];
-    ds[v] = p >= 0 ? (x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY))) : kExit | ~p;
+    ds[v] = p >= 0 ? vb + (p % kLTX) + g.W * ((p / kLTX) % kLTY + g.H * (p / (kLTX * kLTY)))
+                   : kExit | ~p;
     p = sdn[li];
-    as[v] = p >= 0 ? (x0 + p % kLTX) + g.W * ((y0 + (p / kLTX) % kLTY) + g.H * (z0 + p / (kLTX * kLTY))) : kExit | ~p;
+    as[v] = p >= 0 ? vb + (p % kLTX) + g.W * ((p / kLTX) % kLTY + g.H * (p / (kLTX * kLTY)))
+                   : kExit | ~p;
   }
 }
+template __global__ void k_labels_tile<true, true>(Geo, const double*, int*, int*, uint16_t*,
+                                                   uint8_t*, const uint8_t*, uint8_t*);
+template __global__ void k_labels_tile<true, false>(Geo, const double*, int*, int*, uint16_t*,
+                                                    uint8_t*, const uint8_t*, uint8_t*);
+template __global__ void k_labels_tile<false, true>(Geo, const double*, int*, int*, uint16_t*,
+                                                    uint8_t*, const uint8_t*, uint8_t*);
+template __global__ void k_labels_tile<false, false>(Geo, const double*, int*, int*, uint16_t*,
+                                                     uint8_t*, const uint8_t*, uint8_t*);
 
 // Full-grid lower masks through the same LDS ring as k_labels_tile: a block
 // marches one 64x4 column through zlen planes (z chunk zc of the column), so
