@@ -289,8 +289,9 @@ class Run {
   void check_full(int* fbuf, int fctr) {
     const ColGrid cg = col_grid();
     tbegin(EXTREMZ_K_CHECK_FULL);
-    k_mask_cols<1><<<cg.blocks, dim3(64, 4, 1), 0, st()>>>(s_, s_.decp, nullptr, nullptr, fbuf,
-                                                            fctr, cg.nzc, cg.zlen);
+    auto* kern = s_.g.planar ? k_mask_cols<1, false> : k_mask_cols<1, true>;
+    kern<<<cg.blocks, dim3(64, 4, 1), 0, st()>>>(s_, s_.decp, nullptr, nullptr, fbuf, fctr,
+                                                 cg.nzc, cg.zlen);
     tend();
   }
   // the same scan with the frontier's proposals made (delta and the touched
@@ -298,7 +299,8 @@ class Run {
   void scan_propose(int* fbuf, int fctr) {
     const ColGrid cg = col_grid();
     tbegin(EXTREMZ_K_SCAN_PROPOSE);
-    k_scan_propose<<<cg.blocks, dim3(64, 4, 1), 0, st()>>>(s_, fbuf, fctr, cg.nzc, cg.zlen);
+    auto* kern = s_.g.planar ? k_scan_propose<false> : k_scan_propose<true>;
+    kern<<<cg.blocks, dim3(64, 4, 1), 0, st()>>>(s_, fbuf, fctr, cg.nzc, cg.zlen);
     tend();
   }
   // 64x4xkLabelTileZ label tiles

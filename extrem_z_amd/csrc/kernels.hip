@@ -856,7 +856,7 @@ template __global__ void k_labels_tile<false, false>(Geo, const double*, int*, i
 // false-critical-point scan of get_false_criticle_points (src/ExtreM.cpp:
 // 1513-1519), appending to the frontier, one append per block; the
 // original field's classification comes out of its label pass.
-template <int MODE>
+template <int MODE, bool FAST>
 __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_t* type,
                                                    uint16_t* lower, int* front, int fctr,
                                                    int nzc, int zlen) {
@@ -874,7 +874,9 @@ __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_
 #pragma unroll
     for (int j = 0; j < kPPer; j++) {
       int i = tid + 256 * j;
-      double val = 0.0;
+      // FAST: a neighbour outside the field reads NaN, so every compare with it
+      // is false and no class key is needed to leave it out
+      double val = FAST ? __longlong_as_double(0x7ff8000000000000LL) : 0.0;
       if (i < kPN) {
         int px = i % kPX, py = i / kPX, gx = x0 + px - 1, gy = y0 + py - 1;
         if (z >= 0 && z < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
@@ -909,8 +911,7 @@ __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_
     const double* P2 = ring[(lz + 2) % 3];  // z + 1
     if (x < g.W && y < g.H) {
       const int v = x + g.W * (y + g.H * z);
-      const int cls = class_of(g, x, y, z);
-      const unsigned key = c_cls_key[cls];
+      const unsigned key = FAST ? (1u << kNbr) - 1 : c_cls_key[class_of(g, x, y, z)];
       const double c = P1[c0];
       unsigned lm = 0;
 #pragma unroll
@@ -922,15 +923,18 @@ __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_
       }
       if (lm != s.olow[v]) {
         const int ot = s.otype[v];
-        if (type_of_mask(s.lut, cls, lm) != ot || is_saddle_type(ot)) fl.push(v);
+        if (type_of_mask(s.lut, class_of(g, x, y, z), lm) != ot || is_saddle_type(ot))
+          fl.push(v);
       }
     }
     __syncthreads();
   }
   fl.flush();
 }
-template __global__ void k_mask_cols<1>(St, const double*, uint8_t*, uint16_t*, int*, int, int,
-                                        int);
+template __global__ void k_mask_cols<1, true>(St, const double*, uint8_t*, uint16_t*, int*, int,
+                                              int, int);
+template __global__ void k_mask_cols<1, false>(St, const double*, uint8_t*, uint16_t*, int*, int,
+                                               int, int);
 
 // The full false-critical-point scan of decp (k_mask_cols<1>) fused with the
 // c_loops proposals of the frontier it finds (k_propose_c over that list).
@@ -960,13 +964,17 @@ constexpr int kQPer = (kQN + 255) / 256;
 
 // Outgoing proposals of one voxel from LDS planes of decp around it (Q0..Q2,
 // centre qc) and its original lower mask / type.
+// With FAST (the stencil adjacency) the ring holds NaN outside the field, so
+// the lower mask needs no class key; the key is looked up only for the few
+// voxels whose mask differs from the original one.
+template <bool FAST>
 __device__ __forceinline__ unsigned out_proposals(const St& s, const double* Q0, const double* Q1,
                                                   const double* Q2, int qc, int x, int y, int z,
                                                   unsigned ol, int ot) {
   const Geo& g = s.g;
   const int v = x + g.W * (y + g.H * z);
-  const int cls = class_of(g, x, y, z);
-  const unsigned key = c_cls_key[cls];
+  const int cls0 = FAST ? 0 : class_of(g, x, y, z);
+  const unsigned key0 = FAST ? (1u << kNbr) - 1 : c_cls_key[cls0];
   const double c = Q1[qc];
   double fr[kNbr];
   unsigned dl = 0;
@@ -975,9 +983,11 @@ __device__ __forceinline__ unsigned out_proposals(const St& s, const double* Q0,
     const double* Q = kSZ[d] < 0 ? Q0 : (kSZ[d] > 0 ? Q2 : Q1);
     fr[d] = Q[qc + kSX[d] + kQX * kSY[d]];
     const bool lo = fr[d] < c || ((d & 1) && fr[d] == c);
-    dl |= (unsigned)(((key >> d) & 1) && lo) << d;
+    dl |= (unsigned)(((key0 >> d) & 1) && lo) << d;
   }
   if (dl == ol) return 0;
+  const int cls = FAST ? class_of(g, x, y, z) : cls0;
+  const unsigned key = FAST ? c_cls_key[cls] : key0;
   const int dt = type_of_mask(s.lut, cls, dl);
   if (!(dt != ot || is_saddle_type(ot))) return 0;
   unsigned w = 1u << 30;
@@ -1040,6 +1050,7 @@ __device__ __forceinline__ unsigned out_proposals(const St& s, const double* Q0,
 // ring slot (j - z0 + 3) % 5, the words of plane j in sring slot
 // (j - z0 + 1) % 4; each slot is rewritten only a barrier after its last
 // reader.
+template <bool FAST>
 __global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr, int nzc,
                                                       int zlen) {
   __shared__ double ring[5][kQN];
@@ -1065,7 +1076,7 @@ __global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr
 #pragma unroll
     for (int j = 0; j < kQPer; j++) {
       const int i = tid + 256 * j;
-      double a = 0.0;
+      double a = FAST ? __longlong_as_double(0x7ff8000000000000LL) : 0.0;  // outside: NaN
       if (i < kQN) {
         const int gx = x0 + i % kQX - 2, gy = y0 + i / kQX - 2;
         if (z >= 0 && z < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
@@ -1138,8 +1149,8 @@ __global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr
           const int sx = i % kPX, sy = i / kPX, gx = x0 + sx - 1, gy = y0 + sy - 1;
           unsigned w = 0;
           if (p >= 0 && p < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
-            w = out_proposals(s, Q0, Q1, Q2, (sx + 1) + kQX * (sy + 1), gx, gy, p,
-                              j ? ol1 : ol0, j ? ot1 : ot0);
+            w = out_proposals<FAST>(s, Q0, Q1, Q2, (sx + 1) + kQX * (sy + 1), gx, gy, p,
+                                    j ? ol1 : ol0, j ? ot1 : ot0);
           sring[k % 4][i] = w;
         }
       }
@@ -1182,6 +1193,8 @@ __global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr
   fl.flush();
   tl.flush();
 }
+template __global__ void k_scan_propose<true>(St, int*, int, int, int);
+template __global__ void k_scan_propose<false>(St, int*, int, int, int);
 
 // Follows every pointer out of a tile to its root (D7: every vertex is
 // compressed).  A label without the top bit (kExit) is a root: found inside
