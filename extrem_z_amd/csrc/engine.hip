@@ -328,6 +328,8 @@ class Run {
   static constexpr int kWnRounds = 4;
   // Listed sides up to which the wrong-index phases run in one block.
   static constexpr int kWrongOne = 4096;
+  // Blocks of the first stage of the decp extremum reduction.
+  static constexpr int kExtBlocks = 2048;
   static int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     return (int)std::max<long long>(1, std::min<long long>(b, 2048));
@@ -434,8 +436,8 @@ void Run::allocate(long long N) {
     b->ensure(N);
   for (DevBuf<int>* b : {&c->lchg, &c->spos2, &c->spos1, &c->qu, &c->qd}) b->ensure(N);
   c->ilist.ensure(5 * inc_cap(N));
-  c->bmax2.ensure(256);
-  c->bmin2.ensure(256);
+  c->bmax2.ensure(kExtBlocks);
+  c->bmin2.ensure(kExtBlocks);
   c->act0.ensure(2 * N);
   c->actA.ensure(2 * N);
   c->actB.ensure(2 * N);
@@ -534,10 +536,11 @@ void Run::slot_pass(int dec, const double* f, const int* L2, const int* L1, cons
                                                                  kmax, kmin, smax, smin);
   if (dec) {
     // decp global maximum / minimum among the listed extrema
-    k_ext_reduce<<<256, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, f, c_->bmax2.p,
-                                        c_->bmin2.p);
-    k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, 256, c_->bmin2.p, 256, f, s_.ctr + C_DGMAX,
-                                      s_.ctr + C_DGMIN);
+    // the value gathers are scattered: one short stride per lane over kExtBlocks blocks
+    k_ext_reduce<<<kExtBlocks, 256, 0, st()>>>(s_.MX, s_.nmax, s_.MN, s_.nmin, f, c_->bmax2.p,
+                                               c_->bmin2.p);
+    k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, kExtBlocks, c_->bmin2.p, kExtBlocks, f,
+                                      s_.ctr + C_DGMAX, s_.ctr + C_DGMIN);
   }
   tend();
 }
