@@ -333,6 +333,8 @@ __global__ void k_inc_saddles(St s, const double* f, int2* pairs, int* pcnt, lon
 __global__ void k_inc_pull(St s, long long* pkey, int* pid);
 __global__ void k_inc_clear(St s);
 __global__ void k_inc_end_ctr(int* ctr);
+__global__ void k_inc_one(St s, int epoch, long long* kmax, long long* kmin, int2* pairs,
+                          int* pcnt, long long* pkey, int* pid, int end);
 __global__ void k_diff_count(const void* a, const void* b, long long nbytes, int* out);
 __global__ void k_act_copy(St s, int2* dst, int dctr);
 __global__ void k_wn_register(St s, int2* active, int actr, int nctr);

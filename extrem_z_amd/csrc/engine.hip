@@ -330,6 +330,10 @@ class Run {
   static constexpr int kWrongOne = 4096;
   // Blocks of the first stage of the decp extremum reduction.
   static constexpr int kExtBlocks = 2048;
+  // Changed vertices up to which an incremental pass runs in one block (its
+  // lists then hold a few thousand entries at most).
+  static constexpr long long kIncOne = 64;
+  bool ended_ = false;  // the last CP pass cleared its own state (cp_end has no launch)
   static int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     return (int)std::max<long long>(1, std::min<long long>(b, 2048));
@@ -688,6 +692,16 @@ void Run::cp_inc(long long nch) {
   tbegin(EXTREMZ_K_INCREMENTAL);
   long long* kmax = c_->kmax.p;
   long long* kmin = c_->kmin.p;
+  if (nch <= kIncOne && !slots_all_) {
+    // few changes: the whole pass (and, unless it is checked next, its end)
+    // in one block
+    ended_ = !inc_verify_;
+    k_inc_one<<<1, 1024, 0, st()>>>(s_, epoch_, kmax, kmin, c_->pairs.p, c_->pcnt.p, c_->pkey.p,
+                                    c_->pid.p, ended_ ? 1 : 0);
+    tend();
+    if (inc_verify_) verify_inc();
+    return;
+  }
   k_inc_ptrs<<<b1, 256, 0, st()>>>(s_, epoch_);
   k_inc_taint<<<b1, 256, 0, st()>>>(s_);
   k_inc_walk<<<std::min(b1, 512), 256, 0, st()>>>(s_);
@@ -714,8 +728,11 @@ void Run::cp_inc(long long nch) {
 // End of every CP pass of decp: the pass's flags are cleared, the changed
 // sides become the last pass's, and the change list starts empty.
 void Run::cp_end() {
-  k_inc_clear<<<blocks_for(15 * (long long)s_.icap), 256, 0, st()>>>(s_);
-  k_inc_end_ctr<<<1, 64, 0, st()>>>(s_.ctr);
+  if (!ended_) {
+    k_inc_clear<<<blocks_for(15 * (long long)s_.icap), 256, 0, st()>>>(s_);
+    k_inc_end_ctr<<<1, 64, 0, st()>>>(s_.ctr);
+  }
+  ended_ = false;
   std::swap(s_.cl2, s_.cln2);
   std::swap(s_.cl1, s_.cln1);
   std::swap(s_.pprev, s_.pnext);

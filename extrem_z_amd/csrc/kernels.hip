@@ -1263,8 +1263,8 @@ __device__ __forceinline__ int wave_best(int cand, double& fc, bool want_max) {
 __device__ __forceinline__ long long okey_c(double d) { return okey(d + 0.0); }
 
 // Block-wide best (value, id) of one candidate per thread; returns it on
-// thread 0.  All threads of the 256-thread block must call; s_best is four
-// ints of the calling kernel's LDS.
+// thread 0.  All threads of the block must call; s_best is one int per
+// wavefront of the block, in the calling kernel's LDS.
 __device__ __forceinline__ int block_best(int cand, double fc, const double* f, bool want_max,
                                           int* s_best) {
   cand = wave_best(cand, fc, want_max);
@@ -1274,7 +1274,8 @@ __device__ __forceinline__ int block_best(int cand, double fc, const double* f, 
   int best = kNone;
   if (tid == 0) {
     double fb = 0.0;
-    for (int w = 0; w < 4; w++) {
+    const int nw = (blockDim.x * blockDim.y) >> 6;  // s_best holds one entry per wavefront
+    for (int w = 0; w < nw; w++) {
       int c = s_best[w];
       if (c == kNone) continue;
       double fv = f[c];
@@ -1930,10 +1931,9 @@ __global__ void __launch_bounds__(256) k_ext_reduce(const int* inmax, int nmax, 
 // for a maximum's slot, lowest for a minimum's), giving the (value, id) order
 // of computelargestSaddlesForMax / computesmallestSaddlesForMin.  One
 // wavefront per chunk region of the last saddle launch.
-__global__ void __launch_bounds__(256) k_slot_ids(St s, const int2* pairs, const int* pcnt,
-                                                  const double* f, const long long* kmax,
-                                                  const long long* kmin, int* slot_max,
-                                                  int* slot_min) {
+__device__ __forceinline__ void slot_ids_body(const St& s, const int2* pairs, const int* pcnt,
+                                              const double* f, const long long* kmax,
+                                              const long long* kmin, int* slot_max, int* slot_min) {
   const int nc = s.ctr[C_PCHUNK];
   const int lane = threadIdx.x & 63;
   const int nw = (gridDim.x * blockDim.x) >> 6;
@@ -1951,6 +1951,12 @@ __global__ void __launch_bounds__(256) k_slot_ids(St s, const int2* pairs, const
       }
     }
   }
+}
+__global__ void __launch_bounds__(256) k_slot_ids(St s, const int2* pairs, const int* pcnt,
+                                                  const double* f, const long long* kmax,
+                                                  const long long* kmin, int* slot_max,
+                                                  int* slot_min) {
+  slot_ids_body(s, pairs, pcnt, f, kmax, kmin, slot_max, slot_min);
 }
 
 // ext_id[v] = 2 * (position in MX) for maxima, 2 * (position in MN) + 1 for
@@ -2017,7 +2023,7 @@ __global__ void k_spos(St s) {
 // 1 + d its neighbour in direction d; each vertex once per epoch.  New lower
 // masks and steepest directions (the order of k_labels_tile); lists the
 // vertices whose mask (al), up pointer (cu) or down pointer (cd) changed.
-__global__ void __launch_bounds__(256) k_inc_ptrs(St s, int epoch) {
+__device__ __forceinline__ void inc_ptrs_body(const St& s, int epoch) {
   STAGED_LIST(la, int, 512, s.al, &s.ctr[C_AL]);
   STAGED_LIST(lu, int, 512, s.cu, &s.ctr[C_CU]);
   STAGED_LIST(ld, int, 512, s.cd, &s.ctr[C_CD]);
@@ -2075,13 +2081,16 @@ __global__ void __launch_bounds__(256) k_inc_ptrs(St s, int epoch) {
   lu.flush();
   ld.flush();
 }
+__global__ void __launch_bounds__(256) k_inc_ptrs(St s, int epoch) {
+  inc_ptrs_body(s, epoch);
+}
 
 // The old roots of the changed pointers (kept in ou / od, flagged in tu / td).
 // Also: a listed extremum other than the current global one changed value,
 // so the global maximum / minimum of decp is the better of it and the current
 // one (when the current one changed, k_inc_ptrs set C_EXTD, both are cleared
 // here and k_inc_mark recomputes them).
-__global__ void k_inc_taint(St s) {
+__device__ __forceinline__ void inc_taint_body(const St& s) {
   if (s.ctr[C_EXTD]) {  // k_inc_mark recomputes them from scratch
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       s.ctr[C_DGMAX] = kNone;
@@ -2109,6 +2118,9 @@ __global__ void k_inc_taint(St s) {
     s.td[r] = 1;
   }
 }
+__global__ void k_inc_taint(St s) {
+  inc_taint_body(s);
+}
 
 // New root of each changed pointer, written to its root array; the pointers
 // whose root changed start the propagation queues (qu / qd).  A path strictly
@@ -2116,7 +2128,7 @@ __global__ void k_inc_taint(St s) {
 // array entry read on the way is either the old root (tainted when the
 // vertex's path holds a changed pointer, so the walk goes on) or the final new
 // root of a changed pointer, so either answer it gives is exact.
-__global__ void __launch_bounds__(256) k_inc_walk(St s) {
+__device__ __forceinline__ void inc_walk_body(const St& s) {
   STAGED_LIST(su, int, 512, s.qu, &s.ctr[C_QU]);
   STAGED_LIST(sd, int, 512, s.qd, &s.ctr[C_QD]);
   const int nu = s.ctr[C_CU], n = nu + s.ctr[C_CD];
@@ -2159,6 +2171,9 @@ __global__ void __launch_bounds__(256) k_inc_walk(St s) {
       s.ctr[C_SWEEP] = 0;
     }
   }
+}
+__global__ void __launch_bounds__(256) k_inc_walk(St s) {
+  inc_walk_body(s);
 }
 
 // The children of u (neighbours whose pointer of this side names u) that are
@@ -2240,7 +2255,7 @@ __global__ void __launch_bounds__(256) k_inc_sweep(St s) {
 // [C_QLx, C_QHx) to an empty level.  Every lane of the block walks the same
 // rounds (the queue appends are wavefront collectives).  qu / qd end up
 // listing every vertex whose root changed.
-__global__ void __launch_bounds__(1024) k_inc_prop(St s, int qcap) {
+__device__ __forceinline__ void inc_prop_body(const St& s, int qcap) {
   __shared__ int qn;
   const int tid = threadIdx.x;
   for (int side = 0; side < 2; side++) {
@@ -2285,6 +2300,9 @@ __global__ void __launch_bounds__(1024) k_inc_prop(St s, int qcap) {
     __syncthreads();
   }
 }
+__global__ void __launch_bounds__(1024) k_inc_prop(St s, int qcap) {
+  inc_prop_body(s, qcap);
+}
 
 // Queues side q2 of S2 (two) or S1 for the saddle recompute, once per pass.
 template <class L>
@@ -2301,8 +2319,8 @@ __device__ __forceinline__ void mark_side(const St& s, int q2, bool two, L& l2, 
 // chunks of k_inc_mark_csr, empty slot keys and slots for this pass's pushes
 // (as k_slot_init; no kernel before the pushes reads them), and the decp
 // global extrema when k_inc_taint cleared them.
-__global__ void __launch_bounds__(256) k_inc_mark(St s, long long* kmax, long long* kmin,
-                                                  int all_slots) {
+__device__ __forceinline__ void inc_mark_body(const St& s, long long* kmax, long long* kmin,
+                                              int all_slots) {
   if (s.ctr[C_IBAIL]) return;
   // the global maximum / minimum of decp over the lists, when the current one
   // changed value (block bests into the cleared counters; the block barriers
@@ -2322,7 +2340,7 @@ __global__ void __launch_bounds__(256) k_inc_mark(St s, long long* kmax, long lo
       const double fb = s.decp[b];
       if (cmin == kNone || smaller(fb, b, fmin, cmin)) { cmin = b; fmin = fb; }
     }
-    __shared__ int s_best[4];
+    __shared__ int s_best[16];  // blocks of up to 1024 threads
     const int gm = block_best(cmax, fmax, s.decp, true, s_best);
     const int gn = block_best(cmin, fmin, s.decp, false, s_best);
     if (threadIdx.x == 0) {
@@ -2428,16 +2446,19 @@ __global__ void __launch_bounds__(256) k_inc_mark(St s, long long* kmax, long lo
     s.ctr[C_DSLOT] = 0;
   }
 }
+__global__ void __launch_bounds__(256) k_inc_mark(St s, long long* kmax, long long* kmin,
+                                                  int all_slots) {
+  inc_mark_body(s, kmax, kmin, all_slots);
+}
 
 // Slot merge of an incremental pass: every slot (all_slots, the first pass
 // after a full one), else the slots pushed by the last pass (reset by
 // k_inc_mark) and the gathered-again ones; always the slots of this pass's
 // pair records, which are listed (deduplicated) for the next pass.  Other
 // slots keep last pass's keys and slots: nothing they merge changed.
-__global__ void __launch_bounds__(256) k_inc_merge(St s, long long* kmax, long long* kmin,
-                                                   const long long* pkey, const int* pid,
-                                                   const int2* pairs, const int* pcnt,
-                                                   int all_slots) {
+__device__ __forceinline__ void inc_merge_body(const St& s, long long* kmax, long long* kmin,
+                                               const long long* pkey, const int* pid,
+                                               const int2* pairs, const int* pcnt, int all_slots) {
   STAGED_LIST(pl, int, 1024, s.pnext, &s.ctr[C_PNEXT]);
   if (all_slots) {
     GRID_STRIDE(j, s.nmax + s.nmin)
@@ -2462,6 +2483,12 @@ __global__ void __launch_bounds__(256) k_inc_merge(St s, long long* kmax, long l
   }
   pl.flush();
 }
+__global__ void __launch_bounds__(256) k_inc_merge(St s, long long* kmax, long long* kmin,
+                                                   const long long* pkey, const int* pid,
+                                                   const int2* pairs, const int* pcnt,
+                                                   int all_slots) {
+  inc_merge_body(s, kmax, kmin, pkey, pid, pairs, pcnt, all_slots);
+}
 
 // The saddles adjacent in the original to a changed vertex that is a listed
 // extremum and still a root of decp compare its value, so their sides are
@@ -2469,7 +2496,7 @@ __global__ void __launch_bounds__(256) k_inc_merge(St s, long long* kmax, long l
 // queued as last pass's changed sides or by a root change next to them.
 // k_inc_mark cuts those extrema's index ranges into chunks of 1024,
 // k_inc_mark_csr queues one chunk per wavefront.
-__global__ void __launch_bounds__(256) k_inc_mark_csr(St s) {
+__device__ __forceinline__ void inc_mark_csr_body(const St& s) {
   if (s.ctr[C_IBAIL]) return;
   STAGED_LIST(l2, int, 1024, s.dsl2, &s.ctr[C_DSL2]);
   STAGED_LIST(l1, int, 1024, s.dsl1, &s.ctr[C_DSL1]);
@@ -2483,44 +2510,53 @@ __global__ void __launch_bounds__(256) k_inc_mark_csr(St s) {
   l2.flush();
   l1.flush();
 }
+__global__ void __launch_bounds__(256) k_inc_mark_csr(St s) {
+  inc_mark_csr_body(s);
+}
 
 // The queued sides through saddle_entry (kModeDec, with slot re-gather
 // marking), in chunks of 256 queued sides (S2 sides first, each part rounded
 // up to whole chunks), a contiguous range of chunks per block, slot pairs in
-// the chunks' own regions as in k_saddle_list.
-__global__ void __launch_bounds__(256, 4) k_inc_saddles(St s, const double* f, int2* pairs, int* pcnt,
-                                                     long long* kmax, long long* kmin) {
+// the chunks' own regions as in k_saddle_list.  A block of NT threads works
+// NT / 256 chunks of its range at a time, one per 256-thread group.
+template <int NT>
+__device__ __forceinline__ void inc_saddles_body(const St& s, const double* f, int2* pairs,
+                                                 int* pcnt, long long* kmax, long long* kmin) {
+  constexpr int kG = NT / 256;
   __shared__ int s_off[kNbr];
-  __shared__ int s_np;
+  __shared__ int s_np[kG];
   STAGED_LIST(sa, int2, 1024, s.act0, &s.ctr[C_ACT0]);
   STAGED_LIST(sc2, int, 1024, s.cln2, &s.ctr[C_CLN2]);
   STAGED_LIST(sc1, int, 1024, s.cln1, &s.ctr[C_CLN1]);
   STAGED_LIST(sw2, int, 1024, s.wl2, &s.ctr[C_WL2]);
   STAGED_LIST(sw1, int, 1024, s.wl1, &s.ctr[C_WL1]);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, grp = tid >> 8, gt = tid & 255;
   if (tid < kNbr) s_off[tid] = s.g.off[tid];
   const int n2 = s.ctr[C_DSL2], n1 = s.ctr[C_DSL1];
   const int nb2 = (n2 + 255) >> 8, nb = nb2 + ((n1 + 255) >> 8);
   if (blockIdx.x == 0 && tid == 0) s.ctr[C_PCHUNK] = nb;
   const int c0 = (int)((long long)blockIdx.x * nb / gridDim.x),
             c1 = (int)((long long)(blockIdx.x + 1) * nb / gridDim.x);
-  for (int c = c0; c < c1; c++) {
-    if (tid == 0) s_np = 0;
+  // every thread takes the same number of steps (the barriers)
+  for (int cb = c0; cb < c1; cb += kG) {
+    const int c = cb + grp;
+    const bool live = c < c1;
+    if (gt == 0) s_np[grp] = 0;
     __syncthreads();
-    int2* cp = pairs + (long long)c * kChunkPairs;
-    if (c < nb2) {
-      const int k = c * 256 + tid;
-      const bool act = k < n2;
+    int2* cp = pairs + (long long)(live ? c : c0) * kChunkPairs;
+    if (c < nb2) {  // uniform per group (a group's wavefronts share c)
+      const int k = c * 256 + gt;
+      const bool act = live && k < n2;
       saddle_entry<2>(s, kModeDec, f, s.dDS, s.dlow, s.df2, s.dl2, act ? s.dsl2[k] : 0, act,
-                      s_off, cp, &s_np, kmax, 1, sa, sc2, sw2);
+                      s_off, cp, &s_np[grp], kmax, 1, sa, sc2, sw2);
     } else {
-      const int k = (c - nb2) * 256 + tid;
-      const bool act = k < n1;
+      const int k = (c - nb2) * 256 + gt;
+      const bool act = live && k < n1;
       saddle_entry<1>(s, kModeDec, f, s.dAS, s.dlow, s.df1, s.dl1, act ? s.dsl1[k] : 0, act,
-                      s_off, cp, &s_np, kmin, 1, sa, sc1, sw1);
+                      s_off, cp, &s_np[grp], kmin, 1, sa, sc1, sw1);
     }
     __syncthreads();
-    if (tid == 0) pcnt[c] = s_np;
+    if (gt == 0 && live) pcnt[c] = s_np[grp];
   }
   sa.flush();
   sc2.flush();
@@ -2528,12 +2564,19 @@ __global__ void __launch_bounds__(256, 4) k_inc_saddles(St s, const double* f, i
   sw2.flush();
   sw1.flush();
 }
+__global__ void __launch_bounds__(256, 4) k_inc_saddles(St s, const double* f, int2* pairs,
+                                                        int* pcnt, long long* kmax,
+                                                        long long* kmin) {
+  inc_saddles_body<256>(s, f, pairs, pcnt, kmax, kmin);
+}
 
 // The queued slots' gathers: kPullGroup lanes per slot, a wavefront for a
 // slot of more than kPullShort adjacencies (taken after its block's short
-// ones; the loop's trip count is the same for every thread of a block).
-__global__ void __launch_bounds__(256) k_inc_pull(St s, long long* pkey, int* pid) {
-  constexpr int kPer = 256 / kPullGroup;  // slots per block and step
+// ones; the loop's trip count is the same for every thread of a block of NT
+// threads).
+template <int NT>
+__device__ __forceinline__ void inc_pull_body(const St& s, long long* pkey, int* pid) {
+  constexpr int kPer = NT / kPullGroup;  // slots per block and step
   __shared__ int lng[kPer];
   __shared__ int nl;
   if (threadIdx.x == 0) nl = 0;
@@ -2558,7 +2601,7 @@ __global__ void __launch_bounds__(256) k_inc_pull(St s, long long* pkey, int* pi
     if (mine && gl == 0) pull_store(s, j, mx, bk, bp, pkey, pid);
     __syncthreads();
     const int nlong = nl;
-    for (int q = threadIdx.x >> 6; q < nlong; q += 4) {  // a wavefront per long slot
+    for (int q = threadIdx.x >> 6; q < nlong; q += NT / 64) {  // a wavefront per long slot
       const int jj = lng[q];
       mx = jj < s.nmax;
       bk = 0;
@@ -2572,10 +2615,13 @@ __global__ void __launch_bounds__(256) k_inc_pull(St s, long long* pkey, int* pi
     __syncthreads();
   }
 }
+__global__ void __launch_bounds__(256) k_inc_pull(St s, long long* pkey, int* pid) {
+  inc_pull_body<256>(s, pkey, pid);
+}
 
 // End of every CP pass of decp: clears the pass's per-vertex flags through
 // its lists.
-__global__ void k_inc_clear(St s) {
+__device__ __forceinline__ void inc_clear_body(const St& s) {
   const int nu = s.ctr[C_CU], nd = s.ctr[C_CD], nl = s.ctr[C_LCHG];
   GRID_STRIDE(k, nu) {
     s.cflag[s.cu[k]] = 0;
@@ -2587,10 +2633,13 @@ __global__ void k_inc_clear(St s) {
   }
   GRID_STRIDE(k, nl) s.lflag[s.lchg[k]] = 0;
 }
+__global__ void k_inc_clear(St s) {
+  inc_clear_body(s);
+}
 
 // End of a CP pass of decp: the pass counters start at zero, the changed
 // sides and pushed slots become the last pass's, decp counts as unchanged.
-__global__ void k_inc_end_ctr(int* ctr) {
+__device__ __forceinline__ void inc_end_ctr_body(int* ctr) {
   for (int i = C_LCHG + (int)threadIdx.x; i <= C_SWEEP; i += blockDim.x) ctr[i] = 0;
   if (threadIdx.x == 0) {
     ctr[C_CL2] = ctr[C_CLN2];
@@ -2599,6 +2648,58 @@ __global__ void k_inc_end_ctr(int* ctr) {
     ctr[C_PNEXT] = 0;
     ctr[C_ANY_CHANGE] = 0;
   }
+}
+__global__ void k_inc_end_ctr(int* ctr) {
+  inc_end_ctr_body(ctr);
+}
+
+// A phase boundary inside one block whose phases hand lists and counters
+// to each other through global memory (all of it through this CU's L2): every
+// wavefront's loads, stores and atomics have completed before the barrier,
+// and the acquire after it drops the lines the CU's vector cache may hold
+// from before them (the counters are appended to with atomics, which that
+// cache does not see).
+__device__ __forceinline__ void block_phase() {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// The whole incremental CP pass in one block of 1024 threads, for passes of
+// few changed vertices (every list of such a pass holds a few thousand
+// entries at most): the phases of k_inc_ptrs ... k_slot_ids in order, the
+// root propagation entirely level by level in the block (no whole-grid
+// sweeps), and with end the end-of-pass clears of k_inc_clear and
+// k_inc_end_ctr.  Not for the first pass after a full one (every slot is
+// merged then).  Same results as the multi-block sequence: each phase is the
+// same code over the same lists, in the same order.
+__global__ void __launch_bounds__(1024) k_inc_one(St s, int epoch, long long* kmax,
+                                                  long long* kmin, int2* pairs, int* pcnt,
+                                                  long long* pkey, int* pid, int end) {
+  inc_ptrs_body(s, epoch);
+  block_phase();
+  inc_taint_body(s);
+  block_phase();
+  inc_walk_body(s);
+  block_phase();
+  inc_prop_body(s, s.g.N);
+  block_phase();
+  inc_mark_body(s, kmax, kmin, 0);
+  block_phase();
+  inc_mark_csr_body(s);
+  block_phase();
+  inc_saddles_body<1024>(s, s.decp, pairs, pcnt, kmax, kmin);
+  block_phase();
+  inc_pull_body<1024>(s, pkey, pid);
+  block_phase();
+  inc_merge_body(s, kmax, kmin, pkey, pid, pairs, pcnt, 0);
+  block_phase();
+  slot_ids_body(s, pairs, pcnt, s.decp, kmax, kmin, s.dlarg, s.dsmall);
+  if (!end) return;
+  block_phase();
+  inc_clear_body(s);
+  block_phase();
+  inc_end_ctr_body(s.ctr);
 }
 
 // Number of differing bytes of two buffers (out[0], accumulated).
