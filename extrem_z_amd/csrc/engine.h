@@ -275,6 +275,7 @@ __global__ void k_fill_i64(long long* p, long long v, long long n);
 __global__ void k_fill_i32(int* p, int v, long long n);
 __global__ void k_ctr_set(int* ctr, int idx, int v);
 __global__ void k_ctr_set2(int* ctr, int a, int b, int v);
+__global__ void k_ctr_export(const int* ctr, int* host, int n, int seq);
 __global__ void k_f32_to_f64(const float* in, double* out, long long n);
 __global__ void k_check_finite(const double* a, long long n, int* ctr, int f32);
 __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* type,

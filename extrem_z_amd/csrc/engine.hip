@@ -177,7 +177,11 @@ struct extremz_ctx {
   ez::DevBuf<int> lchg, spos2, spos1, ilist, qu, qd, sstamp, slists, slstamp, dslot, pcnt,
       pstamp, plists;
   ez::DevBuf<uint8_t> vsnap;  // snapshots of the self-check (EXTREMZ_VERIFY_INC)
+  // the device counters as the last read left them, in coherent pinned
+  // memory the export kernel writes; h_ctr[C_COUNT_] is that write's sequence
+  // number
   int* h_ctr = nullptr;
+  int exp_seq = 0;
   std::vector<ez::EventPair> ev_pool;
   size_t ev_used = 0;
   int timing = 0;       // bit mask of timed kernel families
@@ -363,12 +367,31 @@ class Run {
   // ---- counters
   void set_ctr(int idx, int v) { k_ctr_set<<<1, 64, 0, st()>>>(s_.ctr, idx, v); }
   void set_ctr2(int a, int b, int v) { k_ctr_set2<<<1, 64, 0, st()>>>(s_.ctr, a, b, v); }
+  // Reads the device counters: a one-wavefront kernel writes them, then its
+  // sequence number, into coherent pinned host memory, and the host waits on
+  // that number (a stream synchronisation costs a few microseconds more per
+  // read, and much more with several contexts reading at once).
   void pull() {
     ez::hip_check(hipGetLastError(), "kernel launch");
-    ez::hip_check(hipMemcpyAsync(c_->h_ctr, s_.ctr, sizeof(int) * C_COUNT_,
-                                 hipMemcpyDeviceToHost, st()),
-                  "hipMemcpyAsync");
-    ez::hip_check(hipStreamSynchronize(st()), "hipStreamSynchronize");
+    const int seq = ++c_->exp_seq;
+    k_ctr_export<<<1, 64, 0, st()>>>(s_.ctr, c_->h_ctr, C_COUNT_, seq);
+    ez::hip_check(hipGetLastError(), "kernel launch");
+    const volatile int* flag = c_->h_ctr + C_COUNT_;
+    for (long long spin = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq; spin++) {
+      if (spin % 256) {
+        __builtin_ia32_pause();
+        continue;
+      }
+      // every few microseconds: a failed kernel ends the wait with its error,
+      // a drained stream with a value that must have arrived
+      const hipError_t e = hipStreamQuery(st());
+      if (e == hipSuccess) {
+        if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq)
+          throw HipError("counter export did not arrive");
+        break;
+      }
+      if (e != hipErrorNotReady) ez::hip_check(e, "hipStreamQuery");
+    }
     c_->host_syncs++;
     if (c_->h_ctr[C_IBAIL]) throw HipError("incremental CP pass overran a work list");
   }
@@ -1418,8 +1441,10 @@ extremz_ctx* new_ctx(int device) {
   extremz_ctx* c = new extremz_ctx();
   c->device = device;
   ez::hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-  ez::hip_check(hipHostMalloc((void**)&c->h_ctr, sizeof(int) * C_COUNT_, hipHostMallocDefault),
-            "hipHostMalloc");
+  ez::hip_check(hipHostMalloc((void**)&c->h_ctr, sizeof(int) * (C_COUNT_ + 1),
+                             hipHostMallocCoherent),
+                "hipHostMalloc");
+  std::memset(c->h_ctr, 0, sizeof(int) * (C_COUNT_ + 1));
   const LutTable& L = lut_table();
   c->lut.ensure(L.table.size());
   ez::hip_check(hipMemcpy(c->lut.p, L.table.data(), L.table.size(), hipMemcpyHostToDevice), "lut");

@@ -330,6 +330,14 @@ __global__ void k_ctr_set2(int* ctr, int a, int b, int v) {
     ctr[b] = v;
   }
 }
+// The counters into coherent host memory, then seq after them (system-scope
+// release), on which the host waits.
+__global__ void k_ctr_export(const int* ctr, int* host, int n, int seq) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) host[i] = ctr[i];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&host[n], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 __global__ void k_f32_to_f64(const float* in, double* out, long long n) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
