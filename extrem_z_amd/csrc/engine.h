@@ -319,7 +319,6 @@ __global__ void k_slot_init(St s, long long* kmax, int* smax, long long* kmin, i
 __global__ void k_list_fill64(const int* list, int n, long long* arr, long long val);
 __global__ void k_ext_reduce(const int* inmax, int nmax, const int* inmin, int nmin,
                              const double* f, int* outmax, int* outmin);
-__global__ void k_spos(St s);
 __global__ void k_inc_ptrs(St s, int epoch);
 __global__ void k_inc_taint(St s);
 __global__ void k_inc_walk(St s);
@@ -350,7 +349,7 @@ __global__ void k_wrong_saddle(St s, int side, int do_fix, int gate);
 __global__ void k_saddle_gate(int* ctr, int gate, int wsc);
 __global__ void k_lists_count(St s, int* cnt, int nb, int tiled);
 __global__ void k_lists_totals(St s, const int* cnt, int nb);
-__global__ void k_lists_write(St s, const int* cnt, int nb, int tiled);
+__global__ void k_lists_write(St s, const int* cnt, int nb, int tiled, int index);
 __global__ void k_presnap(St s);
 __global__ void k_argext(const double* f, int n, int* slot_min, int* slot_max);
 __global__ void k_count_edited(const uint8_t* cnt, long long n, int* ctr);

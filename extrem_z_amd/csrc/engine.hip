@@ -398,7 +398,7 @@ class Run {
   int h(int idx) const { return c_->h_ctr[idx]; }
 
   void allocate(long long N);
-  void list_compaction();
+  void list_compaction(int index);
   void slot_pass(int dec, const double* f, const int* L2, const int* L1, const uint16_t* flow,
                  int* F2, int* La2, int* F1, int* La1, int* smax, int* smin);
   void apply(long long bound = 1LL << 62, int also = -1);
@@ -422,7 +422,7 @@ class Run {
 
 // The CP lists S2 / S1 / MX / MN of the types in s_.otype, in the column
 // order of k_lists_count (32 x 32 columns for fields at least that wide).
-void Run::list_compaction() {
+void Run::list_compaction(int index) {
   const Geo& g = s_.g;
   const int tiled = g.W >= 32 && g.H >= 32;
   const long long K =
@@ -441,7 +441,7 @@ void Run::list_compaction() {
   k_scan_single<<<1, 1024, 0, st()>>>(sums, nblk);
   k_scan_add<<<nblk, 1024, 0, st()>>>(cnt, n1, sums);
   k_lists_totals<<<1, 64, 0, st()>>>(s_, cnt, nb);
-  k_lists_write<<<nb, 256, 0, st()>>>(s_, cnt, nb, tiled);
+  k_lists_write<<<nb, 256, 0, st()>>>(s_, cnt, nb, tiled, index);
 }
 
 void Run::allocate(long long N) {
@@ -989,7 +989,7 @@ void Run::split_tree(const double* d_f, int W, int H, int D, TreeOut& out) {
   hip_check(hipMemsetAsync(s_.ctr, 0, sizeof(int) * C_COUNT_, q), "memset");
   k_classify<<<row_blocks(), dim3(64, 4, 1), 0, q>>>(s_.g, s_.lut, d_f, s_.otype, s_.olow);
   label_pass(d_f, c->oDS.p, c->oAS.p, nullptr);
-  list_compaction();
+  list_compaction(0);
   set_ctr2(C_GMIN, C_GMAX, kNone);
   k_argext<<<blocks_for(N), 256, 0, q>>>(d_f, (int)N, s_.ctr + C_GMIN, s_.ctr + C_GMAX);
   pull();
@@ -1262,7 +1262,6 @@ int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double
     ez::hip_check(hipMemsetAsync(p, 0, N, q), "memset");
   for (int* p : {s_.minsad2, s_.minsad1, s_.win_a, s_.win_b})
     k_fill_i32<<<bN, 256, 0, q>>>(p, 0x7fffffff, N);
-  k_fill_i32<<<bN, 256, 0, q>>>(s_.ext_id, -1, N);
   set_ctr2(C_GMIN, C_GMAX, kNone);
   pull();
   if (h(C_FAIL)) return EXTREMZ_EINVAL;
@@ -1273,7 +1272,7 @@ int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double
   // types) comes out of its own label pass
   label_pass(d_orig, s_.oDS, s_.oAS, s_.olow, s_.opdir, s_.otype);
   // main.cpp:100-119: CP lists, triplets, largest/smallest saddle maps of orig
-  list_compaction();
+  list_compaction(1);  // with the saddle positions and the extremum index
   pull();
   s_.n2 = h(C_N2);
   s_.n1 = h(C_N1);
@@ -1316,11 +1315,7 @@ int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double
     ez::hip_check(hipMemsetAsync(s_.sstamp, 0, sizeof(int) * (n21 + 1), q), "memset");
     ez::hip_check(hipMemsetAsync(s_.slstamp, 0, sizeof(int) * (ns + 1), q), "memset");
     ez::hip_check(hipMemsetAsync(s_.pstamp, 0, sizeof(int) * (ns + 1), q), "memset");
-    k_fill_i32<<<bN, 256, 0, q>>>(s_.spos2, -1, N);
-    k_fill_i32<<<bN, 256, 0, q>>>(s_.spos1, -1, N);
-    k_spos<<<blocks_for(std::max(s_.n2, s_.n1)), 256, 0, q>>>(s_);
   }
-  k_ext_index<<<blocks_for(std::max(s_.nmax, s_.nmin)), 256, 0, q>>>(s_);
   {
     const long long ns = (long long)s_.nmax + s_.nmin;
     c->csr_off.ensure(ns + 1);

@@ -2021,12 +2021,6 @@ __global__ void k_list_fill64(const int* list, int n, long long* arr, long long 
 //  5. k_inc_saddles: those sides through saddle_entry; k_inc_pull: the slots
 //     whose gathered keys changed.
 
-// Saddle positions per vertex (-1 elsewhere, filled first).
-__global__ void k_spos(St s) {
-  GRID_STRIDE(i, s.n2) s.spos2[s.S2[i]] = i;
-  GRID_STRIDE(i, s.n1) s.spos1[s.S1[i]] = i;
-}
-
 // One lane per (changed vertex, candidate): candidate 0 is the vertex,
 // 1 + d its neighbour in direction d; each vertex once per epoch.  New lower
 // masks and steepest directions (the order of k_labels_tile); lists the
@@ -3344,7 +3338,12 @@ __global__ void k_lists_totals(St s, const int* cnt, int nb) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb, int tiled) {
+// With index, every vertex also gets its list positions: spos2 / spos1 (-1
+// off the saddle lists) and ext_id (2 * position in MX, 2 * position in MN +
+// 1, -1 elsewhere); the traversal visits each vertex once, so nothing needs
+// clearing first.
+__global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb, int tiled,
+                                                     int index) {
   __shared__ int w[4][4];
   const int v = list_vertex(s.g, blockIdx.x * 256LL + threadIdx.x, tiled);
   const unsigned fl = list_flags(s, v);
@@ -3357,12 +3356,19 @@ __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int n
   }
   __syncthreads();
   int* lists[4] = {s.S2, s.S1, s.MX, s.MN};
+  int pos[4] = {-1, -1, -1, -1};
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     if (!((fl >> c) & 1u)) continue;
     int base = cnt[(long long)c * nb + blockIdx.x] - cnt[(long long)c * nb];
     for (int k = 0; k < wave; k++) base += w[c][k];
-    lists[c][base + __popcll(m[c] & ((1ull << lane) - 1ull))] = v;
+    pos[c] = base + __popcll(m[c] & ((1ull << lane) - 1ull));
+    lists[c][pos[c]] = v;
+  }
+  if (index && v >= 0) {
+    s.spos2[v] = pos[0];
+    s.spos1[v] = pos[1];
+    s.ext_id[v] = pos[2] >= 0 ? 2 * pos[2] : (pos[3] >= 0 ? 2 * pos[3] + 1 : -1);
   }
 }
 
