@@ -332,6 +332,9 @@ class Run {
   static constexpr int kWnRounds = 4;
   // Listed sides up to which the wrong-index phases run in one block.
   static constexpr int kWrongOne = 4096;
+  // Candidate lists up to which k_wn_solve runs every reservation round in
+  // one block; longer ones take the multi-block rounds.
+  static constexpr int kWnOneMax = 4096;
   // Blocks of the first stage of the decp extremum reduction.
   static constexpr int kExtBlocks = 2048;
   // Changed vertices up to which an incremental pass runs in one block (its
@@ -828,7 +831,7 @@ void Run::verify_inc() {
 void Run::wrong_neighbors() {
   // short candidate lists: every round in one block, one host read
   tbegin(EXTREMZ_K_RESERVE);
-  k_wn_solve<<<1, 1024, 0, st()>>>(s_, c_->actA.p, c_->actB.p, 32768);
+  k_wn_solve<<<1, 1024, 0, st()>>>(s_, c_->actA.p, c_->actB.p, kWnOneMax);
   tend();
   pull();
   if (h(C_WN_BAIL) == 2) throw HipError("wrong-neighbour reservations did not finish");
