@@ -564,6 +564,8 @@ __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int f
 // The other touched length (and zero_idx, when given) is set to zero for the
 // next producer.
 __global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
+  __shared__ int changed;  // one store of C_ANY_CHANGE per block, not one per lane
+  if (threadIdx.x == 0) changed = 0;
   STAGED_LIST(dl, int, kStage, s.dirty, &s.ctr[s.dctr]);
   STAGED_LIST(cl, int, kStage, s.lchg, &s.ctr[C_LCHG]);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -605,7 +607,7 @@ __global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
     s.decp[t] = nx;
     s.cnt[t] = (uint8_t)c;
     if (nx != x) {
-      s.ctr[C_ANY_CHANGE] = 1;
+      changed = 1;
       if (!s.dflag[t]) {
         s.dflag[t] = 1;
         dl.push(t);
@@ -618,6 +620,7 @@ __global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
   }
   dl.flush();
   cl.flush();
+  if (threadIdx.x == 0 && changed) s.ctr[C_ANY_CHANGE] = 1;  // (after the flushes' barriers)
 }
 
 // ----------------------------------------------------- manifold labels ----
@@ -1460,8 +1463,15 @@ __device__ __forceinline__ void regather_slots(const St& s, int v) {
     const int e = s.ext_id[Lo[v + s.g.off[d]]];
     if (e < 0 || (e & 1) != (SIDE == 1)) continue;
     const int j = slot_index(s, e);
-    if (atomicExch(&s.slstamp[j], s.iepoch) != s.iepoch)
-      s.dslot[atomicAdd(&s.ctr[C_DSLOT], 1)] = j;
+    if (atomicExch(&s.slstamp[j], s.iepoch) != s.iepoch) {
+      // one counter atomic per wavefront for the lanes appending here
+      const unsigned long long m = __ballot(1);
+      const int lane = __lane_id(), leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&s.ctr[C_DSLOT], __popcll(m));
+      base = __shfl(base, leader);
+      s.dslot[base + __popcll(m & ((1ull << lane) - 1ull))] = j;
+    }
   }
 }
 
