@@ -1212,26 +1212,29 @@ template __global__ void k_scan_propose<false>(St, int*, int, int, int);
 // the tile by k_labels_tile or written here, so voxels resolved in their tile
 // are only read.  A label with it names the next vertex on the steepest path;
 // other lanes' writes only ever replace it by that path's root.  The walk is
-// bounded by N.
-__device__ __forceinline__ void chase_one(int* S, int v, int n) {
-  int r = S[v];
-  if (r >= 0) return;  // resolved inside its tile
-  r &= ~kExit;
-  for (int step = 0; step < n; step++) {
-    const int rr = __hip_atomic_load(&S[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (rr >= 0) {
-      r = rr;
-      break;
-    }
-    r = rr & ~kExit;
-  }
-  S[v] = r;
-}
+// bounded by N.  Both labels of a voxel are walked together, so their loads
+// are in flight at once.
 __global__ void __launch_bounds__(256) k_chase(int n, int* ds, int* as) {
   int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  chase_one(ds, v, n);
-  chase_one(as, v, n);
+  const int a0 = ds[v], b0 = as[v];
+  if (a0 >= 0 && b0 >= 0) return;  // both resolved inside the tile
+  int ra = a0 & ~kExit, rb = b0 & ~kExit;
+  bool da = a0 >= 0, db = b0 >= 0;
+  for (int step = 0; step < n && !(da && db); step++) {
+    const int na = da ? 0 : __hip_atomic_load(&ds[ra], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int nb = db ? 0 : __hip_atomic_load(&as[rb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!da) {
+      da = na >= 0;
+      ra = da ? na : na & ~kExit;
+    }
+    if (!db) {
+      db = nb >= 0;
+      rb = db ? nb : nb & ~kExit;
+    }
+  }
+  if (a0 < 0) ds[v] = ra;
+  if (b0 < 0) as[v] = rb;
 }
 
 // ------------------------------------------------------- CP structures ----
