@@ -277,7 +277,8 @@ using Touched = StagedList<int, kStage>;
 
 // swap + the caller's `d > oldValue` gate, sequentially a max (D5).  The
 // first proposal above the sentinel lists the vertex as touched.
-__device__ __forceinline__ void propose(const St& s, int t, double d, Touched& tl) {
+template <class TL>
+__device__ __forceinline__ void propose(const St& s, int t, double d, TL& tl) {
   // fixed-step rule: every proposal is -xi/q; the float32 fork's fixes are
   // -bound / (1 << 2) (src/ExtreM_cuda_f.cu:2703, 2749, 2804)
   if (s.step_q || s.f32) d = s.step;
@@ -294,7 +295,8 @@ __device__ __forceinline__ double star_fix(const St& s, double o, double x) {
   if (s.f32) return (double)(((float)o - (float)s.xi) - (float)x);
   return (o - s.xi) - x;
 }
-__device__ __forceinline__ void propose_star(const St& s, int t, double d, Touched& tl) {
+template <class TL>
+__device__ __forceinline__ void propose_star(const St& s, int t, double d, TL& tl) {
   long long k = okey(d);
   if (k <= s.sent) return;
   long long old = atomicMax(&s.delta[t], k);
@@ -309,6 +311,22 @@ __device__ __forceinline__ double raise_mid(const St& s, int v) {
 
 #define GRID_STRIDE(i, n) \
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+// GRID_STRIDE in block-uniform rounds for loops that append to staged lists:
+// body(i) for this thread's item of a round (returning early is its
+// `continue`), then every list is emptied when fewer than room entries are
+// free (room: the most one round can push), so a long input never appends
+// with a global atomic per wavefront on one counter.  Every thread of the
+// block must call it.
+template <class F, class... Ls>
+__device__ __forceinline__ void block_rounds(long long n, int room, F&& body, Ls&... ls) {
+  for (long long k0 = blockIdx.x * (long long)blockDim.x; k0 < n;
+       k0 += (long long)gridDim.x * blockDim.x) {
+    if (k0 + threadIdx.x < n) body(k0 + threadIdx.x);
+    __syncthreads();
+    (ls.drain(room), ...);
+  }
+}
 
 // ---------------------------------------------------------- utilities ----
 __global__ void k_fill_i64(long long* p, long long v, long long n) {
@@ -404,19 +422,27 @@ __global__ void __launch_bounds__(256) k_check_list(St s, const int* list, int l
   // candidate 1 + d its neighbour in direction d
   const int per = with_nbrs ? kNbr + 1 : 1;
   const long long n = (long long)s.ctr[lctr] * per;
-  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
-       k += (long long)gridDim.x * blockDim.x) {
+  auto one = [&](long long k) {
     const int i = (int)(k / per), j = (int)(k - (long long)i * per);
     int u = list[i];
     if (clear_dirty && j == 0) s.dflag[u] = 0;
     if (j > 0) {
       int x, y, z;
       coords(s.g, u, x, y, z);
-      if (!((c_cls_key[class_of(s.g, x, y, z)] >> (j - 1)) & 1)) continue;
+      if (!((c_cls_key[class_of(s.g, x, y, z)] >> (j - 1)) & 1)) return;
       u += s.g.off[j - 1];
     }
-    if (atomicExch(&s.stamp[u], epoch) == epoch) continue;
+    if (atomicExch(&s.stamp[u], epoch) == epoch) return;
     if (mismatch_at(s, u)) fl.push(u);
+  };
+  // block-uniform rounds of at most one push per thread, the buffer emptied
+  // between them (a long list never appends with a global atomic per
+  // wavefront on one counter)
+  for (long long k0 = blockIdx.x * (long long)blockDim.x; k0 < n;
+       k0 += (long long)gridDim.x * blockDim.x) {
+    if (k0 + threadIdx.x < n) one(k0 + threadIdx.x);
+    __syncthreads();
+    fl.drain(256);
   }
   fl.flush();
 }
@@ -434,8 +460,7 @@ __global__ void __launch_bounds__(256) k_check_two(St s, const int* flist, int n
     s.ctr[zero_idx] = 0;
   }
   const long long n = nf + (long long)s.ctr[s.dctr] * (kNbr + 1);
-  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
-       k += (long long)gridDim.x * blockDim.x) {
+  auto one = [&](long long k) {
     int u;
     if (k < nf) {
       u = flist[k];
@@ -448,12 +473,19 @@ __global__ void __launch_bounds__(256) k_check_two(St s, const int* flist, int n
       } else {
         int x, y, z;
         coords(s.g, u, x, y, z);
-        if (!((c_cls_key[class_of(s.g, x, y, z)] >> (j - 1)) & 1)) continue;
+        if (!((c_cls_key[class_of(s.g, x, y, z)] >> (j - 1)) & 1)) return;
         u += s.g.off[j - 1];
       }
     }
-    if (atomicExch(&s.stamp[u], epoch) == epoch) continue;
+    if (atomicExch(&s.stamp[u], epoch) == epoch) return;
     if (mismatch_at(s, u)) fl.push(u);
+  };
+  // block-uniform rounds, the buffer emptied between them (as k_check_list)
+  for (long long k0 = blockIdx.x * (long long)blockDim.x; k0 < n;
+       k0 += (long long)gridDim.x * blockDim.x) {
+    if (k0 + threadIdx.x < n) one(k0 + threadIdx.x);
+    __syncthreads();
+    fl.drain(256);
   }
   fl.flush();
 }
@@ -472,87 +504,97 @@ __global__ void k_clear_dirty(St s, int zero_idx) {
 // ------------------------------------------------- c_loops proposals ----
 // c_loop (src/ExtreM.cpp:1152-1236) for false maxima / minima and
 // get_false_stars (src/ExtreM.cpp:1412-1445) for saddle-list entries.
+// A frontier entry makes at most 15 proposals (itself or its 14 neighbours):
+// block-uniform rounds with the touched buffer emptied whenever a round
+// could overfill it, so a dense frontier never appends with a global atomic
+// per wavefront on one counter.
+constexpr int kProposeStage = 4096;
 __global__ void __launch_bounds__(256) k_propose_c(St s, const int* front, int fctr) {
-  STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
-  int n = s.ctr[fctr];
-  GRID_STRIDE(i, n) {
-    int v = front[i];
-    int x, y, z;
-    coords(s.g, v, x, y, z);
-    int cls = class_of(s.g, x, y, z);
-    int key = c_cls_key[cls];
-    unsigned dl = lower_mask_at(s.g, s.decp, v, cls);
-    unsigned ol = s.olow[v];
-    int dt = type_of_mask(s.lut, cls, dl);
-    int ot = s.otype[v];
-    if ((dt == 4) != (ot == 4)) {  // all_max, direction 0
-      if (ot != 4) {
-        propose(s, v, lower_mid(s, v), tl);
-      } else {
-        int largest = v;
-        double fl = s.decp[v];
-        for (int d = 0; d < kNbr; d++) {
-          if (!((key >> d) & 1)) continue;
-          int r = v + s.g.off[d];
-          double fr = s.decp[r];
-          if (larger(fr, r, fl, largest)) {
-            largest = r;
-            fl = fr;
+  STAGED_LIST(tl, int, kProposeStage, s.touched, &s.ctr[s.tctr]);
+  const int n = s.ctr[fctr];
+  for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+    const int i = i0 + threadIdx.x;
+    if (i < n) {
+      int v = front[i];
+      int x, y, z;
+      coords(s.g, v, x, y, z);
+      int cls = class_of(s.g, x, y, z);
+      int key = c_cls_key[cls];
+      unsigned dl = lower_mask_at(s.g, s.decp, v, cls);
+      unsigned ol = s.olow[v];
+      int dt = type_of_mask(s.lut, cls, dl);
+      int ot = s.otype[v];
+      if ((dt == 4) != (ot == 4)) {  // all_max, direction 0
+        if (ot != 4) {
+          propose(s, v, lower_mid(s, v), tl);
+        } else {
+          int largest = v;
+          double fl = s.decp[v];
+          for (int d = 0; d < kNbr; d++) {
+            if (!((key >> d) & 1)) continue;
+            int r = v + s.g.off[d];
+            double fr = s.decp[r];
+            if (larger(fr, r, fl, largest)) {
+              largest = r;
+              fl = fr;
+            }
           }
+          double fv = s.decp[v];
+          if (!(fv > fl || (fv == fl && v > largest))) propose(s, largest, lower_mid(s, largest), tl);
         }
-        double fv = s.decp[v];
-        if (!(fv > fl || (fv == fl && v > largest))) propose(s, largest, lower_mid(s, largest), tl);
       }
-    }
-    if ((dt == 0) != (ot == 0)) {  // all_min, direction 1
-      if (ot != 0) {
-        int smallest = v;
-        double fo = s.orig[v];
-        for (int d = 0; d < kNbr; d++) {
-          if (!((key >> d) & 1)) continue;
-          int r = v + s.g.off[d];
-          double fr = s.orig[r];
-          if (smaller(fr, r, fo, smallest)) {
-            smallest = r;
-            fo = fr;
+      if ((dt == 0) != (ot == 0)) {  // all_min, direction 1
+        if (ot != 0) {
+          int smallest = v;
+          double fo = s.orig[v];
+          for (int d = 0; d < kNbr; d++) {
+            if (!((key >> d) & 1)) continue;
+            int r = v + s.g.off[d];
+            double fr = s.orig[r];
+            if (smaller(fr, r, fo, smallest)) {
+              smallest = r;
+              fo = fr;
+            }
           }
+          double dv = s.decp[v], ds = s.decp[smallest];
+          if (!(dv > ds || (dv == ds && v > smallest)))
+            propose(s, smallest, lower_mid(s, smallest), tl);
+        } else {
+          propose(s, v, lower_mid(s, v), tl);
         }
-        double dv = s.decp[v], ds = s.decp[smallest];
-        if (!(dv > ds || (dv == ds && v > smallest)))
-          propose(s, smallest, lower_mid(s, smallest), tl);
-      } else {
-        propose(s, v, lower_mid(s, v), tl);
+      }
+      bool sad = (dt != ot) ? (is_saddle_type(dt) || is_saddle_type(ot))
+                            : (is_saddle_type(ot) && dl != ol);
+      if (sad && dl != ol) {  // get_false_stars
+        // propose() for every star neighbour, with all loads and atomics of the
+        // star issued before any result is used
+        const unsigned a = ol & ~dl;
+        double po[kNbr], pd[kNbr];
+  #pragma unroll
+        for (int d = 0; d < kNbr; d++) {
+          const bool on = (a >> d) & 1;
+          po[d] = on ? s.orig[v + s.g.off[d]] : 0.0;
+          pd[d] = on ? s.decp[v + s.g.off[d]] : 0.0;
+        }
+        long long old[kNbr];
+        unsigned sent = 0;  // atomics issued
+  #pragma unroll
+        for (int d = 0; d < kNbr; d++) {
+          old[d] = 0;
+          if (!((a >> d) & 1)) continue;
+          const long long k = okey(star_fix(s, po[d], pd[d]));
+          if (k <= s.sent) continue;
+          old[d] = atomicMax(&s.delta[v + s.g.off[d]], k);
+          sent |= 1u << d;
+        }
+  #pragma unroll
+        for (int d = 0; d < kNbr; d++)
+          if (((sent >> d) & 1) && old[d] == s.sent) tl.push(v + s.g.off[d]);
+        if (dl & ~ol) propose_star(s, v, star_fix(s, s.orig[v], s.decp[v]), tl);
       }
     }
-    bool sad = (dt != ot) ? (is_saddle_type(dt) || is_saddle_type(ot))
-                          : (is_saddle_type(ot) && dl != ol);
-    if (sad && dl != ol) {  // get_false_stars
-      // propose() for every star neighbour, with all loads and atomics of the
-      // star issued before any result is used
-      const unsigned a = ol & ~dl;
-      double po[kNbr], pd[kNbr];
-#pragma unroll
-      for (int d = 0; d < kNbr; d++) {
-        const bool on = (a >> d) & 1;
-        po[d] = on ? s.orig[v + s.g.off[d]] : 0.0;
-        pd[d] = on ? s.decp[v + s.g.off[d]] : 0.0;
-      }
-      long long old[kNbr];
-      unsigned sent = 0;  // atomics issued
-#pragma unroll
-      for (int d = 0; d < kNbr; d++) {
-        old[d] = 0;
-        if (!((a >> d) & 1)) continue;
-        const long long k = okey(star_fix(s, po[d], pd[d]));
-        if (k <= s.sent) continue;
-        old[d] = atomicMax(&s.delta[v + s.g.off[d]], k);
-        sent |= 1u << d;
-      }
-#pragma unroll
-      for (int d = 0; d < kNbr; d++)
-        if (((sent >> d) & 1) && old[d] == s.sent) tl.push(v + s.g.off[d]);
-      if (dl & ~ol) propose_star(s, v, star_fix(s, s.orig[v], s.decp[v]), tl);
-    }
+    __syncthreads();
+    tl.drain(256 * (kNbr + 1));
   }
   tl.flush();
 }
@@ -572,51 +614,60 @@ __global__ void __launch_bounds__(256) k_apply(St s, int zero_idx) {
     s.ctr[s.tctr == C_TOUCHED ? C_TOUCHED2 : C_TOUCHED] = 0;
     if (zero_idx >= 0) s.ctr[zero_idx] = 0;
   }
-  int n = s.ctr[s.tctr];
-  GRID_STRIDE(i, n) {
-    int t = s.touched[i];
-    double dd = okey_inv(s.delta[t]);
-    s.delta[t] = s.sent;
-    double o = s.orig[t], x = s.decp[t];
-    int c = s.cnt[t];
-    double nx;
-    if (s.f32) {  // applyDeltaBuffer of the float32 fork (src/ExtreM_cuda_f.cu:2854-2866)
-      const float df = (float)dd, xf = (float)x, of = (float)o, bf = (float)s.xi;
-      if (fabsf(df) > 1e-6 && c < 6 && fabsf(of - (xf + df)) < bf) {
-        nx = (double)(xf + df);
-        c += 1;
-      } else {
-        nx = (double)(of - bf);
-        c = 7;
-      }
-    } else if (s.step_q) {  // applyDeltaBuffer, edit_type 0 (src/ExtreM_cuda.cu:3065-3076)
-      if (c < s.step_q - 1 && fabs(o - (x + dd)) <= s.xi) {
+  const int n = s.ctr[s.tctr];
+  // block-uniform rounds: a round pushes at most one entry per thread to
+  // each list, and the buffers are emptied between rounds, so a long touched
+  // list never appends with a global atomic per wavefront on one counter
+  for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+    const int i = i0 + threadIdx.x;
+    if (i < n) {
+      int t = s.touched[i];
+      double dd = okey_inv(s.delta[t]);
+      s.delta[t] = s.sent;
+      double o = s.orig[t], x = s.decp[t];
+      int c = s.cnt[t];
+      double nx;
+      if (s.f32) {  // applyDeltaBuffer of the float32 fork (src/ExtreM_cuda_f.cu:2854-2866)
+        const float df = (float)dd, xf = (float)x, of = (float)o, bf = (float)s.xi;
+        if (fabsf(df) > 1e-6 && c < 6 && fabsf(of - (xf + df)) < bf) {
+          nx = (double)(xf + df);
+          c += 1;
+        } else {
+          nx = (double)(of - bf);
+          c = 7;
+        }
+      } else if (s.step_q) {  // applyDeltaBuffer, edit_type 0 (src/ExtreM_cuda.cu:3065-3076)
+        if (c < s.step_q - 1 && fabs(o - (x + dd)) <= s.xi) {
+          nx = x + dd;
+          c += 1;
+        } else {
+          nx = o - s.xi;
+          c = s.step_q;
+        }
+      } else if (fabs(dd) > 1e-15 && c < 5 && fabs(o - x - dd) <= s.xi) {
         nx = x + dd;
         c += 1;
       } else {
         nx = o - s.xi;
-        c = s.step_q;
+        c = 6;
       }
-    } else if (fabs(dd) > 1e-15 && c < 5 && fabs(o - x - dd) <= s.xi) {
-      nx = x + dd;
-      c += 1;
-    } else {
-      nx = o - s.xi;
-      c = 6;
-    }
-    s.decp[t] = nx;
-    s.cnt[t] = (uint8_t)c;
-    if (nx != x) {
-      changed = 1;
-      if (!s.dflag[t]) {
-        s.dflag[t] = 1;
-        dl.push(t);
-      }
-      if (!s.lflag[t]) {  // one touched-list entry per vertex: no other lane sets it
-        s.lflag[t] = 1;
-        cl.push(t);
+      s.decp[t] = nx;
+      s.cnt[t] = (uint8_t)c;
+      if (nx != x) {
+        changed = 1;
+        if (!s.dflag[t]) {
+          s.dflag[t] = 1;
+          dl.push(t);
+        }
+        if (!s.lflag[t]) {  // one touched-list entry per vertex: no other lane sets it
+          s.lflag[t] = 1;
+          cl.push(t);
+        }
       }
     }
+    __syncthreads();
+    dl.drain(256);
+    cl.drain(256);
   }
   dl.flush();
   cl.flush();
@@ -916,6 +967,10 @@ __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_
     const int lz = z - z0;
     store((lz + 2) % 3);
     __syncthreads();
+    // a plane pushes at most one entry per owned voxel: the buffer is emptied
+    // before it could fill (a dense frontier, up to most of the grid, would
+    // otherwise append with a global atomic per wavefront on one counter)
+    fl.drain(256);
     if (z + 2 < z1 + 1) fetch(z + 2);
     const double* P0 = ring[lz % 3];        // z - 1
     const double* P1 = ring[(lz + 1) % 3];  // z
@@ -3063,7 +3118,8 @@ __global__ void k_wn_reset(St s) {
 __global__ void __launch_bounds__(256) k_fixpath(St s, int fast) {
   STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
   int n2 = s.ctr[C_NFC], n1 = s.ctr[C_NFC1];
-  GRID_STRIDE(e, n2 + n1) {
+  block_rounds(n2 + n1, 256, [&](long long ee) {
+    const int e = (int)ee;
     int dir = e < n2 ? 0 : 1;
     int cur = dir == 0 ? s.wnl2[e] : s.wnl1[e - n2];
     int tru = kNone, fal = kNone;
@@ -3107,10 +3163,10 @@ __global__ void __launch_bounds__(256) k_fixpath(St s, int fast) {
       if (nx == cur) break;
       cur = nx;
     }
-    if (tru == fal) continue;
+    if (tru == fal) return;
     if (dir == 0) propose(s, fal, lower_mid(s, fal), tl);
     else propose(s, tru, lower_mid(s, tru), tl);
-  }
+  }, tl);
   tl.flush();
 }
 
@@ -3135,7 +3191,9 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
     s.ctr[side == 2 ? C_WMC : C_WMINC2] = 0;
     s.ctr[side == 2 ? C_WMC2 : C_WMINC] = 0;
   }
-  GRID_STRIDE(k, n) {
+  // (at most two proposals per listed side)
+  block_rounds(n, 512, [&](long long kk) {
+    const int k = (int)kk;
     const int i = WL[k];
     int sv = S[i];
     int a = of[i], b = ol[i], c = df[i], d = dl[i];
@@ -3166,7 +3224,7 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
       if (mf) s.win_a[a] = 0x7fffffff;
       if (ml) s.win_b[b] = 0x7fffffff;
     }
-  }
+  }, tl);
   tl.flush();
   cf.flush();
   cl.flush();
@@ -3271,20 +3329,21 @@ __global__ void __launch_bounds__(256) k_wrong_saddle(St s, int side, int do_fix
   int dg = s.ctr[side == 2 ? C_DGMAX : C_DGMIN];
   STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
   STAGED_COUNT(cnt, &s.ctr[side == 2 ? C_WSC : C_WSCJ]);
-  GRID_STRIDE(i, n) {
+  block_rounds(n, 256, [&](long long k) {
+    const int i = (int)k;
     int m = E[i];
     int ov = m == og ? kNone : os[i];
     int dv = m == dg ? kNone : ds[i];
     if (ov == 0x7fffffff) ov = kNone;  // empty minimum slot
     if (dv == 0x7fffffff) dv = kNone;
-    if (ov == dv || ov == kNone || dv == kNone) continue;
+    if (ov == dv || ov == kNone || dv == kNone) return;
     cnt.inc();
-    if (!do_fix) continue;
+    if (!do_fix) return;
     double ft = s.decp[ov], ff = s.decp[dv];
-    if (ft > ff || (ft == ff && ov > dv)) continue;
+    if (ft > ff || (ft == ff && ov > dv)) return;
     if (side == 2) propose(s, dv, lower_mid(s, dv), tl);
     else propose(s, ov, raise_mid(s, ov), tl);
-  }
+  }, tl);
   tl.flush();
   cnt.flush();
 }
