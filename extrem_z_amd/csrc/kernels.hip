@@ -230,10 +230,12 @@ struct StagedList {
   // Moves the buffer to the global list and empties it when fewer than room
   // entries are free, so the pushes that follow stay in LDS.  Every thread of
   // the block calls it right after a barrier, with no push in flight (so all
-  // read the same count).
+  // read the same count).  room counts as at most half the buffer: a round
+  // that could push more may spill its excess with a global atomic per
+  // wavefront, but a small list is not moved out after every round.
   __device__ __forceinline__ void drain(int room) {
     const int cnt = *n < K ? *n : K;
-    if (cnt + room <= K) return;
+    if (cnt == 0 || cnt + (room < K / 2 ? room : K / 2) <= K) return;
     const int tid = threadIdx.x + blockDim.x * threadIdx.y;
     const int nt = blockDim.x * blockDim.y;
     if (tid == 0) *base = atomicAdd(gcount, cnt);
@@ -314,17 +316,17 @@ __device__ __forceinline__ double raise_mid(const St& s, int v) {
 
 // GRID_STRIDE in block-uniform rounds for loops that append to staged lists:
 // body(i) for this thread's item of a round (returning early is its
-// `continue`), then every list is emptied when fewer than room entries are
-// free (room: the most one round can push), so a long input never appends
-// with a global atomic per wavefront on one counter.  Every thread of the
-// block must call it.
+// `continue`), then every list is emptied when fewer entries are free than a
+// round can push (per: the most one item pushes to a list), so a long input
+// never appends with a global atomic per wavefront on one counter.  Every
+// thread of the (1-D) block must call it.
 template <class F, class... Ls>
-__device__ __forceinline__ void block_rounds(long long n, int room, F&& body, Ls&... ls) {
+__device__ __forceinline__ void block_rounds(long long n, int per, F&& body, Ls&... ls) {
   for (long long k0 = blockIdx.x * (long long)blockDim.x; k0 < n;
        k0 += (long long)gridDim.x * blockDim.x) {
     if (k0 + threadIdx.x < n) body(k0 + threadIdx.x);
     __syncthreads();
-    (ls.drain(room), ...);
+    (ls.drain(per * (int)blockDim.x), ...);
   }
 }
 
@@ -1697,6 +1699,12 @@ __global__ void __launch_bounds__(256, 4) k_saddle_list(St s, const double* f, c
     }
     __syncthreads();
     if (tid == 0) pcnt[c] = s_np;
+    // (a side pushes at most one entry to each list)
+    sa.drain(256);
+    sc2.drain(256);
+    sc1.drain(256);
+    sw2.drain(256);
+    sw1.drain(256);
   }
   if (MODE == kModeOrig) {
     __syncthreads();
@@ -2098,8 +2106,8 @@ __device__ __forceinline__ void inc_ptrs_body(const St& s, int epoch) {
   STAGED_LIST(lu, int, 512, s.cu, &s.ctr[C_CU]);
   STAGED_LIST(ld, int, 512, s.cd, &s.ctr[C_CD]);
   const long long n = (long long)s.ctr[C_LCHG] * (kNbr + 1);
-  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
-       k += (long long)gridDim.x * blockDim.x) {
+  // (one entry per item at most to each list)
+  block_rounds(n, 1, [&](long long k) {
     const int i = (int)(k / (kNbr + 1)), j = (int)(k - (long long)i * (kNbr + 1));
     int u = s.lchg[i];
     int x, y, z;
@@ -2108,11 +2116,11 @@ __device__ __forceinline__ void inc_ptrs_body(const St& s, int epoch) {
     if (j == 0) {  // the global extremum of decp changed value: k_ext_reduce runs
       if (u == s.ctr[C_DGMAX] || u == s.ctr[C_DGMIN]) s.ctr[C_EXTD] = 1;
     } else {
-      if (!((c_cls_key[cls] >> (j - 1)) & 1)) continue;
+      if (!((c_cls_key[cls] >> (j - 1)) & 1)) return;
       u += s.g.off[j - 1];
       cls = class_of(s.g, x + c_dx[j - 1], y + c_dy[j - 1], z + c_dz[j - 1]);
     }
-    if (atomicExch(&s.stamp[u], epoch) == epoch) continue;
+    if (atomicExch(&s.stamp[u], epoch) == epoch) return;
     const int key = c_cls_key[cls];
     const double c = s.decp[u];
     unsigned lm = 0;
@@ -2146,7 +2154,7 @@ __device__ __forceinline__ void inc_ptrs_body(const St& s, int epoch) {
       s.pdir[u] = (uint8_t)nd;
       s.cflag[u] = (uint8_t)cf;
     }
-  }
+  }, la, lu, ld);
   la.flush();
   lu.flush();
   ld.flush();
@@ -2202,7 +2210,8 @@ __device__ __forceinline__ void inc_walk_body(const St& s) {
   STAGED_LIST(su, int, 512, s.qu, &s.ctr[C_QU]);
   STAGED_LIST(sd, int, 512, s.qd, &s.ctr[C_QD]);
   const int nu = s.ctr[C_CU], n = nu + s.ctr[C_CD];
-  GRID_STRIDE(k, n) {
+  block_rounds(n, 1, [&](long long kk) {
+    const int k = (int)kk;
     const bool up = k < nu;
     const int c = up ? s.cu[k] : s.cd[k - nu];
     int* R = up ? s.dDS : s.dAS;
@@ -2228,7 +2237,7 @@ __device__ __forceinline__ void inc_walk_body(const St& s) {
       if (up) su.push(c);
       else sd.push(c);
     }
-  }
+  }, su, sd);
   su.flush();
   sd.flush();
   // the last block to finish opens the first propagation level
@@ -2374,6 +2383,10 @@ __global__ void __launch_bounds__(1024) k_inc_prop(St s, int qcap) {
   inc_prop_body(s, qcap);
 }
 
+// Staged entries of the side-marking lists (an item of k_inc_mark marks up to
+// 14 sides; a wavefront of k_inc_mark_csr up to 1024).
+constexpr int kMarkStage = 4096;
+
 // Queues side q2 of S2 (two) or S1 for the saddle recompute, once per pass.
 template <class L>
 __device__ __forceinline__ void mark_side(const St& s, int q2, bool two, L& l2, L& l1) {
@@ -2419,20 +2432,21 @@ __device__ __forceinline__ void inc_mark_body(const St& s, long long* kmax, long
     }
   }
 
-  STAGED_LIST(l2, int, 1024, s.dsl2, &s.ctr[C_DSL2]);
-  STAGED_LIST(l1, int, 1024, s.dsl1, &s.ctr[C_DSL1]);
+  STAGED_LIST(l2, int, kMarkStage, s.dsl2, &s.ctr[C_DSL2]);
+  STAGED_LIST(l1, int, kMarkStage, s.dsl1, &s.ctr[C_DSL1]);
   const long long nal = s.ctr[C_AL], nch = s.ctr[C_LCHG], nqu = s.ctr[C_QU],
                   nqd = s.ctr[C_QD], ncl2 = s.ctr[C_CL2], ncl1 = s.ctr[C_CL1];
   const long long n = nal + nch + nqu + nqd + ncl2 + ncl1;
-  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
-       k += (long long)gridDim.x * blockDim.x) {
+  // (an item marks up to 14 sides: the lists hold kMarkStage entries and are
+  // emptied between rounds)
+  block_rounds(n, kNbr, [&](long long k) {
     long long t = k;
     if (t < nal + nch) {
       const int u = t < nal ? s.al[t] : s.lchg[t - nal];
       const int p2 = s.spos2[u], p1 = s.spos1[u];
       if (p2 >= 0) mark_side(s, p2, true, l2, l1);
       if (p1 >= 0) mark_side(s, p1, false, l2, l1);
-      continue;
+      return;
     }
     t -= nal + nch;
     if (t < nqu + nqd) {
@@ -2448,12 +2462,12 @@ __device__ __forceinline__ void inc_mark_body(const St& s, long long* kmax, long
 #pragma unroll
       for (int d = 0; d < kNbr; d++)
         if (p[d] >= 0) mark_side(s, p[d], up, l2, l1);
-      continue;
+      return;
     }
     t -= nqu + nqd;
     if (t < ncl2) mark_side(s, s.cl2[t], true, l2, l1);
     else mark_side(s, s.cl1[t - ncl2], false, l2, l1);
-  }
+  }, l2, l1);
   l2.flush();
   l1.flush();
   // the extremum->saddle index ranges of the changed extrema that are still
@@ -2568,14 +2582,22 @@ __global__ void __launch_bounds__(256) k_inc_merge(St s, long long* kmax, long l
 // k_inc_mark_csr queues one chunk per wavefront.
 __device__ __forceinline__ void inc_mark_csr_body(const St& s) {
   if (s.ctr[C_IBAIL]) return;
-  STAGED_LIST(l2, int, 1024, s.dsl2, &s.ctr[C_DSL2]);
-  STAGED_LIST(l1, int, 1024, s.dsl1, &s.ctr[C_DSL1]);
+  STAGED_LIST(l2, int, kMarkStage, s.dsl2, &s.ctr[C_DSL2]);
+  STAGED_LIST(l1, int, kMarkStage, s.dsl1, &s.ctr[C_DSL1]);
   const int n = s.ctr[C_NITEM];
   const int lane = threadIdx.x & 63;
-  const int nw = (gridDim.x * blockDim.x) >> 6;
-  for (int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < n; w += nw) {
-    const int4 it = s.citems[w];
-    for (int k = it.x + lane; k < it.y; k += 64) mark_side(s, s.csr[k], it.z != 0, l2, l1);
+  const int bw = blockDim.x >> 6, nw = gridDim.x * bw;
+  // block-uniform rounds of one chunk per wavefront (up to 1024 marks each),
+  // the lists emptied between rounds
+  for (int w0 = blockIdx.x * bw; w0 < n; w0 += nw) {
+    const int w = w0 + (threadIdx.x >> 6);
+    if (w < n) {
+      const int4 it = s.citems[w];
+      for (int k = it.x + lane; k < it.y; k += 64) mark_side(s, s.csr[k], it.z != 0, l2, l1);
+    }
+    __syncthreads();
+    l2.drain(1024 * bw);
+    l1.drain(1024 * bw);
   }
   l2.flush();
   l1.flush();
@@ -2627,6 +2649,12 @@ __device__ __forceinline__ void inc_saddles_body(const St& s, const double* f, i
     }
     __syncthreads();
     if (gt == 0 && live) pcnt[c] = s_np[grp];
+    // (a side pushes at most one entry to each list)
+    sa.drain(NT);
+    sc2.drain(NT);
+    sc1.drain(NT);
+    sw2.drain(NT);
+    sw1.drain(NT);
   }
   sa.flush();
   sc2.flush();
@@ -3118,7 +3146,7 @@ __global__ void k_wn_reset(St s) {
 __global__ void __launch_bounds__(256) k_fixpath(St s, int fast) {
   STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
   int n2 = s.ctr[C_NFC], n1 = s.ctr[C_NFC1];
-  block_rounds(n2 + n1, 256, [&](long long ee) {
+  block_rounds(n2 + n1, 1, [&](long long ee) {
     const int e = (int)ee;
     int dir = e < n2 ? 0 : 1;
     int cur = dir == 0 ? s.wnl2[e] : s.wnl1[e - n2];
@@ -3192,7 +3220,7 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
     s.ctr[side == 2 ? C_WMC2 : C_WMINC] = 0;
   }
   // (at most two proposals per listed side)
-  block_rounds(n, 512, [&](long long kk) {
+  block_rounds(n, 2, [&](long long kk) {
     const int k = (int)kk;
     const int i = WL[k];
     int sv = S[i];
@@ -3329,7 +3357,7 @@ __global__ void __launch_bounds__(256) k_wrong_saddle(St s, int side, int do_fix
   int dg = s.ctr[side == 2 ? C_DGMAX : C_DGMIN];
   STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
   STAGED_COUNT(cnt, &s.ctr[side == 2 ? C_WSC : C_WSCJ]);
-  block_rounds(n, 256, [&](long long k) {
+  block_rounds(n, 1, [&](long long k) {
     const int i = (int)k;
     int m = E[i];
     int ov = m == og ? kNone : os[i];
