@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/extremz.h"
@@ -382,7 +383,10 @@ class Run {
     const volatile int* flag = c_->h_ctr + C_COUNT_;
     for (long long spin = 1; __atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq; spin++) {
       if (spin % 256) {
-        __builtin_ia32_pause();
+        // a short wait spins; past ~0.1 ms (a full-grid pass) the thread
+        // gives its core to others between polls
+        if (spin < 4096) __builtin_ia32_pause();
+        else std::this_thread::yield();
         continue;
       }
       // every few microseconds: a failed kernel ends the wait with its error,
