@@ -349,8 +349,7 @@ __global__ void k_wrong_saddle(St s, int side, int do_fix, int gate);
 __global__ void k_saddle_gate(int* ctr, int gate, int wsc);
 __global__ void k_lists_count(St s, int* cnt, int nb, int tiled);
 __global__ void k_lists_totals(St s, const int* cnt, int nb);
-__global__ void k_lists_write(St s, const int* cnt, int nb, int tiled, int index);
-__global__ void k_presnap(St s);
+__global__ void k_lists_write(St s, const int* cnt, int nb, int tiled, int index, int snap);
 __global__ void k_argext(const double* f, int n, int* slot_min, int* slot_max);
 __global__ void k_count_edited(const uint8_t* cnt, long long n, int* ctr);
 __global__ void k_bitmap3(const double* before, const double* after, const uint8_t* cnt,

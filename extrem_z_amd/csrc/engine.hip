@@ -405,7 +405,7 @@ class Run {
   int h(int idx) const { return c_->h_ctr[idx]; }
 
   void allocate(long long N);
-  void list_compaction(int index);
+  void list_compaction(int index, int snap = 0);
   void slot_pass(int dec, const double* f, const int* L2, const int* L1, const uint16_t* flow,
                  int* F2, int* La2, int* F1, int* La1, int* smax, int* smin);
   void apply(long long bound = 1LL << 62, int also = -1);
@@ -429,7 +429,7 @@ class Run {
 
 // The CP lists S2 / S1 / MX / MN of the types in s_.otype, in the column
 // order of k_lists_count (32 x 32 columns for fields at least that wide).
-void Run::list_compaction(int index) {
+void Run::list_compaction(int index, int snap) {
   const Geo& g = s_.g;
   const int tiled = g.W >= 32 && g.H >= 32;
   const long long K =
@@ -448,7 +448,7 @@ void Run::list_compaction(int index) {
   k_scan_single<<<1, 1024, 0, st()>>>(sums, nblk);
   k_scan_add<<<nblk, 1024, 0, st()>>>(cnt, n1, sums);
   k_lists_totals<<<1, 64, 0, st()>>>(s_, cnt, nb);
-  k_lists_write<<<nb, 256, 0, st()>>>(s_, cnt, nb, tiled, index);
+  k_lists_write<<<nb, 256, 0, st()>>>(s_, cnt, nb, tiled, index, snap);
 }
 
 void Run::allocate(long long N) {
@@ -1279,7 +1279,9 @@ int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double
   // types) comes out of its own label pass
   label_pass(d_orig, s_.oDS, s_.oAS, s_.olow, s_.opdir, s_.otype);
   // main.cpp:100-119: CP lists, triplets, largest/smallest saddle maps of orig
-  list_compaction(1);  // with the saddle positions and the extremum index
+  // with the saddle positions, the extremum index and (main.cpp:74-75,
+  // 122-131; a caller that drives the steps snaps itself) the pre-snap
+  list_compaction(1, presnap ? 1 : 0);
   pull();
   s_.n2 = h(C_N2);
   s_.n1 = h(C_N1);
@@ -1341,8 +1343,6 @@ int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double
 
   slot_pass(0, d_orig, s_.oDS, s_.oAS, s_.olow, s_.of2, s_.ol2, s_.of1, s_.ol1, s_.olarg,
             s_.osmall);
-  // main.cpp:74-75, 122-131 (a caller that drives the steps snaps itself)
-  if (presnap) k_presnap<<<(int)((N + 255) / 256), 256, 0, q>>>(s_);
   return 0;
 }
 

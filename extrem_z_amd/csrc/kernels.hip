@@ -3441,9 +3441,11 @@ __global__ void k_lists_totals(St s, const int* cnt, int nb) {
 // With index, every vertex also gets its list positions: spos2 / spos1 (-1
 // off the saddle lists) and ext_id (2 * position in MX, 2 * position in MN +
 // 1, -1 elsewhere); the traversal visits each vertex once, so nothing needs
-// clearing first.
+// clearing first.  With snap, the pre-snap of main.cpp:74-75 and 122-131 is
+// made on the way: decp := orig - xi at every original saddle (types 1-3) and
+// at the global minimum and maximum.
 __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb, int tiled,
-                                                     int index) {
+                                                     int index, int snap) {
   __shared__ int w[4][4];
   const int v = list_vertex(s.g, blockIdx.x * 256LL + threadIdx.x, tiled);
   const unsigned fl = list_flags(s, v);
@@ -3470,17 +3472,7 @@ __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int n
     s.spos1[v] = pos[1];
     s.ext_id[v] = pos[2] >= 0 ? 2 * pos[2] : (pos[3] >= 0 ? 2 * pos[3] + 1 : -1);
   }
-}
-
-// main.cpp:74-75 and 122-131: decp := orig - xi at the global min / max and
-// at every original 1- and 2-saddle.
-__global__ void k_presnap(St s) {
-  // one lane per voxel (coalesced; the saddles are a quarter of a rough
-  // field); the listed saddles are the vertices of types 1-3
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= s.g.N) return;
-  const int t = s.otype[v];
-  if ((t >= 1 && t <= 3) || v == s.ctr[C_GMIN] || v == s.ctr[C_GMAX])
+  if (snap && v >= 0 && ((fl & 3u) || v == s.ctr[C_GMIN] || v == s.ctr[C_GMAX]))
     s.decp[v] = s.f32 ? (double)((float)s.orig[v] - (float)s.xi) : s.orig[v] - s.xi;
 }
 
