@@ -280,10 +280,13 @@ __global__ void k_f32_to_f64(const float* in, double* out, long long n);
 __global__ void k_check_finite(const double* a, long long n, int* ctr, int f32);
 __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* type,
                            uint16_t* lower);
-template <int MODE, bool FAST>
+// rows per k_scan_propose block (64 x kScanTY threads)
+constexpr int kScanTY = 16;
+// TY rows per block: 4 (64 x 4 threads)
+template <int MODE, bool FAST, int TY>
 __global__ void k_mask_cols(St s, const double* f, uint8_t* type, uint16_t* lower, int* front,
                             int fctr, int nzc, int zlen);
-template <bool FAST>
+template <bool FAST, int TY>
 __global__ void k_scan_propose(St s, int* front, int fctr, int nzc, int zlen);
 __global__ void k_check_list(St s, const int* list, int lctr, int with_nbrs, int* front,
                              int fctr, int epoch, int clear_dirty, int zero_idx);

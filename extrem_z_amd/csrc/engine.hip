@@ -278,14 +278,14 @@ class Run {
   }
   // 64x4 row blocks (k_classify)
   int row_blocks() const { return ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4) * s_.g.D; }
-  // 64x4 columns cut into nzc chunks of zlen planes, about kColBlocks blocks
-  // (k_mask_cols)
+  // 64 x ty columns cut into nzc chunks of zlen planes, about target blocks
+  // (k_mask_cols: 64 x 4, k_scan_propose: 64 x kScanTY)
   struct ColGrid {
     int blocks, nzc, zlen;
   };
-  ColGrid col_grid() const {
-    const int ncol = ((s_.g.W + 63) / 64) * ((s_.g.H + 3) / 4);
-    int nzc = std::max(1, std::min(s_.g.D, (kColBlocks + ncol - 1) / ncol));
+  ColGrid col_grid(int ty = 4, int target = kColBlocks) const {
+    const int ncol = ((s_.g.W + 63) / 64) * ((s_.g.H + ty - 1) / ty);
+    int nzc = std::max(1, std::min(s_.g.D, (target + ncol - 1) / ncol));
     const int zlen = (s_.g.D + nzc - 1) / nzc;
     nzc = (s_.g.D + zlen - 1) / zlen;
     return {ncol * nzc, nzc, zlen};
@@ -294,7 +294,7 @@ class Run {
   void check_full(int* fbuf, int fctr) {
     const ColGrid cg = col_grid();
     tbegin(EXTREMZ_K_CHECK_FULL);
-    auto* kern = s_.g.planar ? k_mask_cols<1, false> : k_mask_cols<1, true>;
+    auto* kern = s_.g.planar ? k_mask_cols<1, false, 4> : k_mask_cols<1, true, 4>;
     kern<<<cg.blocks, dim3(64, 4, 1), 0, st()>>>(s_, s_.decp, nullptr, nullptr, fbuf, fctr,
                                                  cg.nzc, cg.zlen);
     tend();
@@ -302,10 +302,10 @@ class Run {
   // the same scan with the frontier's proposals made (delta and the touched
   // list)
   void scan_propose(int* fbuf, int fctr) {
-    const ColGrid cg = col_grid();
+    const ColGrid cg = col_grid(kScanTY, kScanBlocks);
     tbegin(EXTREMZ_K_SCAN_PROPOSE);
-    auto* kern = s_.g.planar ? k_scan_propose<false> : k_scan_propose<true>;
-    kern<<<cg.blocks, dim3(64, 4, 1), 0, st()>>>(s_, fbuf, fctr, cg.nzc, cg.zlen);
+    auto* kern = s_.g.planar ? k_scan_propose<false, kScanTY> : k_scan_propose<true, kScanTY>;
+    kern<<<cg.blocks, dim3(64, kScanTY, 1), 0, st()>>>(s_, fbuf, fctr, cg.nzc, cg.zlen);
     tend();
   }
   // 64x4xkLabelTileZ label tiles
@@ -327,6 +327,10 @@ class Run {
   static constexpr int kSaddleBlocks = 4096;
   // Blocks of a column sweep (one frontier append each).
   static constexpr int kColBlocks = 4096;
+  // k_scan_propose: 64 x kScanTY columns (one 1024-thread block per CU at its
+  // register use), about kScanBlocks blocks.  Rows of 4 / 8 / 16: 2.97 /
+  // 2.70 / 2.53 ms per 512^3 scan (fewer halo words per owned voxel)
+  static constexpr int kScanBlocks = 512;
   // Blocks of the incremental saddle launches (sizes known on the device only).
   static constexpr int kIncSaddleBlocks = 2048;
   // Wrong-neighbour reservation rounds launched per read of the list length.

@@ -920,28 +920,29 @@ template __global__ void k_labels_tile<false, false>(Geo, const double*, int*, i
 // false-critical-point scan of get_false_criticle_points (src/ExtreM.cpp:
 // 1513-1519), appending to the frontier, one append per block; the
 // original field's classification comes out of its label pass.
-template <int MODE, bool FAST>
-__global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_t* type,
-                                                   uint16_t* lower, int* front, int fctr,
-                                                   int nzc, int zlen) {
-  __shared__ double ring[3][kPN];
+template <int MODE, bool FAST, int TY>
+__global__ void __launch_bounds__(64 * TY) k_mask_cols(St s, const double* f, uint8_t* type,
+                                                       uint16_t* lower, int* front, int fctr,
+                                                       int nzc, int zlen) {
+  constexpr int NT = 64 * TY, PN = kPX * (TY + 2), PPer = (PN + NT - 1) / NT;
+  __shared__ double ring[3][PN];
   STAGED_LIST(fl, int, 2048, front, &s.ctr[fctr]);
   const Geo& g = s.g;
-  const int ntx = (g.W + kLTX - 1) / kLTX;
+  const int ntx = (g.W + 63) / 64;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int col = L / nzc;
-  const int x0 = (col % ntx) * kLTX, y0 = (col / ntx) * kLTY;
+  const int x0 = (col % ntx) * 64, y0 = (col / ntx) * TY;
   const int z0 = (L % nzc) * zlen, z1 = min(g.D, z0 + zlen);
   const int lx = threadIdx.x, ly = threadIdx.y, tid = lx + 64 * ly;
-  double pre[kPPer];
+  double pre[PPer];
   auto fetch = [&](int z) {
 #pragma unroll
-    for (int j = 0; j < kPPer; j++) {
-      int i = tid + 256 * j;
+    for (int j = 0; j < PPer; j++) {
+      int i = tid + NT * j;
       // FAST: a neighbour outside the field reads NaN, so every compare with it
       // is false and no class key is needed to leave it out
       double val = FAST ? __longlong_as_double(0x7ff8000000000000LL) : 0.0;
-      if (i < kPN) {
+      if (i < PN) {
         int px = i % kPX, py = i / kPX, gx = x0 + px - 1, gy = y0 + py - 1;
         if (z >= 0 && z < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
           val = f[gx + g.W * (gy + g.H * z)];
@@ -951,9 +952,9 @@ __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_
   };
   auto store = [&](int slot) {
 #pragma unroll
-    for (int j = 0; j < kPPer; j++) {
-      int i = tid + 256 * j;
-      if (i < kPN) ring[slot][i] = pre[j];
+    for (int j = 0; j < PPer; j++) {
+      int i = tid + NT * j;
+      if (i < PN) ring[slot][i] = pre[j];
     }
   };
   if (z0 < z1) {
@@ -972,7 +973,7 @@ __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_
     // a plane pushes at most one entry per owned voxel: the buffer is emptied
     // before it could fill (a dense frontier, up to most of the grid, would
     // otherwise append with a global atomic per wavefront on one counter)
-    fl.drain(256);
+    fl.drain(NT);
     if (z + 2 < z1 + 1) fetch(z + 2);
     const double* P0 = ring[lz % 3];        // z - 1
     const double* P1 = ring[(lz + 1) % 3];  // z
@@ -999,10 +1000,10 @@ __global__ void __launch_bounds__(256) k_mask_cols(St s, const double* f, uint8_
   }
   fl.flush();
 }
-template __global__ void k_mask_cols<1, true>(St, const double*, uint8_t*, uint16_t*, int*, int,
-                                              int, int);
-template __global__ void k_mask_cols<1, false>(St, const double*, uint8_t*, uint16_t*, int*, int,
-                                               int, int);
+template __global__ void k_mask_cols<1, true, 4>(St, const double*, uint8_t*, uint16_t*, int*,
+                                                 int, int, int);
+template __global__ void k_mask_cols<1, false, 4>(St, const double*, uint8_t*, uint16_t*, int*,
+                                                  int, int, int);
 
 // The full false-critical-point scan of decp (k_mask_cols<1>) fused with the
 // c_loops proposals of the frontier it finds (k_propose_c over that list).
@@ -1118,34 +1119,40 @@ __device__ __forceinline__ unsigned out_proposals(const St& s, const double* Q0,
 // ring slot (j - z0 + 3) % 5, the words of plane j in sring slot
 // (j - z0 + 1) % 4; each slot is rewritten only a barrier after its last
 // reader.
-template <bool FAST>
-__global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr, int nzc,
-                                                      int zlen) {
-  __shared__ double ring[5][kQN];
-  __shared__ unsigned sring[4][kPN];
+// TY rows of 64 columns per block (64 * TY threads); the word planes have a
+// halo of 1, the decp planes a halo of 2.
+template <bool FAST, int TY>
+__global__ void __launch_bounds__(64 * TY) k_scan_propose(St s, int* front, int fctr, int nzc,
+                                                          int zlen) {
+  constexpr int NT = 64 * TY;
+  constexpr int PN = kPX * (TY + 2), QN = kQX * (TY + 4);
+  constexpr int PPer = (PN + NT - 1) / NT, QPer = (QN + NT - 1) / NT;
+  static_assert(PPer == 2, "two word-plane entries per thread");
+  __shared__ double ring[5][QN];
+  __shared__ unsigned sring[4][PN];
   STAGED_LIST(fl, int, 1024, front, &s.ctr[fctr]);
   STAGED_LIST(tl, int, kStage, s.touched, &s.ctr[s.tctr]);
   const Geo& g = s.g;
-  const int ntx = (g.W + kLTX - 1) / kLTX;
+  const int ntx = (g.W + 63) / 64;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int col = L / nzc;
-  const int x0 = (col % ntx) * kLTX, y0 = (col / ntx) * kLTY;
+  const int x0 = (col % ntx) * 64, y0 = (col / ntx) * TY;
   const int z0 = (L % nzc) * zlen, z1 = min(g.D, z0 + zlen);
   const int lx = threadIdx.x, ly = threadIdx.y, tid = lx + 64 * ly;
   const int x = x0 + lx, y = y0 + ly;
   const bool own_in = x < g.W && y < g.H;
   // prefetched a step ahead: a decp plane (halo 2), the original lower masks
   // and types of a word plane (halo 1), orig of the owned voxel of a gather plane
-  double pd[kQPer];
-  unsigned pol[2];
-  int pot[2];
+  double pd[QPer];
+  unsigned pol[PPer];
+  int pot[PPer];
   double pog = 0.0;
   auto fetch = [&](int z) {
 #pragma unroll
-    for (int j = 0; j < kQPer; j++) {
-      const int i = tid + 256 * j;
+    for (int j = 0; j < QPer; j++) {
+      const int i = tid + NT * j;
       double a = FAST ? __longlong_as_double(0x7ff8000000000000LL) : 0.0;  // outside: NaN
-      if (i < kQN) {
+      if (i < QN) {
         const int gx = x0 + i % kQX - 2, gy = y0 + i / kQX - 2;
         if (z >= 0 && z < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
           a = s.decp[gx + g.W * (gy + g.H * z)];
@@ -1155,18 +1162,18 @@ __global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr
   };
   auto store = [&](int slot) {
 #pragma unroll
-    for (int j = 0; j < kQPer; j++) {
-      const int i = tid + 256 * j;
-      if (i < kQN) ring[slot][i] = pd[j];
+    for (int j = 0; j < QPer; j++) {
+      const int i = tid + NT * j;
+      if (i < QN) ring[slot][i] = pd[j];
     }
   };
   auto fetch_words = [&](int z) {
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int i = tid + 256 * j;
+    for (int j = 0; j < PPer; j++) {
+      const int i = tid + NT * j;
       unsigned a = 0;
       int b = 0;
-      if (i < kPN) {
+      if (i < PN) {
         const int gx = x0 + i % kPX - 1, gy = y0 + i / kPX - 1;
         if (z >= 0 && z < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H) {
           const int v = gx + g.W * (gy + g.H * z);
@@ -1199,8 +1206,8 @@ __global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr
     const double og = pog;
     __syncthreads();
     // a step pushes at most one entry per owned voxel to each list
-    fl.drain(256);
-    tl.drain(256);
+    fl.drain(NT);
+    tl.drain(NT);
     if (p + 1 <= z1) {
       fetch(p + 2);
       fetch_words(p + 1);
@@ -1212,8 +1219,8 @@ __global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr
       const double* Q2 = ring[(k + 3) % 5];
 #pragma unroll 1
       for (int j = 0; j < 2; j++) {
-        const int i = tid + 256 * j;
-        if (i < kPN) {
+        const int i = tid + NT * j;
+        if (i < PN) {
           const int sx = i % kPX, sy = i / kPX, gx = x0 + sx - 1, gy = y0 + sy - 1;
           unsigned w = 0;
           if (p >= 0 && p < g.D && gx >= 0 && gx < g.W && gy >= 0 && gy < g.H)
@@ -1261,8 +1268,8 @@ __global__ void __launch_bounds__(256) k_scan_propose(St s, int* front, int fctr
   fl.flush();
   tl.flush();
 }
-template __global__ void k_scan_propose<true>(St, int*, int, int, int);
-template __global__ void k_scan_propose<false>(St, int*, int, int, int);
+template __global__ void k_scan_propose<true, kScanTY>(St, int*, int, int, int);
+template __global__ void k_scan_propose<false, kScanTY>(St, int*, int, int, int);
 
 // Follows every pointer out of a tile to its root (D7: every vertex is
 // compressed).  A label without the top bit (kExit) is a root: found inside
