@@ -295,9 +295,19 @@ __global__ void k_check_two(St s, const int* flist, int nf, int* front, int fctr
                             int zero_idx);
 __global__ void k_propose_c(St s, const int* front, int fctr);
 __global__ void k_apply(St s, int zero_idx);
+// The pre-snap a type-writing label pass of the original field makes on the
+// way: decp := orig - xi (in float32 under the float32 rule set) at every
+// saddle and at the global minimum and maximum (ctr[C_GMIN], ctr[C_GMAX]).
+// decp == nullptr: none.
+struct Presnap {
+  double* decp;
+  double xi;
+  int f32;
+  const int* ctr;
+};
 template <bool FAST, bool TYPE>
 __global__ void k_labels_tile(Geo g, const double* f, int* ds, int* as, uint16_t* lower,
-                              uint8_t* pdir, const uint8_t* lut, uint8_t* type);
+                              uint8_t* pdir, const uint8_t* lut, uint8_t* type, Presnap ps);
 __global__ void k_chase(int n, int* ds, int* as);
 __global__ void k_list_fill(const int* list, int n, int* arr, int val);
 template <int MODE>
@@ -352,8 +362,9 @@ __global__ void k_wrong_saddle(St s, int side, int do_fix, int gate);
 __global__ void k_saddle_gate(int* ctr, int gate, int wsc);
 __global__ void k_lists_count(St s, int* cnt, int nb, int tiled);
 __global__ void k_lists_totals(St s, const int* cnt, int nb);
-__global__ void k_lists_write(St s, const int* cnt, int nb, int tiled, int index, int snap);
-__global__ void k_argext(const double* f, int n, int* slot_min, int* slot_max);
+__global__ void k_lists_write(St s, const int* cnt, int nb, int tiled, int index);
+__global__ void k_argext(const double* f, long long n, int* bmin, int* bmax, int* ctr, int check,
+                         int f32);
 __global__ void k_count_edited(const uint8_t* cnt, long long n, int* ctr);
 __global__ void k_bitmap3(const double* before, const double* after, const uint8_t* cnt,
                           long long n, uint8_t* out, long long nbytes);

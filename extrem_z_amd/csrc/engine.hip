@@ -408,14 +408,25 @@ class Run {
   }
   int h(int idx) const { return c_->h_ctr[idx]; }
 
+  // The global minimum / maximum of f by (value, id) into ctr[C_GMIN] /
+  // ctr[C_GMAX]; with check, a value that is not finite (or not a float32
+  // under the float32 rules) sets ctr[C_FAIL].
+  void argext(const double* f, int check) {
+    c_->bmax2.ensure(kExtBlocks);
+    c_->bmin2.ensure(kExtBlocks);
+    const int nb = std::min(blocks_for(s_.g.N), kExtBlocks);
+    k_argext<<<nb, 256, 0, st()>>>(f, s_.g.N, c_->bmin2.p, c_->bmax2.p, s_.ctr, check, s_.f32);
+    k_ext_reduce<<<1, 256, 0, st()>>>(c_->bmax2.p, nb, c_->bmin2.p, nb, f, s_.ctr + C_GMAX,
+                                      s_.ctr + C_GMIN);
+  }
   void allocate(long long N);
-  void list_compaction(int index, int snap = 0);
+  void list_compaction(int index);
   void slot_pass(int dec, const double* f, const int* L2, const int* L1, const uint16_t* flow,
                  int* F2, int* La2, int* F1, int* La1, int* smax, int* smin);
   void apply(long long bound = 1LL << 62, int also = -1);
   void dirty_consumed();
   void label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir = nullptr,
-                  uint8_t* type = nullptr);
+                  uint8_t* type = nullptr, Presnap ps = Presnap{});
   void c_loops(bool entered = false);
   void c_loops_entry();
   bool proposed_ = false;  // the frontier's proposals are made already
@@ -433,7 +444,7 @@ class Run {
 
 // The CP lists S2 / S1 / MX / MN of the types in s_.otype, in the column
 // order of k_lists_count (32 x 32 columns for fields at least that wide).
-void Run::list_compaction(int index, int snap) {
+void Run::list_compaction(int index) {
   const Geo& g = s_.g;
   const int tiled = g.W >= 32 && g.H >= 32;
   const long long K =
@@ -452,7 +463,7 @@ void Run::list_compaction(int index, int snap) {
   k_scan_single<<<1, 1024, 0, st()>>>(sums, nblk);
   k_scan_add<<<nblk, 1024, 0, st()>>>(cnt, n1, sums);
   k_lists_totals<<<1, 64, 0, st()>>>(s_, cnt, nb);
-  k_lists_write<<<nb, 256, 0, st()>>>(s_, cnt, nb, tiled, index, snap);
+  k_lists_write<<<nb, 256, 0, st()>>>(s_, cnt, nb, tiled, index);
 }
 
 void Run::allocate(long long N) {
@@ -598,7 +609,7 @@ void Run::dirty_consumed() { s_.dctr = s_.dctr == C_DIRTY ? C_DIRTY2 : C_DIRTY; 
 // round before the chase costs more than the chase saves).  No host round
 // trip.
 void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t* pdir,
-                     uint8_t* type) {
+                     uint8_t* type, Presnap ps) {
   counts_[EXTREMZ_STAT_LABEL_PASSES]++;
   tbegin(EXTREMZ_K_LABEL_TILE);
   // every voxel on the ordered scan: W >= 3 on the stencil
@@ -606,7 +617,7 @@ void Run::label_pass(const double* f, int* ds, int* as, uint16_t* lower, uint8_t
   auto* kern = fast ? (type ? k_labels_tile<true, true> : k_labels_tile<true, false>)
                     : (type ? k_labels_tile<false, true> : k_labels_tile<false, false>);
   kern<<<tile_blocks(), dim3(64, 4, 1), 0, st()>>>(s_.g, f, ds, as, lower, pdir, c_->lut.p,
-                                                   type);
+                                                   type, ps);
   tend();
   int nb = (s_.g.N + 255) / 256;
   tbegin(EXTREMZ_K_LABEL_CHASE);
@@ -1001,8 +1012,7 @@ void Run::split_tree(const double* d_f, int W, int H, int D, TreeOut& out) {
   k_classify<<<row_blocks(), dim3(64, 4, 1), 0, q>>>(s_.g, s_.lut, d_f, s_.otype, s_.olow);
   label_pass(d_f, c->oDS.p, c->oAS.p, nullptr);
   list_compaction(0);
-  set_ctr2(C_GMIN, C_GMAX, kNone);
-  k_argext<<<blocks_for(N), 256, 0, q>>>(d_f, (int)N, s_.ctr + C_GMIN, s_.ctr + C_GMAX);
+  argext(d_f, 0);
   pull();
   const int n2 = h(C_N2), nmax = h(C_NMAX), gmin = h(C_GMIN);
   out = TreeOut{};
@@ -1261,7 +1271,8 @@ int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double
   hipStream_t q = st();
   const int bN = blocks_for(N);
   ez::hip_check(hipMemsetAsync(s_.ctr, 0, sizeof(int) * C_COUNT_, q), "memset");
-  k_check_finite<<<bN, 256, 0, q>>>(d_orig, N, s_.ctr, s_.f32);
+  // main.cpp:67-75 (the global extrema of orig), with the check of orig
+  argext(d_orig, 1);
   k_check_finite<<<bN, 256, 0, q>>>(d_decp, N, s_.ctr, s_.f32);
   k_fill_i64<<<bN, 256, 0, q>>>(s_.delta, s_.sent, N);
   ez::hip_check(hipMemsetAsync(s_.cnt, 0, N, q), "memset");
@@ -1273,19 +1284,17 @@ int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double
     ez::hip_check(hipMemsetAsync(p, 0, N, q), "memset");
   for (int* p : {s_.minsad2, s_.minsad1, s_.win_a, s_.win_b})
     k_fill_i32<<<bN, 256, 0, q>>>(p, 0x7fffffff, N);
-  set_ctr2(C_GMIN, C_GMAX, kNone);
   pull();
   if (h(C_FAIL)) return EXTREMZ_EINVAL;
 
-  // main.cpp:67-75
-  k_argext<<<bN, 256, 0, q>>>(d_orig, N, s_.ctr + C_GMIN, s_.ctr + C_GMAX);
   // main.cpp:84-88: the original field's classification (lower masks and
-  // types) comes out of its own label pass
-  label_pass(d_orig, s_.oDS, s_.oAS, s_.olow, s_.opdir, s_.otype);
+  // types) comes out of its own label pass, which also makes the pre-snap
+  // (main.cpp:74-75, 122-131; a caller that drives the steps snaps itself)
+  label_pass(d_orig, s_.oDS, s_.oAS, s_.olow, s_.opdir, s_.otype,
+             presnap ? Presnap{s_.decp, s_.xi, s_.f32, s_.ctr} : Presnap{});
   // main.cpp:100-119: CP lists, triplets, largest/smallest saddle maps of orig
-  // with the saddle positions, the extremum index and (main.cpp:74-75,
-  // 122-131; a caller that drives the steps snaps itself) the pre-snap
-  list_compaction(1, presnap ? 1 : 0);
+  // with the saddle positions and the extremum index
+  list_compaction(1);
   pull();
   s_.n2 = h(C_N2);
   s_.n1 = h(C_N1);

@@ -713,7 +713,8 @@ constexpr unsigned kFaceXN = 0x1442u, kFaceXP = 0x2881u, kFaceYN = 0x2288u, kFac
 template <bool FAST, bool TYPE>
 __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int* ds, int* as,
                                                      uint16_t* lower, uint8_t* pdir,
-                                                     const uint8_t* lut, uint8_t* type) {
+                                                     const uint8_t* lut, uint8_t* type,
+                                                     Presnap ps) {
   __shared__ double ring[3][kPN];
   __shared__ int sup[kLTN];
   __shared__ int sdn[kLTN];
@@ -852,6 +853,10 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
           t = type_of_mask(lut, cls, lm);
         }
         type[v] = (uint8_t)t;
+        // the pre-snap (main.cpp:74-75, 122-131): decp := orig - xi at every
+        // saddle (types 1-3) and at the global minimum and maximum
+        if (ps.decp && ((t >= 1 && t <= 3) || v == ps.ctr[C_GMIN] || v == ps.ctr[C_GMAX]))
+          ps.decp[v] = ps.f32 ? (double)((float)c - (float)ps.xi) : c - ps.xi;
       }
       if (pdir) pdir[v] = (uint8_t)((du + 1) | ((dd + 1) << 4));
       unsigned inside = in_xy;
@@ -903,13 +908,13 @@ __global__ void __launch_bounds__(256) k_labels_tile(Geo g, const double* f, int
   }
 }
 template __global__ void k_labels_tile<true, true>(Geo, const double*, int*, int*, uint16_t*,
-                                                   uint8_t*, const uint8_t*, uint8_t*);
+                                                   uint8_t*, const uint8_t*, uint8_t*, Presnap);
 template __global__ void k_labels_tile<true, false>(Geo, const double*, int*, int*, uint16_t*,
-                                                    uint8_t*, const uint8_t*, uint8_t*);
+                                                    uint8_t*, const uint8_t*, uint8_t*, Presnap);
 template __global__ void k_labels_tile<false, true>(Geo, const double*, int*, int*, uint16_t*,
-                                                    uint8_t*, const uint8_t*, uint8_t*);
+                                                    uint8_t*, const uint8_t*, uint8_t*, Presnap);
 template __global__ void k_labels_tile<false, false>(Geo, const double*, int*, int*, uint16_t*,
-                                                     uint8_t*, const uint8_t*, uint8_t*);
+                                                     uint8_t*, const uint8_t*, uint8_t*, Presnap);
 
 // Full-grid lower masks through the same LDS ring as k_labels_tile: a block
 // marches one 64x4 column through zlen planes (z chunk zc of the column), so
@@ -3448,11 +3453,9 @@ __global__ void k_lists_totals(St s, const int* cnt, int nb) {
 // With index, every vertex also gets its list positions: spos2 / spos1 (-1
 // off the saddle lists) and ext_id (2 * position in MX, 2 * position in MN +
 // 1, -1 elsewhere); the traversal visits each vertex once, so nothing needs
-// clearing first.  With snap, the pre-snap of main.cpp:74-75 and 122-131 is
-// made on the way: decp := orig - xi at every original saddle (types 1-3) and
-// at the global minimum and maximum.
+// clearing first.
 __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb, int tiled,
-                                                     int index, int snap) {
+                                                     int index) {
   __shared__ int w[4][4];
   const int v = list_vertex(s.g, blockIdx.x * 256LL + threadIdx.x, tiled);
   const unsigned fl = list_flags(s, v);
@@ -3479,40 +3482,62 @@ __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int n
     s.spos1[v] = pos[1];
     s.ext_id[v] = pos[2] >= 0 ? 2 * pos[2] : (pos[3] >= 0 ? 2 * pos[3] + 1 : -1);
   }
-  if (snap && v >= 0 && ((fl & 3u) || v == s.ctr[C_GMIN] || v == s.ctr[C_GMAX]))
-    s.decp[v] = s.f32 ? (double)((float)s.orig[v] - (float)s.xi) : s.orig[v] - s.xi;
 }
 
-__global__ void k_argext(const double* f, int n, int* slot_min, int* slot_max) {
-  int bmin = kNone, bmax = kNone;
+// Global minimum and maximum of f by (value, id) (main.cpp:67-75), first
+// stage: each block's best into bmin / bmax[blockIdx.x]; k_ext_reduce over
+// those with one block finishes (no atomics on the two result words).  With
+// check, a value that is not finite (or, with f32, not a float32) sets
+// ctr[C_FAIL]: the same read serves the input check.
+__global__ void __launch_bounds__(256) k_argext(const double* f, long long n, int* bmin,
+                                                int* bmax, int* ctr, int check, int f32) {
+  int cmin = kNone, cmax = kNone;
   double fmin = 0.0, fmax = 0.0;
-  GRID_STRIDE(v, n) {
-    double fv = f[v];
-    if (bmin == kNone || smaller(fv, v, fmin, bmin)) { bmin = v; fmin = fv; }
-    if (bmax == kNone || larger(fv, v, fmax, bmax)) { bmax = v; fmax = fv; }
+  bool bad = false;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double fv = f[i];
+    const int v = (int)i;
+    if (check) bad |= !isfinite(fv) || (f32 && (double)(float)fv != fv);
+    if (cmin == kNone || smaller(fv, v, fmin, cmin)) { cmin = v; fmin = fv; }
+    if (cmax == kNone || larger(fv, v, fmax, cmax)) { cmax = v; fmax = fv; }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    int b = __shfl_xor(bmin, o);
-    double fb = __shfl_xor(fmin, o);
-    if (b != kNone && (bmin == kNone || smaller(fb, b, fmin, bmin))) { bmin = b; fmin = fb; }
-    b = __shfl_xor(bmax, o);
-    fb = __shfl_xor(fmax, o);
-    if (b != kNone && (bmax == kNone || larger(fb, b, fmax, bmax))) { bmax = b; fmax = fb; }
-  }
-  if (__lane_id() == 0 && bmin != kNone) {
-    cas_best(slot_min, bmin, f, false);
-    cas_best(slot_max, bmax, f, true);
+  if (bad) ctr[C_FAIL] = 1;
+  __shared__ int s_best[4];
+  const int gm = block_best(cmax, fmax, f, true, s_best);
+  const int gn = block_best(cmin, fmin, f, false, s_best);
+  if (threadIdx.x == 0) {
+    bmax[blockIdx.x] = gm;
+    bmin[blockIdx.x] = gn;
   }
 }
 
-// Number of vertices with a non-zero edit count.
-__global__ void k_count_edited(const uint8_t* cnt, long long n, int* ctr) {
+// Number of vertices with a non-zero edit count: 4 counts per load where the
+// array is 4-byte aligned, one atomic per block.
+__global__ void __launch_bounds__(256) k_count_edited(const uint8_t* cnt, long long n, int* ctr) {
+  const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long nt = (long long)gridDim.x * blockDim.x;
   int c = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    c += cnt[i] != 0;
+  long long done = 0;
+  if (((uintptr_t)cnt & 3) == 0) {
+    const unsigned* w = (const unsigned*)cnt;
+    const long long nw = n >> 2;
+    for (long long i = tid; i < nw; i += nt) {
+      const unsigned x = w[i];
+      c += ((x & 0xffu) != 0) + ((x & 0xff00u) != 0) + ((x & 0xff0000u) != 0) +
+           ((x & 0xff000000u) != 0);
+    }
+    done = nw << 2;
+  }
+  for (long long i = done + tid; i < n; i += nt) c += cnt[i] != 0;
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-  if (__lane_id() == 0 && c) atomicAdd(&ctr[C_EDITED], c);
+  __shared__ int s_c[4];
+  if (__lane_id() == 0) s_c[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int t = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+    if (t) atomicAdd(&ctr[C_EDITED], t);
+  }
 }
 
 // --------------------------------------------------------- encoding -----

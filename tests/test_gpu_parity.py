@@ -120,10 +120,12 @@ def test_invariants_at_128cube():
     assert np.array_equal(m0[sad], m1[sad])
 
 
-def test_nonfinite_input_rejected():
+@pytest.mark.parametrize("where,val", [("decp", np.nan), ("orig", np.nan), ("orig", np.inf),
+                                       ("orig", -np.inf)])
+def test_nonfinite_input_rejected(where, val):
     o = np.zeros((4, 4, 4))
     d = o.copy()
-    d[1, 1, 1] = np.nan
+    (d if where == "decp" else o)[1, 1, 1] = val
     with pytest.raises(ez().ExtremzError) as e:
         ez().correct(o, d, 0.1)
     assert e.value.code == -22
