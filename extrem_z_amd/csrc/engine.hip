@@ -449,7 +449,7 @@ void Run::list_compaction(int index) {
   const int tiled = g.W >= 32 && g.H >= 32;
   const long long K =
       tiled ? (long long)((g.W + 31) / 32) * ((g.H + 31) / 32) * 1024LL * g.D : (long long)g.N;
-  const int nb = (int)((K + 255) / 256);
+  const int nb = (int)((K + kListPer - 1) / kListPer);
   // the 4 * nb block counts (plus a zero that receives the total) scanned as
   // one array; block sums after them
   const long long n1 = 4LL * nb + 1;

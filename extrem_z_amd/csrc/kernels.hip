@@ -3420,18 +3420,24 @@ __device__ __forceinline__ unsigned list_flags(const St& s, int v) {
          (unsigned)(t == 4) << 2 | (unsigned)(t == 0) << 3;
 }
 
-// cnt[c * nb + block] = number of category-c vertices in the block (256
-// traversal positions per block; categories 2-saddle, 1-saddle, maximum,
+// cnt[c * nb + block] = number of category-c vertices in the block's
+// kListPer traversal positions (categories 2-saddle, 1-saddle, maximum,
 // minimum).
 __global__ void __launch_bounds__(256) k_lists_count(St s, int* cnt, int nb, int tiled) {
   __shared__ int w[4][4];
-  const int v = list_vertex(s.g, blockIdx.x * 256LL + threadIdx.x, tiled);
-  const unsigned fl = list_flags(s, v);
+  const long long k0 = blockIdx.x * (long long)kListPer + threadIdx.x;
+  int n[4] = {0, 0, 0, 0};
+#pragma unroll 4
+  for (int j = 0; j < kListPer / 256; j++) {
+    const unsigned fl = list_flags(s, list_vertex(s.g, k0 + 256 * j, tiled));
+#pragma unroll
+    for (int c = 0; c < 4; c++) n[c] += (fl >> c) & 1u;
+  }
   const int wave = threadIdx.x >> 6;
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    unsigned long long m = __ballot((fl >> c) & 1u);
-    if (__lane_id() == 0) w[c][wave] = __popcll(m);
+    for (int o = 32; o > 0; o >>= 1) n[c] += __shfl_xor(n[c], o);
+    if (__lane_id() == 0) w[c][wave] = n[c];
   }
   __syncthreads();
   if (threadIdx.x < 4) {
@@ -3456,31 +3462,42 @@ __global__ void k_lists_totals(St s, const int* cnt, int nb) {
 // clearing first.
 __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb, int tiled,
                                                      int index) {
-  __shared__ int w[4][4];
-  const int v = list_vertex(s.g, blockIdx.x * 256LL + threadIdx.x, tiled);
-  const unsigned fl = list_flags(s, v);
+  __shared__ int w[2][4][4];  // per round (alternating): category count per wavefront
   const int wave = threadIdx.x >> 6, lane = __lane_id();
-  unsigned long long m[4];
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    m[c] = __ballot((fl >> c) & 1u);
-    if (lane == 0) w[c][wave] = __popcll(m[c]);
-  }
-  __syncthreads();
+  const long long k0 = blockIdx.x * (long long)kListPer + threadIdx.x;
   int* lists[4] = {s.S2, s.S1, s.MX, s.MN};
-  int pos[4] = {-1, -1, -1, -1};
+  int base[4];  // the block's next position per list, the same in every thread
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    if (!((fl >> c) & 1u)) continue;
-    int base = cnt[(long long)c * nb + blockIdx.x] - cnt[(long long)c * nb];
-    for (int k = 0; k < wave; k++) base += w[c][k];
-    pos[c] = base + __popcll(m[c] & ((1ull << lane) - 1ull));
-    lists[c][pos[c]] = v;
-  }
-  if (index && v >= 0) {
-    s.spos2[v] = pos[0];
-    s.spos1[v] = pos[1];
-    s.ext_id[v] = pos[2] >= 0 ? 2 * pos[2] : (pos[3] >= 0 ? 2 * pos[3] + 1 : -1);
+  for (int c = 0; c < 4; c++) base[c] = cnt[(long long)c * nb + blockIdx.x] - cnt[(long long)c * nb];
+  for (int j = 0; j < kListPer / 256; j++) {
+    const int v = list_vertex(s.g, k0 + 256 * j, tiled);
+    const unsigned fl = list_flags(s, v);
+    unsigned long long m[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      m[c] = __ballot((fl >> c) & 1u);
+      if (lane == 0) w[j & 1][c][wave] = __popcll(m[c]);
+    }
+    // w alternates between rounds: a buffer is rewritten two rounds later,
+    // when every thread has passed the barrier after its reads
+    __syncthreads();
+    int pos[4] = {-1, -1, -1, -1};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int* wc = w[j & 1][c];
+      if ((fl >> c) & 1u) {
+        int b = base[c];
+        for (int k = 0; k < wave; k++) b += wc[k];
+        pos[c] = b + __popcll(m[c] & ((1ull << lane) - 1ull));
+        lists[c][pos[c]] = v;
+      }
+      base[c] += wc[0] + wc[1] + wc[2] + wc[3];
+    }
+    if (index && v >= 0) {
+      s.spos2[v] = pos[0];
+      s.spos1[v] = pos[1];
+      s.ext_id[v] = pos[2] >= 0 ? 2 * pos[2] : (pos[3] >= 0 ? 2 * pos[3] + 1 : -1);
+    }
   }
 }
 
