@@ -137,7 +137,7 @@ struct St {
   uint8_t *wn2, *wn1;             // wrong_neighbors(_ds) flags
   int *wnl2, *wnl1;               // wrong_neighbors(_ds)_index
   int *minsad2, *minsad1;         // greedy reservations
-  int *win_a, *win_b;             // first-saddle-wins reservations
+  int *win_a, *win_b;             // first-saddle-wins reservations, one per extremum slot
   // Incremental CP passes of decp: what changed since the last one, and the
   // per-pass work lists.
   uint8_t* pdir;                  // steepest (up + 1) | (down + 1) << 4 per voxel; 0: itself

@@ -478,7 +478,7 @@ void Run::allocate(long long N) {
   for (DevBuf<int>* b : {&c->oDS, &c->oAS, &c->dDS, &c->dAS, &c->stamp, &c->touched, &c->dirty,
                          &c->frontA, &c->frontB, &c->S2, &c->S1, &c->MX, &c->MN, &c->olarg,
                          &c->osmall, &c->dlarg, &c->dsmall, &c->minsad2, &c->minsad1,
-                         &c->win_a, &c->win_b, &c->of2, &c->ol2, &c->of1, &c->ol1, &c->df2,
+                         &c->of2, &c->ol2, &c->of1, &c->ol1, &c->df2,
                          &c->dl2, &c->df1, &c->dl1, &c->wnl2, &c->wnl1})
     b->ensure(N);
   for (DevBuf<uint8_t>* b : {&c->pdir, &c->opdir, &c->lflag, &c->cflag, &c->tu, &c->td})
@@ -1245,8 +1245,6 @@ int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double
   s_.wnl1 = c->wnl1.p;
   s_.minsad2 = c->minsad2.p;
   s_.minsad1 = c->minsad1.p;
-  s_.win_a = c->win_a.p;
-  s_.win_b = c->win_b.p;
   s_.pdir = c->pdir.p;
   s_.opdir = c->opdir.p;
   s_.lflag = c->lflag.p;
@@ -1282,7 +1280,7 @@ int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double
   ez::hip_check(hipMemsetAsync(s_.wn1, 0, N, q), "memset");
   for (uint8_t* p : {s_.lflag, s_.cflag, s_.tu, s_.td})
     ez::hip_check(hipMemsetAsync(p, 0, N, q), "memset");
-  for (int* p : {s_.minsad2, s_.minsad1, s_.win_a, s_.win_b})
+  for (int* p : {s_.minsad2, s_.minsad1})
     k_fill_i32<<<bN, 256, 0, q>>>(p, 0x7fffffff, N);
   pull();
   if (h(C_FAIL)) return EXTREMZ_EINVAL;
@@ -1337,6 +1335,13 @@ int Run::begin(const double* d_orig, double* d_decp, int W, int H, int D, double
     ez::hip_check(hipMemsetAsync(s_.sstamp, 0, sizeof(int) * (n21 + 1), q), "memset");
     ez::hip_check(hipMemsetAsync(s_.slstamp, 0, sizeof(int) * (ns + 1), q), "memset");
     ez::hip_check(hipMemsetAsync(s_.pstamp, 0, sizeof(int) * (ns + 1), q), "memset");
+    // first-saddle-wins words of the wrong-index steps, one per slot
+    c->win_a.ensure(ns + 1);
+    c->win_b.ensure(ns + 1);
+    s_.win_a = c->win_a.p;
+    s_.win_b = c->win_b.p;
+    k_fill_i32<<<blocks_for(ns + 1), 256, 0, q>>>(s_.win_a, 0x7fffffff, ns + 1);
+    k_fill_i32<<<blocks_for(ns + 1), 256, 0, q>>>(s_.win_b, 0x7fffffff, ns + 1);
   }
   {
     const long long ns = (long long)s_.nmax + s_.nmin;

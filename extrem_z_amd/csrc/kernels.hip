@@ -3215,6 +3215,11 @@ __global__ void __launch_bounds__(256) k_fixpath(St s, int fast) {
 // first saddle (lowest id) per original extremum wins; pairs are compared by
 // vertex id.  phase 0 registers, phase 1 counts (and proposes, s_loops /
 // s_loops_join, src/ExtreM.cpp:2838-2886), phase 2 releases.
+// The registration words (win_a / win_b) are one per listed extremum, at its
+// slot: an original triplet end is an extremum of the original field, so it
+// has a list position.
+__device__ __forceinline__ int end_slot(const St& s, int e) { return slot_index(s, s.ext_id[e]); }
+
 __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, int do_fix) {
   // only the listed sides can have triplet ends that differ
   const int n = s.ctr[side == 2 ? C_WL2 : C_WL1];
@@ -3241,10 +3246,10 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
     // is no mismatch on either side
     bool mf = a != c && a != kNone && c != kNone, ml = b != d && b != kNone && d != kNone;
     if (phase == 0) {
-      if (mf) atomicMin(&s.win_a[a], sv);
-      if (ml) atomicMin(&s.win_b[b], sv);
+      if (mf) atomicMin(&s.win_a[end_slot(s, a)], sv);
+      if (ml) atomicMin(&s.win_b[end_slot(s, b)], sv);
     } else if (phase == 1) {
-      if (mf && s.win_a[a] == sv) {
+      if (mf && s.win_a[end_slot(s, a)] == sv) {
         cf.inc();
         if (side == 2) {  // triplet[0] -> wrong_max_counter, fix dir 0 lowers orig
           if (do_fix) propose(s, a, lower_mid(s, a), tl);
@@ -3252,7 +3257,7 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
           if (do_fix && c != kNone) propose(s, c, lower_mid(s, c), tl);
         }
       }
-      if (ml && s.win_b[b] == sv) {
+      if (ml && s.win_b[end_slot(s, b)] == sv) {
         cl.inc();
         if (side == 2) {  // triplet[last] -> wrong_max_counter_2, fix dir 1 lowers decp's
           if (do_fix && d != kNone) propose(s, d, lower_mid(s, d), tl);
@@ -3261,8 +3266,8 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
         }
       }
     } else {
-      if (mf) s.win_a[a] = 0x7fffffff;
-      if (ml) s.win_b[b] = 0x7fffffff;
+      if (mf) s.win_a[end_slot(s, a)] = 0x7fffffff;
+      if (ml) s.win_b[end_slot(s, b)] = 0x7fffffff;
     }
   }, tl);
   tl.flush();
@@ -3296,8 +3301,8 @@ __global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do
   for (int k = t; k < n; k += 1024) {  // register
     const int i = WL[k];
     const int sv = S[i], a = of[i], b = ol[i];
-    if (differs(a, df[i])) atomicMin(&s.win_a[a], sv);
-    if (differs(b, dl[i])) atomicMin(&s.win_b[b], sv);
+    if (differs(a, df[i])) atomicMin(&s.win_a[end_slot(s, a)], sv);
+    if (differs(b, dl[i])) atomicMin(&s.win_b[end_slot(s, b)], sv);
   }
   __syncthreads();
   for (int k = t; k < n; k += 1024) {  // count and fix
@@ -3305,7 +3310,8 @@ __global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do
     const int sv = S[i];
     const int a = of[i], b = ol[i], c = df[i], d = dl[i];
     if (differs(a, c) &&
-        __hip_atomic_load(&s.win_a[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == sv) {
+        __hip_atomic_load(&s.win_a[end_slot(s, a)], __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT) == sv) {
       cf.inc();
       if (side == 2) {
         if (do_fix) propose(s, a, lower_mid(s, a), tl);
@@ -3314,7 +3320,8 @@ __global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do
       }
     }
     if (differs(b, d) &&
-        __hip_atomic_load(&s.win_b[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == sv) {
+        __hip_atomic_load(&s.win_b[end_slot(s, b)], __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT) == sv) {
       cl.inc();
       if (side == 2) {
         if (do_fix && d != kNone) propose(s, d, lower_mid(s, d), tl);
@@ -3328,9 +3335,11 @@ __global__ void __launch_bounds__(1024) k_wrong_index_one(St s, int side, int do
     const int i = WL[k];
     const int a = of[i], b = ol[i];
     if (differs(a, df[i]))
-      __hip_atomic_store(&s.win_a[a], 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&s.win_a[end_slot(s, a)], 0x7fffffff, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     if (differs(b, dl[i]))
-      __hip_atomic_store(&s.win_b[b], 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&s.win_b[end_slot(s, b)], 0x7fffffff, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
   }
   tl.flush();
   cf.flush();

@@ -31,7 +31,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 // Device bytes of one correction of N voxels, in two parts.  fixed: the
-// per-voxel work arrays every correction sizes by N (240 B, 31 of them the
+// per-voxel work arrays every correction sizes by N (232 B, 31 of them the
 // change lists, pointers and flags of the incremental CP pass) plus the
 // small per-extremum arrays.  var: what depends on the data or the caller's
 // buffers -- the saddle pair records, side and slot lists, stamps, indices
@@ -43,7 +43,7 @@ struct Need {
 };
 Need working_set(const extremz_job& j) {
   const int64_t N = (int64_t)j.W * j.H * j.D;
-  Need n{240 * N + (1 << 20), 92 * N};
+  Need n{232 * N + (1 << 20), 92 * N};
   if (j.on_host || j.orig_dtype == EXTREMZ_DTYPE_F32) n.var += 8 * N;  // orig as float64
   if (j.on_host) n.var += 9 * N;                                       // decp + counts
   return n;

@@ -107,8 +107,8 @@ def test_budget_bounds_the_footprint(cases):
     """A budget that holds one large job at a time: jobs queue, the footprint
     never exceeds the budget, and the results are unchanged."""
     big = max(c[0].size for c in cases)
-    # working set of a host job: 240 B/voxel fixed + 92 B data-dependent + 17 B staging
-    budget = 349 * big + (1 << 20) + (4 << 20)
+    # working set of a host job: 232 B/voxel fixed + 92 B data-dependent + 17 B staging
+    budget = 341 * big + (1 << 20) + (4 << 20)
     with ez().Pool(0, lanes=4, hbm_budget=budget) as pool:
         jobs, keep = [], []
         for orig, decp, xi, _, _ in cases:
