@@ -447,10 +447,12 @@ class Run {
 void Run::list_compaction(int index) {
   const Geo& g = s_.g;
   const int tiled = g.W >= 32 && g.H >= 32;
-  const long long K =
-      tiled ? (long long)((g.W + 31) / 32) * ((g.H + 31) / 32) * 1024LL * g.D : (long long)g.N;
-  const int nb = (int)((K + kListPer - 1) / kListPer);
-  // the 4 * nb block counts (plus a zero that receives the total) scanned as
+  // units of 1024 traversal positions (a z-plane of a 32 x 32 column, or 1024
+  // vertices), eight per block: neighbouring columns of one plane
+  const int ntx = (g.W + 31) / 32, nty = (g.H + 31) / 32, ngx = (ntx + 7) / 8;
+  const int nb = (int)(tiled ? (long long)ntx * nty * g.D : (g.N + 1023) / 1024);
+  const int blocks = (int)(tiled ? (long long)ngx * nty * g.D : (nb + 7LL) / 8);
+  // the 4 * nb unit counts (plus a zero that receives the total) scanned as
   // one array; block sums after them
   const long long n1 = 4LL * nb + 1;
   const int nblk = (int)((n1 + 1023) / 1024);
@@ -458,12 +460,12 @@ void Run::list_compaction(int index) {
   int* cnt = c_->lcnt.p;
   int* sums = cnt + n1;
   k_fill_i32<<<1, 64, 0, st()>>>(cnt + 4LL * nb, 0, 1);
-  k_lists_count<<<nb, 256, 0, st()>>>(s_, cnt, nb, tiled);
+  k_lists_count<<<blocks, 256, 0, st()>>>(s_, cnt, nb, tiled, ntx, ngx);
   k_scan_block<<<nblk, 1024, 0, st()>>>(cnt, cnt, n1, sums);
   k_scan_single<<<1, 1024, 0, st()>>>(sums, nblk);
   k_scan_add<<<nblk, 1024, 0, st()>>>(cnt, n1, sums);
   k_lists_totals<<<1, 64, 0, st()>>>(s_, cnt, nb);
-  k_lists_write<<<nb, 256, 0, st()>>>(s_, cnt, nb, tiled, index);
+  k_lists_write<<<blocks, 256, 0, st()>>>(s_, cnt, nb, tiled, ntx, ngx, index);
 }
 
 void Run::allocate(long long N) {

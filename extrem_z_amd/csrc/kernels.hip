@@ -3408,18 +3408,41 @@ __global__ void __launch_bounds__(256) k_wrong_saddle(St s, int side, int do_fix
 // order-dependent choice compares vertex ids.
 constexpr int kListTile = 32;
 
-__device__ __forceinline__ int list_vertex(const Geo& g, long long k, int tiled) {
-  if (!tiled) return k < g.N ? (int)k : -1;
-  const int ntx = (g.W + kListTile - 1) / kListTile;
-  const long long per_col = (long long)kListTile * kListTile * g.D;
-  const long long col = k / per_col;
-  long long r = k - col * per_col;
-  const int z = (int)(r / (kListTile * kListTile));
-  r -= (long long)z * kListTile * kListTile;
-  const int x = (int)(col % ntx) * kListTile + (int)(r % kListTile);
-  const int y = (int)(col / ntx) * kListTile + (int)(r / kListTile);
-  if (x >= g.W || y >= g.H) return -1;
-  return x + g.W * (y + g.H * z);
+// The traversal in units of 1024 positions, kListTile rows of kListTile: one
+// z-plane of one column (unit col * D + z, col = cy * ntx + cx) in the tiled
+// order, 1024 consecutive vertices otherwise.  A half-wavefront takes one
+// unit, a block of 256 threads eight: in the tiled order eight neighbouring
+// columns of one z-plane (block (cy * ngx + gx) * D + z), so every row the
+// block reads is 256 consecutive voxels of the grid.
+struct UnitLane {
+  long long u;  // the unit (-1: a column past the grid's last)
+  int x0, y0, z;
+};
+__device__ __forceinline__ UnitLane unit_lane(const Geo& g, int tiled, int ntx, int ngx) {
+  const int hw = threadIdx.x >> 5;
+  UnitLane ul{0, 0, 0, 0};
+  if (tiled) {
+    long long b = blockIdx.x;
+    ul.z = (int)(b % g.D);
+    b /= g.D;
+    const int cx = (int)(b % ngx) * 8 + hw, cy = (int)(b / ngx);
+    ul.x0 = cx * kListTile;
+    ul.y0 = cy * kListTile;
+    ul.u = cx < ntx ? ((long long)cy * ntx + cx) * g.D + ul.z : -1;
+  } else {
+    ul.u = blockIdx.x * 8LL + hw;
+  }
+  return ul;
+}
+// The vertex at row r of the unit, or -1 for a position outside the grid.
+__device__ __forceinline__ int unit_vertex(const Geo& g, int tiled, const UnitLane& ul, int r) {
+  const int l = threadIdx.x & 31;
+  if (tiled) {
+    const int x = ul.x0 + l, y = ul.y0 + r;
+    return (ul.u >= 0 && x < g.W && y < g.H) ? x + g.W * (y + g.H * ul.z) : -1;
+  }
+  const long long k = ul.u * (kListTile * kListTile) + r * kListTile + l;
+  return k < g.N ? (int)k : -1;
 }
 
 __device__ __forceinline__ unsigned list_flags(const St& s, int v) {
@@ -3429,29 +3452,24 @@ __device__ __forceinline__ unsigned list_flags(const St& s, int v) {
          (unsigned)(t == 4) << 2 | (unsigned)(t == 0) << 3;
 }
 
-// cnt[c * nb + block] = number of category-c vertices in the block's
-// kListPer traversal positions (categories 2-saddle, 1-saddle, maximum,
-// minimum).
-__global__ void __launch_bounds__(256) k_lists_count(St s, int* cnt, int nb, int tiled) {
-  __shared__ int w[4][4];
-  const long long k0 = blockIdx.x * (long long)kListPer + threadIdx.x;
+// cnt[c * nb + unit] = number of category-c vertices in the unit
+// (categories 2-saddle, 1-saddle, maximum, minimum; nb units).
+__global__ void __launch_bounds__(256) k_lists_count(St s, int* cnt, int nb, int tiled, int ntx,
+                                                     int ngx) {
+  const UnitLane ul = unit_lane(s.g, tiled, ntx, ngx);
   int n[4] = {0, 0, 0, 0};
 #pragma unroll 4
-  for (int j = 0; j < kListPer / 256; j++) {
-    const unsigned fl = list_flags(s, list_vertex(s.g, k0 + 256 * j, tiled));
+  for (int r = 0; r < kListTile; r++) {
+    const unsigned fl = list_flags(s, unit_vertex(s.g, tiled, ul, r));
 #pragma unroll
     for (int c = 0; c < 4; c++) n[c] += (fl >> c) & 1u;
   }
-  const int wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    for (int o = 32; o > 0; o >>= 1) n[c] += __shfl_xor(n[c], o);
-    if (__lane_id() == 0) w[c][wave] = n[c];
-  }
-  __syncthreads();
-  if (threadIdx.x < 4) {
-    const int c = threadIdx.x;
-    cnt[c * nb + blockIdx.x] = w[c][0] + w[c][1] + w[c][2] + w[c][3];
+  for (int c = 0; c < 4; c++)
+    for (int o = 16; o > 0; o >>= 1) n[c] += __shfl_xor(n[c], o);  // within the half
+  if ((threadIdx.x & 31) == 0 && ul.u >= 0 && ul.u < nb) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) cnt[(long long)c * nb + ul.u] = n[c];
   }
 }
 
@@ -3470,37 +3488,28 @@ __global__ void k_lists_totals(St s, const int* cnt, int nb) {
 // 1, -1 elsewhere); the traversal visits each vertex once, so nothing needs
 // clearing first.
 __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb, int tiled,
-                                                     int index) {
-  __shared__ int w[2][4][4];  // per round (alternating): category count per wavefront
-  const int wave = threadIdx.x >> 6, lane = __lane_id();
-  const long long k0 = blockIdx.x * (long long)kListPer + threadIdx.x;
+                                                     int ntx, int ngx, int index) {
+  const UnitLane ul = unit_lane(s.g, tiled, ntx, ngx);
+  const bool act = ul.u >= 0 && ul.u < nb;
+  const int shift = threadIdx.x & 32;  // this half's bits of a ballot
+  const unsigned below = (1u << (threadIdx.x & 31)) - 1u;
   int* lists[4] = {s.S2, s.S1, s.MX, s.MN};
-  int base[4];  // the block's next position per list, the same in every thread
+  int base[4];  // the unit's next position per list, the same in the half's lanes
 #pragma unroll
-  for (int c = 0; c < 4; c++) base[c] = cnt[(long long)c * nb + blockIdx.x] - cnt[(long long)c * nb];
-  for (int j = 0; j < kListPer / 256; j++) {
-    const int v = list_vertex(s.g, k0 + 256 * j, tiled);
+  for (int c = 0; c < 4; c++)
+    base[c] = act ? cnt[(long long)c * nb + ul.u] - cnt[(long long)c * nb] : 0;
+  for (int r = 0; r < kListTile; r++) {
+    const int v = act ? unit_vertex(s.g, tiled, ul, r) : -1;
     const unsigned fl = list_flags(s, v);
-    unsigned long long m[4];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      m[c] = __ballot((fl >> c) & 1u);
-      if (lane == 0) w[j & 1][c][wave] = __popcll(m[c]);
-    }
-    // w alternates between rounds: a buffer is rewritten two rounds later,
-    // when every thread has passed the barrier after its reads
-    __syncthreads();
     int pos[4] = {-1, -1, -1, -1};
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      const int* wc = w[j & 1][c];
+      const unsigned hm = (unsigned)(__ballot((fl >> c) & 1u) >> shift);
       if ((fl >> c) & 1u) {
-        int b = base[c];
-        for (int k = 0; k < wave; k++) b += wc[k];
-        pos[c] = b + __popcll(m[c] & ((1ull << lane) - 1ull));
+        pos[c] = base[c] + __popc(hm & below);
         lists[c][pos[c]] = v;
       }
-      base[c] += wc[0] + wc[1] + wc[2] + wc[3];
+      base[c] += __popc(hm);
     }
     if (index && v >= 0) {
       s.spos2[v] = pos[0];
