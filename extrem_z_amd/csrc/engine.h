@@ -360,10 +360,11 @@ __global__ void k_wrong_index(St s, int side, int phase, int do_fix);
 __global__ void k_wrong_index_one(St s, int side, int do_fix, int wsc);
 __global__ void k_wrong_saddle(St s, int side, int do_fix, int gate);
 __global__ void k_saddle_gate(int* ctr, int gate, int wsc);
-__global__ void k_lists_count(St s, int* cnt, int nb, int tiled, int ntx, int ngx);
+template <bool TILED>
+__global__ void k_lists_count(St s, int* cnt, int nb, int ntx, int ngx);
 __global__ void k_lists_totals(St s, const int* cnt, int nb);
-__global__ void k_lists_write(St s, const int* cnt, int nb, int tiled, int ntx, int ngx,
-                              int index);
+template <bool TILED>
+__global__ void k_lists_write(St s, const int* cnt, int nb, int ntx, int ngx, int index);
 __global__ void k_argext(const double* f, long long n, int* bmin, int* bmax, int* ctr, int check,
                          int f32);
 __global__ void k_count_edited(const uint8_t* cnt, long long n, int* ctr);

@@ -460,12 +460,14 @@ void Run::list_compaction(int index) {
   int* cnt = c_->lcnt.p;
   int* sums = cnt + n1;
   k_fill_i32<<<1, 64, 0, st()>>>(cnt + 4LL * nb, 0, 1);
-  k_lists_count<<<blocks, 256, 0, st()>>>(s_, cnt, nb, tiled, ntx, ngx);
+  auto* count = tiled ? k_lists_count<true> : k_lists_count<false>;
+  auto* write = tiled ? k_lists_write<true> : k_lists_write<false>;
+  count<<<blocks, 256, 0, st()>>>(s_, cnt, nb, ntx, ngx);
   k_scan_block<<<nblk, 1024, 0, st()>>>(cnt, cnt, n1, sums);
   k_scan_single<<<1, 1024, 0, st()>>>(sums, nblk);
   k_scan_add<<<nblk, 1024, 0, st()>>>(cnt, n1, sums);
   k_lists_totals<<<1, 64, 0, st()>>>(s_, cnt, nb);
-  k_lists_write<<<blocks, 256, 0, st()>>>(s_, cnt, nb, tiled, ntx, ngx, index);
+  write<<<blocks, 256, 0, st()>>>(s_, cnt, nb, ntx, ngx, index);
 }
 
 void Run::allocate(long long N) {

@@ -1033,8 +1033,7 @@ template __global__ void k_mask_cols<1, false, 4>(St, const double*, uint8_t*, u
 //              smallest original neighbour)
 //   bit 29     the lower midpoint to the voxel itself
 //   bit 30     the voxel is a frontier entry
-constexpr int kQX = kLTX + 4, kQY = kLTY + 4, kQN = kQX * kQY;
-constexpr int kQPer = (kQN + 255) / 256;
+constexpr int kQX = kLTX + 4;  // decp plane width of k_scan_propose (halo 2)
 
 // Outgoing proposals of one voxel from LDS planes of decp around it (Q0..Q2,
 // centre qc) and its original lower mask / type.
@@ -3415,52 +3414,57 @@ constexpr int kListTile = 32;
 // columns of one z-plane (block (cy * ngx + gx) * D + z), so every row the
 // block reads is 256 consecutive voxels of the grid.
 struct UnitLane {
-  long long u;  // the unit (-1: a column past the grid's last)
+  int u;  // the unit (-1: a column past the grid's last)
   int x0, y0, z;
 };
-__device__ __forceinline__ UnitLane unit_lane(const Geo& g, int tiled, int ntx, int ngx) {
+template <bool TILED>
+__device__ __forceinline__ UnitLane unit_lane(const Geo& g, int ntx, int ngx) {
   const int hw = threadIdx.x >> 5;
   UnitLane ul{0, 0, 0, 0};
-  if (tiled) {
-    long long b = blockIdx.x;
-    ul.z = (int)(b % g.D);
-    b /= g.D;
-    const int cx = (int)(b % ngx) * 8 + hw, cy = (int)(b / ngx);
+  if (TILED) {
+    const int b = blockIdx.x;
+    ul.z = b % g.D;
+    const int q = b / g.D;
+    const int cx = (q % ngx) * 8 + hw, cy = q / ngx;
     ul.x0 = cx * kListTile;
     ul.y0 = cy * kListTile;
-    ul.u = cx < ntx ? ((long long)cy * ntx + cx) * g.D + ul.z : -1;
+    ul.u = cx < ntx ? (cy * ntx + cx) * g.D + ul.z : -1;
   } else {
-    ul.u = blockIdx.x * 8LL + hw;
+    ul.u = blockIdx.x * 8 + hw;
   }
   return ul;
 }
 // The vertex at row r of the unit, or -1 for a position outside the grid.
-__device__ __forceinline__ int unit_vertex(const Geo& g, int tiled, const UnitLane& ul, int r) {
+template <bool TILED>
+__device__ __forceinline__ int unit_vertex(const Geo& g, const UnitLane& ul, int r) {
   const int l = threadIdx.x & 31;
-  if (tiled) {
+  if (TILED) {
     const int x = ul.x0 + l, y = ul.y0 + r;
     return (ul.u >= 0 && x < g.W && y < g.H) ? x + g.W * (y + g.H * ul.z) : -1;
   }
-  const long long k = ul.u * (kListTile * kListTile) + r * kListTile + l;
+  const long long k = (long long)ul.u * (kListTile * kListTile) + r * kListTile + l;
   return k < g.N ? (int)k : -1;
 }
-
-__device__ __forceinline__ unsigned list_flags(const St& s, int v) {
-  if (v < 0) return 0u;
-  int t = s.otype[v];
-  return (unsigned)(t == 2 || t == 3) | (unsigned)(t == 1 || t == 3) << 1 |
-         (unsigned)(t == 4) << 2 | (unsigned)(t == 0) << 3;
+// The original type at v, loaded for every lane (at vertex 0 for v < 0), so
+// the loads of successive rows need no branch and stay in flight together.
+__device__ __forceinline__ int unit_type(const St& s, int v) { return s.otype[v >= 0 ? v : 0]; }
+// List flags of a vertex by its original type t (0 for v < 0).
+__device__ __forceinline__ unsigned unit_flags(int v, int t) {
+  const unsigned f = (unsigned)(t == 2 || t == 3) | (unsigned)(t == 1 || t == 3) << 1 |
+                     (unsigned)(t == 4) << 2 | (unsigned)(t == 0) << 3;
+  return v >= 0 ? f : 0u;
 }
 
 // cnt[c * nb + unit] = number of category-c vertices in the unit
 // (categories 2-saddle, 1-saddle, maximum, minimum; nb units).
-__global__ void __launch_bounds__(256) k_lists_count(St s, int* cnt, int nb, int tiled, int ntx,
-                                                     int ngx) {
-  const UnitLane ul = unit_lane(s.g, tiled, ntx, ngx);
+template <bool TILED>
+__global__ void __launch_bounds__(256) k_lists_count(St s, int* cnt, int nb, int ntx, int ngx) {
+  const UnitLane ul = unit_lane<TILED>(s.g, ntx, ngx);
   int n[4] = {0, 0, 0, 0};
-#pragma unroll 4
+#pragma unroll 8
   for (int r = 0; r < kListTile; r++) {
-    const unsigned fl = list_flags(s, unit_vertex(s.g, tiled, ul, r));
+    const int v = unit_vertex<TILED>(s.g, ul, r);
+    const unsigned fl = unit_flags(v, unit_type(s, v));
 #pragma unroll
     for (int c = 0; c < 4; c++) n[c] += (fl >> c) & 1u;
   }
@@ -3472,6 +3476,8 @@ __global__ void __launch_bounds__(256) k_lists_count(St s, int* cnt, int nb, int
     for (int c = 0; c < 4; c++) cnt[(long long)c * nb + ul.u] = n[c];
   }
 }
+template __global__ void k_lists_count<true>(St, int*, int, int, int);
+template __global__ void k_lists_count<false>(St, int*, int, int, int);
 
 // List lengths from the exclusive scan of the 4 * nb block counts taken as
 // one array (entry 4 * nb holds the grand total): category c's blocks start
@@ -3487,9 +3493,10 @@ __global__ void k_lists_totals(St s, const int* cnt, int nb) {
 // off the saddle lists) and ext_id (2 * position in MX, 2 * position in MN +
 // 1, -1 elsewhere); the traversal visits each vertex once, so nothing needs
 // clearing first.
-__global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb, int tiled,
-                                                     int ntx, int ngx, int index) {
-  const UnitLane ul = unit_lane(s.g, tiled, ntx, ngx);
+template <bool TILED>
+__global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int nb, int ntx, int ngx,
+                                                     int index) {
+  const UnitLane ul = unit_lane<TILED>(s.g, ntx, ngx);
   const bool act = ul.u >= 0 && ul.u < nb;
   const int shift = threadIdx.x & 32;  // this half's bits of a ballot
   const unsigned below = (1u << (threadIdx.x & 31)) - 1u;
@@ -3498,9 +3505,15 @@ __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int n
 #pragma unroll
   for (int c = 0; c < 4; c++)
     base[c] = act ? cnt[(long long)c * nb + ul.u] - cnt[(long long)c * nb] : 0;
+  // every row's type is loaded before the first store: a wait for a load
+  // also waits for the stores issued before it
+  int tv[kListTile];
+#pragma unroll
+  for (int r = 0; r < kListTile; r++) tv[r] = unit_type(s, act ? unit_vertex<TILED>(s.g, ul, r) : -1);
+#pragma unroll
   for (int r = 0; r < kListTile; r++) {
-    const int v = act ? unit_vertex(s.g, tiled, ul, r) : -1;
-    const unsigned fl = list_flags(s, v);
+    const int v = act ? unit_vertex<TILED>(s.g, ul, r) : -1;
+    const unsigned fl = unit_flags(v, tv[r]);
     int pos[4] = {-1, -1, -1, -1};
 #pragma unroll
     for (int c = 0; c < 4; c++) {
@@ -3518,6 +3531,8 @@ __global__ void __launch_bounds__(256) k_lists_write(St s, const int* cnt, int n
     }
   }
 }
+template __global__ void k_lists_write<true>(St, const int*, int, int, int, int);
+template __global__ void k_lists_write<false>(St, const int*, int, int, int, int);
 
 // Global minimum and maximum of f by (value, id) (main.cpp:67-75), first
 // stage: each block's best into bmin / bmax[blockIdx.x]; k_ext_reduce over
