@@ -1746,14 +1746,31 @@ template __global__ void k_saddle_list<kModeDec>(St, const double*, const int*, 
 // written at that base plus its rank.  A pair whose probe finds no room takes
 // a global atomic of its own.
 constexpr int kFillN = 1024;
+// The pairs of the block's next chunk are loaded while the current one is
+// placed (before its atomics and stores: a wait for a load also waits for the
+// stores issued before it), and kept in registers through the placement.
 __global__ void __launch_bounds__(256) k_csr_fill(St s, const int2* pairs, const int* pcnt,
                                                   int nb) {
   __shared__ int2 hc[kFillN];  // (slot, count), then (slot, base)
   constexpr int kPer = kChunkPairs / 256;
   const int tid = threadIdx.x;
+  int2 nxt[kPer];
+  int nn = 0;
+  auto load = [&](int cc) {  // block-uniform
+    nn = cc < nb ? pcnt[cc] : 0;
+    const int2* P = pairs + (long long)cc * kChunkPairs;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int j = tid + 256 * k;
+      nxt[k] = j < nn ? P[j] : make_int2(-1, 0);
+    }
+  };
+  load(blockIdx.x);
   for (int c = blockIdx.x; c < nb; c += gridDim.x) {  // block-uniform
-    const int n = pcnt[c];
-    const int2* P = pairs + (long long)c * kChunkPairs;
+    const int n = nn;
+    int2 pr[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) pr[k] = nxt[k];
     for (int j = tid; j < kFillN; j += 256) hc[j] = make_int2(-1, 0);
     __syncthreads();
     int hs[kPer], rk[kPer];
@@ -1763,7 +1780,7 @@ __global__ void __launch_bounds__(256) k_csr_fill(St s, const int2* pairs, const
       hs[k] = -1;
       rk[k] = 0;
       if (j >= n) continue;
-      const int slot = P[j].x;
+      const int slot = pr[k].x;
       unsigned h = ((unsigned)slot * 2654435761u) >> 22;  // kFillN = 2^10
       for (int p = 0; p < 32; p++) {
         const int old = atomicCAS(&hc[h].x, -1, slot);
@@ -1776,6 +1793,7 @@ __global__ void __launch_bounds__(256) k_csr_fill(St s, const int2* pairs, const
       }
     }
     __syncthreads();
+    load(c + gridDim.x);
     for (int j = tid; j < kFillN; j += 256) {
       const int2 e = hc[j];
       if (e.x >= 0) hc[j].y = atomicAdd(&s.csr_cur[e.x], e.y);
@@ -1785,9 +1803,8 @@ __global__ void __launch_bounds__(256) k_csr_fill(St s, const int2* pairs, const
     for (int k = 0; k < kPer; k++) {
       const int j = tid + 256 * k;
       if (j >= n) continue;
-      const int2 pr = P[j];
-      const int at = hs[k] >= 0 ? hc[hs[k]].y + rk[k] : atomicAdd(&s.csr_cur[pr.x], 1);
-      s.csr[at] = pr.y;
+      const int at = hs[k] >= 0 ? hc[hs[k]].y + rk[k] : atomicAdd(&s.csr_cur[pr[k].x], 1);
+      s.csr[at] = pr[k].y;
     }
     __syncthreads();
   }
@@ -1805,42 +1822,45 @@ constexpr int kPullShort = 256;
 constexpr int kPullGroup = 4;
 constexpr int kPullChunk = 1024;
 
-__device__ __forceinline__ bool pull_better(bool mx, long long ka, int pa, long long kb, int pb) {
-  return mx ? (ka > kb || (ka == kb && pa > pb)) : (ka < kb || (ka == kb && pa < pb));
+// The candidates are list positions (of S2 for a maximum's slot, S1 for a
+// minimum's): (key, saddle id) order with the ids looked up only where the
+// keys are equal; the winner's id is looked up once, when it is stored.
+__device__ __forceinline__ bool pull_better(const St& s, bool mx, long long ka, int pa,
+                                            long long kb, int pb) {
+  if (ka != kb) return mx ? ka > kb : ka < kb;
+  const int* S = mx ? s.S2 : s.S1;
+  const int ia = S[pa], ib = S[pb];
+  return mx ? ia > ib : ia < ib;
 }
 
 __device__ __forceinline__ void pull_range(const St& s, bool mx, int k0, int k1, int step,
                                            long long& bk, int& bp) {
   const long long* pv = mx ? s.pv2 : s.pv1;
-  const int* S = mx ? s.S2 : s.S1;
   int k = k0;
   for (; k + 3 * step < k1; k += 4 * step) {
-    int p[4], id[4];
+    int p[4];
     long long q[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) p[u] = s.csr[k + u * step];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      q[u] = pv[p[u]];
-      id[u] = S[p[u]];
-    }
+    for (int u = 0; u < 4; u++) q[u] = pv[p[u]];
 #pragma unroll
     for (int u = 0; u < 4; u++)
-      if (bp < 0 || pull_better(mx, q[u], id[u], bk, bp)) { bk = q[u]; bp = id[u]; }
+      if (bp < 0 || pull_better(s, mx, q[u], p[u], bk, bp)) { bk = q[u]; bp = p[u]; }
   }
   for (; k < k1; k += step) {
     const int p = s.csr[k];
     const long long q = pv[p];
-    const int id = S[p];
-    if (bp < 0 || pull_better(mx, q, id, bk, bp)) { bk = q; bp = id; }
+    if (bp < 0 || pull_better(s, mx, q, p, bk, bp)) { bk = q; bp = p; }
   }
 }
 
-__device__ __forceinline__ void pull_group(bool mx, int width, long long& bk, int& bp) {
+__device__ __forceinline__ void pull_group(const St& s, bool mx, int width, long long& bk,
+                                           int& bp) {
   for (int o = 1; o < width; o <<= 1) {
     const long long ok = __shfl_xor(bk, o);
     const int op = __shfl_xor(bp, o);
-    if (op >= 0 && (bp < 0 || pull_better(mx, ok, op, bk, bp))) { bk = ok; bp = op; }
+    if (op >= 0 && (bp < 0 || pull_better(s, mx, ok, op, bk, bp))) { bk = ok; bp = op; }
   }
 }
 
@@ -1849,7 +1869,7 @@ __device__ __forceinline__ void pull_store(const St& s, int j, bool mx, long lon
   const long long empty = mx ? (long long)0x8000000000000000ULL : 0x7fffffffffffffffLL;
   const bool none = bp < 0 || bk == empty;
   pkey[j] = none ? empty : bk;
-  pid[j] = none ? kNone : bp;
+  pid[j] = none ? kNone : (mx ? s.S2 : s.S1)[bp];
 }
 
 __global__ void __launch_bounds__(256) k_slot_pull(St s, long long* pkey, int* pid) {
@@ -1868,7 +1888,7 @@ __global__ void __launch_bounds__(256) k_slot_pull(St s, long long* pkey, int* p
     long long bk = 0;
     int bp = -1;
     if (mine) pull_range(s, mx, k0 + gl, k1, kPullGroup, bk, bp);
-    pull_group(mx, kPullGroup, bk, bp);
+    pull_group(s, mx, kPullGroup, bk, bp);
     if (j < n && mine && gl == 0) pull_store(s, j, mx, bk, bp, pkey, pid);
   }
 }
@@ -1884,7 +1904,7 @@ __global__ void __launch_bounds__(256) k_slot_pull_long(St s, long long* ckey, i
     long long bk = 0;
     int bp = -1;
     pull_range(s, mx, ch.y + threadIdx.x, ch.z, 256, bk, bp);
-    pull_group(mx, 64, bk, bp);
+    pull_group(s, mx, 64, bk, bp);
     if (__lane_id() == 0) {
       wk[threadIdx.x >> 6] = bk;
       wp[threadIdx.x >> 6] = bp;
@@ -1892,7 +1912,7 @@ __global__ void __launch_bounds__(256) k_slot_pull_long(St s, long long* ckey, i
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int w = 1; w < 4; w++)
-        if (wp[w] >= 0 && (bp < 0 || pull_better(mx, wk[w], wp[w], bk, bp))) {
+        if (wp[w] >= 0 && (bp < 0 || pull_better(s, mx, wk[w], wp[w], bk, bp))) {
           bk = wk[w];
           bp = wp[w];
         }
@@ -1914,7 +1934,7 @@ __global__ void k_slot_pull_join(St s, const long long* ckey, const int* cpos, l
     long long bk = 0;
     int bp = -1;
     for (int c = l.y; c < l.y + nch; c++)
-      if (cpos[c] >= 0 && (bp < 0 || pull_better(mx, ckey[c], cpos[c], bk, bp))) {
+      if (cpos[c] >= 0 && (bp < 0 || pull_better(s, mx, ckey[c], cpos[c], bk, bp))) {
         bk = ckey[c];
         bp = cpos[c];
       }
@@ -2706,7 +2726,7 @@ __device__ __forceinline__ void inc_pull_body(const St& s, long long* pkey, int*
     long long bk = 0;
     int bp = -1;
     if (mine) pull_range(s, mx, k0 + gl, k1, kPullGroup, bk, bp);
-    pull_group(mx, kPullGroup, bk, bp);
+    pull_group(s, mx, kPullGroup, bk, bp);
     if (mine && gl == 0) pull_store(s, j, mx, bk, bp, pkey, pid);
     __syncthreads();
     const int nlong = nl;
@@ -2716,7 +2736,7 @@ __device__ __forceinline__ void inc_pull_body(const St& s, long long* pkey, int*
       bk = 0;
       bp = -1;
       pull_range(s, mx, s.csr_off[jj] + lane, s.csr_off[jj + 1], 64, bk, bp);
-      pull_group(mx, 64, bk, bp);
+      pull_group(s, mx, 64, bk, bp);
       if (lane == 0) pull_store(s, jj, mx, bk, bp, pkey, pid);
     }
     __syncthreads();
