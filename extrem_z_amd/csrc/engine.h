@@ -281,7 +281,7 @@ __global__ void k_check_finite(const double* a, long long n, int* ctr, int f32);
 __global__ void k_classify(Geo g, const uint8_t* lut, const double* f, uint8_t* type,
                            uint16_t* lower);
 // rows per k_scan_propose block (64 x kScanTY threads)
-constexpr int kScanTY = 16;
+constexpr int kScanTY = 8;
 // TY rows per block: 4 (64 x 4 threads)
 template <int MODE, bool FAST, int TY>
 __global__ void k_mask_cols(St s, const double* f, uint8_t* type, uint16_t* lower, int* front,

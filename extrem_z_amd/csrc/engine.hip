@@ -327,10 +327,11 @@ class Run {
   static constexpr int kSaddleBlocks = 4096;
   // Blocks of a column sweep (one frontier append each).
   static constexpr int kColBlocks = 4096;
-  // k_scan_propose: 64 x kScanTY columns (one 1024-thread block per CU at its
-  // register use), about kScanBlocks blocks.  Rows of 4 / 8 / 16: 2.97 /
-  // 2.70 / 2.53 ms per 512^3 scan (fewer halo words per owned voxel)
-  static constexpr int kScanBlocks = 512;
+  // k_scan_propose: 64 x kScanTY columns, about kScanBlocks blocks.  Rows of
+  // 4 / 8 / 16: 2.97 / 2.70 / 2.53 ms per 512^3 scan (fewer halo words per
+  // owned voxel), but a 1024-thread block needs a whole CU free: beside other
+  // contexts' kernels (a serving pool) rows of 16 wait for one
+  static constexpr int kScanBlocks = 1024;
   // Blocks of the incremental saddle launches (sizes known on the device only).
   static constexpr int kIncSaddleBlocks = 2048;
   // Wrong-neighbour reservation rounds launched per read of the list length.
