@@ -927,7 +927,6 @@ bool Run::s_loops(int side, int do_fix, int known, bool gate) {
     tbegin(EXTREMZ_K_PROPOSE);
     k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 1, do_fix);
     tend();
-    k_wrong_index<<<b, 256, 0, st()>>>(s_, side, 2, 0);
   }
   if (do_fix) apply();
   return gated;

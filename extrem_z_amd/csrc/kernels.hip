@@ -3233,7 +3233,9 @@ __global__ void __launch_bounds__(256) k_fixpath(St s, int fast) {
 // get_wrong_index_max / get_wrong_index_min (src/ExtreM.cpp:2595-2670): the
 // first saddle (lowest id) per original extremum wins; pairs are compared by
 // vertex id.  phase 0 registers, phase 1 counts (and proposes, s_loops /
-// s_loops_join, src/ExtreM.cpp:2838-2886), phase 2 releases.
+// s_loops_join, src/ExtreM.cpp:2838-2886) and releases: a registration word
+// holds its winner's id until the winner itself resets it, and any other
+// value a loser reads (that id or the reset one) tells it the same.
 // The registration words (win_a / win_b) are one per listed extremum, at its
 // slot: an original triplet end is an extremum of the original field, so it
 // has a list position.
@@ -3267,8 +3269,9 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
     if (phase == 0) {
       if (mf) atomicMin(&s.win_a[end_slot(s, a)], sv);
       if (ml) atomicMin(&s.win_b[end_slot(s, b)], sv);
-    } else if (phase == 1) {
+    } else {
       if (mf && s.win_a[end_slot(s, a)] == sv) {
+        s.win_a[end_slot(s, a)] = 0x7fffffff;
         cf.inc();
         if (side == 2) {  // triplet[0] -> wrong_max_counter, fix dir 0 lowers orig
           if (do_fix) propose(s, a, lower_mid(s, a), tl);
@@ -3277,6 +3280,7 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
         }
       }
       if (ml && s.win_b[end_slot(s, b)] == sv) {
+        s.win_b[end_slot(s, b)] = 0x7fffffff;
         cl.inc();
         if (side == 2) {  // triplet[last] -> wrong_max_counter_2, fix dir 1 lowers decp's
           if (do_fix && d != kNone) propose(s, d, lower_mid(s, d), tl);
@@ -3284,9 +3288,6 @@ __global__ void __launch_bounds__(256) k_wrong_index(St s, int side, int phase, 
           if (do_fix) propose(s, b, lower_mid(s, b), tl);
         }
       }
-    } else {
-      if (mf) s.win_a[end_slot(s, a)] = 0x7fffffff;
-      if (ml) s.win_b[end_slot(s, b)] = 0x7fffffff;
     }
   }, tl);
   tl.flush();
