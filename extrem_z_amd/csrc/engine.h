@@ -351,8 +351,7 @@ __global__ void k_inc_one(St s, int epoch, long long* kmax, long long* kmin, int
 __global__ void k_diff_count(const void* a, const void* b, long long nbytes, int* out);
 __global__ void k_act_copy(St s, int2* dst, int dctr);
 __global__ void k_wn_register(St s, int2* active, int actr, int nctr);
-__global__ void k_wn_commit(St s, int2* active, int actr);
-__global__ void k_wn_next(St s, const int2* active, int actr, int2* next, int nctr);
+__global__ void k_wn_commit(St s, const int2* active, int actr, int2* next, int nctr);
 __global__ void k_wn_reset(St s);
 __global__ void k_wn_solve(St s, int2* bufA, int2* bufB, int limit);
 __global__ void k_fixpath(St s, int fast);

@@ -880,8 +880,7 @@ void Run::wrong_neighbors() {
     tbegin(EXTREMZ_K_RESERVE);
     for (int r = 0; r < kWnRounds; r++, round++) {
       k_wn_register<<<b, 256, 0, st()>>>(s_, act, actr, nctr);
-      k_wn_commit<<<b, 256, 0, st()>>>(s_, act, actr);
-      k_wn_next<<<b, 256, 0, st()>>>(s_, act, actr, nxt, nctr);
+      k_wn_commit<<<b, 256, 0, st()>>>(s_, act, actr, nxt, nctr);
       std::swap(act, nxt);
       std::swap(actr, nctr);
     }

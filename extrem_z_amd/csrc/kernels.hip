@@ -2856,7 +2856,7 @@ __global__ void k_act_copy(St s, int2* dst, int dctr) {
 // Drop flagged candidates; every remaining candidate records the smallest
 // saddle id that may still take it.
 __global__ void __launch_bounds__(256) k_wn_register(St s, int2* active, int actr, int nctr) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr[nctr] = 0;  // the list k_wn_next fills
+  if (blockIdx.x == 0 && threadIdx.x == 0) s.ctr[nctr] = 0;  // the list k_wn_commit fills
   int n = s.ctr[actr];
   GRID_STRIDE(a, n) {
     int2 e = active[a];
@@ -2878,18 +2878,35 @@ __global__ void __launch_bounds__(256) k_wn_register(St s, int2* active, int act
 }
 
 // A saddle whose first candidate is reserved by itself takes it (flag +
-// append, as get_wrong_split_neighbors does before its return).
-__global__ void __launch_bounds__(256) k_wn_commit(St s, int2* active, int actr) {
+// append, as get_wrong_split_neighbors does before its return); the others
+// go to the next round's list.  Each saddle also releases the reservations it
+// holds (those equal to its id): a reservation is changed only by its holder,
+// and any other value a saddle reads there (the holder's id, or the released
+// one) tells it the same, that it does not hold it.
+__global__ void __launch_bounds__(256) k_wn_commit(St s, const int2* active, int actr,
+                                                   int2* next, int nctr) {
   STAGED_LIST(l2, int, kStage, s.wnl2, &s.ctr[C_NFC]);
   STAGED_LIST(l1, int, kStage, s.wnl1, &s.ctr[C_NFC1]);
+  STAGED_LIST(nl, int2, kStage, next, &s.ctr[nctr]);
   int n = s.ctr[actr];
   GRID_STRIDE(a, n) {
-    int2 e = active[a];
-    unsigned mask = (unsigned)e.y;
+    const int2 e = active[a];
+    const unsigned mask = (unsigned)e.y;
     if (!mask) continue;
-    int sv = e.x >> 1, side = e.x & 1;
-    int r = sv + s.g.off[__ffs(mask) - 1];
-    if ((side ? s.minsad1 : s.minsad2)[r] == sv) {
+    const int sv = e.x >> 1, side = e.x & 1;
+    int* ms = side ? s.minsad1 : s.minsad2;
+    const int first = __ffs(mask) - 1;
+    bool took = false;
+    for (int d = first; d < kNbr; d++) {
+      if (!((mask >> d) & 1)) continue;
+      const int r = sv + s.g.off[d];
+      if (ms[r] == sv) {
+        ms[r] = 0x7fffffff;
+        took |= d == first;
+      }
+    }
+    if (took) {
+      const int r = sv + s.g.off[first];
       if (!side) {
         s.wn2[r] = 1;
         l2.push(r);
@@ -2897,27 +2914,12 @@ __global__ void __launch_bounds__(256) k_wn_commit(St s, int2* active, int actr)
         s.wn1[r] = 1;
         l1.push(r);
       }
-      active[a].y = (int)(mask | 0x8000u);  // done
+    } else {
+      nl.push(e);
     }
   }
   l2.flush();
   l1.flush();
-}
-
-// Releases this round's reservations and keeps unfinished saddles.
-__global__ void __launch_bounds__(256) k_wn_next(St s, const int2* active, int actr, int2* next,
-                                                 int nctr) {
-  STAGED_LIST(nl, int2, kStage, next, &s.ctr[nctr]);
-  int n = s.ctr[actr];
-  GRID_STRIDE(a, n) {
-    int2 e = active[a];
-    unsigned mask = (unsigned)e.y;
-    int sv = e.x >> 1, side = e.x & 1;
-    int* ms = side ? s.minsad1 : s.minsad2;
-    for (int d = 0; d < kNbr; d++)
-      if ((mask >> d) & 1) ms[sv + s.g.off[d]] = 0x7fffffff;
-    if (mask && !(mask & 0x8000u)) nl.push(e);
-  }
   nl.flush();
 }
 
@@ -3132,7 +3134,7 @@ __global__ void __launch_bounds__(1024) k_wn_solve(St s, int2* bufA, int2* bufB,
       }
     }
     __syncthreads();
-    // release and compaction (k_wn_next)
+    // release and compaction
     for (int a = t; a < n; a += 1024) {
       const int ex = va[a].x;
       const unsigned mask = (unsigned)va[a].y;
